@@ -1,0 +1,123 @@
+/*
+ * rtx - MI355X-native DG transport operator and Krylov kernels (C ABI).
+ *
+ * The reference (jasonltorchinsky/radiative-transfer-amr) is pure Python: its
+ * "boundary" for this path is the method surface of rt.Problem
+ * (src/rt/class_Problem.py:11-32) which assembles PETSc matrices and calls
+ * PETSc KSP (src/rt/solve.py:81-117).  Each entry point below replaces one
+ * piece of that path; the Python host package (rt.Problem, rtx.*) binds them
+ * with ctypes.  All pointers named d_* are device pointers; everything else
+ * is host memory that is copied during the call.  Work is enqueued on the
+ * caller's stream (a hipStream_t passed as void*).  Return value: 0 on
+ * success, a negative rtx_status on failure, with a message available from
+ * rtx_last_error() (thread-local).
+ */
+#ifndef RTX_H
+#define RTX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    RTX_OK = 0,
+    RTX_ERR_ARG = -1,      /* bad argument / size out of range          */
+    RTX_ERR_HIP = -2,      /* a HIP runtime call failed                  */
+    RTX_ERR_STATE = -3,    /* call out of order (e.g. no coefficients)   */
+    RTX_ERR_NUMERIC = -4   /* singular block, non-finite values          */
+} rtx_status;
+
+/* Operator parts for rtx_apply (bit mask). */
+enum {
+    RTX_OP_MASS = 1u,      /* M      : src/rt/mass_matrix.py:140-173                 */
+    RTX_OP_SCAT = 2u,      /* -S     : src/rt/scattering_matrix.py:142-237 (returns -S) */
+    RTX_OP_ICONV = 4u,     /* -IC    : src/rt/interior_convection_matrix.py:161-243  */
+    RTX_OP_BCONV = 8u,     /* +BC    : src/rt/boundary_convection_matrix.py:258-537  */
+    RTX_OP_RESTRICT = 16u, /* zero x and y on inflow-boundary DOFs: P_I A P_I^T,
+                              src/dg/matrix/get_masks.py:80-110, split_matrix.py:46 */
+    RTX_OP_ACCUM = 32u     /* y += op(x) instead of y = op(x)                      */
+};
+#define RTX_OP_SYSTEM (RTX_OP_MASS | RTX_OP_SCAT | RTX_OP_ICONV | RTX_OP_BCONV)
+
+/* Arithmetic for rtx_apply. */
+enum {
+    RTX_PREC_F64 = 0,      /* everything in FP64                                    */
+    RTX_PREC_F32 = 1,      /* x, y FP32; FP32 arithmetic                            */
+    RTX_PREC_MIXED = 2     /* x, y FP32; convection FP32, scattering BF16 on MFMA
+                              with FP32 accumulation                                */
+};
+
+/* Largest supported number of nodes per direction in a cell. */
+#define RTX_NMAX 10
+
+/*
+ * Flattened hp mesh.  Columns are in ascending key order and cells in
+ * ascending key order within a column, so the DOF order is the reference's
+ * (src/dg/matrix/get_idxs.py:11-80, src/rt/mass_matrix.py:54-74).
+ * Inflow face couplings ("links") are those of
+ * src/rt/boundary_convection_matrix.py:348-524, one per (cell, inflow face,
+ * neighbour cell), sorted by target cell.
+ */
+typedef struct {
+    int32_t ncol, ncell, nlink;
+    int64_t ndof;
+    /* columns */
+    const int32_t *col_cell_start;  /* ncol + 1                                   */
+    const int32_t *col_nx, *col_ny; /* LGL orders                                 */
+    const int32_t *col_sp_off;      /* first spatial node (kappa/sigma arrays)     */
+    const int32_t *col_nang;        /* angular nodes of the column (sum of nth)   */
+    const int64_t *col_G_off;       /* offset of the column's G^T in the G pool    */
+    const double *col_geom;         /* 4 per column: x0, y0, x1, y1                */
+    /* cells */
+    const int32_t *cell_dof_off;    /* first DOF of the cell                       */
+    const int32_t *cell_col;        /* owning column                               */
+    const int32_t *cell_nth;        /* Gauss orders in theta                       */
+    const int32_t *cell_quad;       /* quadrant 0..3, -1 when not in one           */
+    const int32_t *cell_ang_off;    /* first angular index within the column       */
+    const int32_t *cell_bdry;       /* bit F: face F is a domain face whose
+                                       directions flow in (masked DOFs)            */
+    const double *cell_geom;        /* 2 per cell: th0, th1                        */
+    /* inflow links */
+    const int32_t *link_start;      /* ncell + 1                                   */
+    const int32_t *link_src;        /* source (upwind) cell                        */
+    const int32_t *link_face;       /* face F of the target cell                   */
+    const int32_t *link_Es_off;     /* E^x or E^y (n1 x n0, row-major) in Es pool  */
+    const int32_t *link_Eth_off;    /* E^theta (nth1 x nth0, row-major)            */
+    int64_t nEs, nEth;
+    const double *Es_pool, *Eth_pool;
+    /* 1-D rules: for n = 1..RTX_NMAX, index (n * RTX_NMAX + k):
+       lgl_x, lgl_w, lg_x, lg_w; and lgl_D[(n * RTX_NMAX + q) * RTX_NMAX + j]
+       = l_q'(xi_j) */
+    const double *lgl_x, *lgl_w, *lg_x, *lg_w, *lgl_D;
+} rtx_mesh_desc;
+
+typedef struct rtx_plan rtx_plan;
+
+const char *rtx_last_error(void);
+
+/* Copy the flattened mesh to the device (replaces the per-call assembly set-up
+   of every src/rt/*_matrix.py).  Uses the current HIP device. */
+int rtx_plan_create(const rtx_mesh_desc *desc, rtx_plan **out);
+void rtx_plan_destroy(rtx_plan *plan);
+
+/* Nodal coefficients (host arrays).  kappa, sigma: one value per spatial node
+   in column order (node (i, j) of column c at col_sp_off[c] + i * ny + j),
+   the callables of Problem (src/rt/class_Problem.py:12-23) evaluated there.
+   G_pool: per column angular mesh, G^T[A, R] = (dth_K / 8)(dth_K' / 2)
+   w_r w_a Phi(th_R, th_A) with R in cell K, A in cell K'
+   (src/rt/scattering_matrix.py:160-205). */
+int rtx_plan_set_coeffs(rtx_plan *plan, int64_t nsp, const double *kappa,
+                        const double *sigma, int64_t nG, const double *G_pool);
+
+/* y = op(x) (or y += op(x) with RTX_OP_ACCUM) for the operator parts in
+   op_mask: the matrix-free counterpart of PETSc MatMult with the assembled
+   (BC - IC) + (M + S_ret) of src/rt/solve.py:52. */
+int rtx_apply(rtx_plan *plan, uint32_t op_mask, int precision, const void *d_x,
+              void *d_y, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTX_H */

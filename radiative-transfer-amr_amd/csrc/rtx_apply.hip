@@ -1,0 +1,266 @@
+// Matrix-free DG transport operator: generic hp kernels.
+//
+//   k_transport : one wavefront per angular cell.  Mass (M), interior
+//                 convection (-IC), outflow face terms and upwind inflow
+//                 couplings through the link table (+BC).  The cell's nodal
+//                 values are staged in LDS.
+//   k_scatter   : one workgroup per spatial column.  Adds the angular
+//                 scattering contraction (-S) per spatial node with the
+//                 column's G^T table, the angular values of a tile of
+//                 spatial nodes staged in LDS.
+//
+// Formulas (matrix entries of the reference restated as actions):
+//   M      : src/rt/mass_matrix.py:140-173
+//   IC     : src/rt/interior_convection_matrix.py:161-243
+//   BC     : src/rt/boundary_convection_matrix.py:258-537
+//   -S     : src/rt/scattering_matrix.py:142-237
+#include "rtx_internal.h"
+
+namespace rtx {
+
+constexpr int NM = RTX_NMAX;
+constexpr int SCAT_THREADS = 256;
+constexpr int SCAT_LDS_DOUBLES = 5120;   // 40 KiB of staged angular values
+constexpr int MAX_NANG = 1024;           // angular nodes per column
+
+__device__ inline bool on_bdry(int bd, int i, int j, int nx, int ny) {
+    return ((bd & 1) && i == nx - 1) || ((bd & 2) && j == ny - 1) || ((bd & 4) && i == 0) ||
+           ((bd & 8) && j == 0);
+}
+
+// Theta_F(th) = cos(th - F pi / 2) from the cos/sin of th.
+__device__ inline double theta_f(int F, double c, double s) {
+    return F == 0 ? c : (F == 1 ? s : (F == 2 ? -c : -s));
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int accumulate,
+                                                  const T *__restrict__ x, T *__restrict__ y) {
+    __shared__ double xs[NM * NM * NM];
+    __shared__ double ys[NM * NM * NM];
+    __shared__ double tr[NM * NM];
+    __shared__ double cth[NM], sth[NM];
+    const int k = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int c = m.cell_col[k];
+    const int nx = m.col_nx[c], ny = m.col_ny[c], nth = m.cell_nth[k];
+    const int n = nx * ny * nth;
+    const int off = m.cell_dof_off[k];
+    const bool restrict_ = (ops & RTX_OP_RESTRICT) != 0;
+    const int bd = restrict_ ? m.cell_bdry[k] : 0;
+    const double x0 = m.col_geom[4 * c + 0], y0 = m.col_geom[4 * c + 1];
+    const double dx = m.col_geom[4 * c + 2] - x0, dy = m.col_geom[4 * c + 3] - y0;
+    const double th0 = m.cell_geom[2 * k + 0], dth = m.cell_geom[2 * k + 1] - th0;
+    const double *wx = m.lgl_w + nx * NM, *wy = m.lgl_w + ny * NM, *wt = m.lg_w + nth * NM;
+    const int nyt = ny * nth;
+
+    for (int t = lane; t < n; t += 64) {
+        const int i = t / nyt, j = (t / nth) % ny;
+        double v = double(x[off + t]);
+        if (bd && on_bdry(bd, i, j, nx, ny)) v = 0.0;
+        xs[t] = v;
+    }
+    if (lane < nth) {
+        const double th = th0 + dth * 0.5 * (m.lg_x[nth * NM + lane] + 1.0);
+        double s, cc;
+        sincos(th, &s, &cc);
+        cth[lane] = cc;
+        sth[lane] = s;
+    }
+    __syncthreads();
+
+    const double *Dx = m.lgl_D + size_t(nx) * NM * NM;
+    const double *Dy = m.lgl_D + size_t(ny) * NM * NM;
+    const int quad = m.cell_quad[k];
+    const int sp0 = m.col_sp_off[c];
+    for (int t = lane; t < n; t += 64) {
+        const int p = t / nyt, q = (t / nth) % ny, r = t % nth;
+        double acc = 0.0;
+        if (ops & RTX_OP_MASS)
+            acc += (dx * dy * dth * 0.125) * wx[p] * wy[q] * wt[r] * m.kappa[sp0 + p * ny + q] *
+                   xs[t];
+        if (ops & RTX_OP_ICONV) {
+            double sx = 0.0, sy = 0.0;
+            for (int i = 0; i < nx; ++i) sx += wx[i] * Dx[p * NM + i] * xs[i * nyt + q * nth + r];
+            for (int j = 0; j < ny; ++j) sy += wy[j] * Dy[q * NM + j] * xs[p * nyt + j * nth + r];
+            acc -= (dy * dth * 0.25) * wy[q] * wt[r] * cth[r] * sx +
+                   (dx * dth * 0.25) * wx[p] * wt[r] * sth[r] * sy;
+        }
+        if ((ops & RTX_OP_BCONV) && quad >= 0) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int F = (quad + h) & 3;
+                bool on = (F == 0 && p == nx - 1) || (F == 2 && p == 0) ||
+                          (F == 1 && q == ny - 1) || (F == 3 && q == 0);
+                if (on) {
+                    const double cf = ((F & 1) ? dx : dy) * dth * 0.25;
+                    const double wm = (F & 1) ? wx[p] : wy[q];
+                    acc += cf * wm * wt[r] * theta_f(F, cth[r], sth[r]) * xs[t];
+                }
+            }
+        }
+        ys[t] = acc;
+    }
+
+    if (ops & RTX_OP_BCONV) {
+        const int l0 = m.link_start[k], l1 = m.link_start[k + 1];
+        for (int l = l0; l < l1; ++l) {
+            const int s = m.link_src[l], F = m.link_face[l];
+            const int c1 = m.cell_col[s];
+            const int nx1 = m.col_nx[c1], ny1 = m.col_ny[c1], nt1 = m.cell_nth[s];
+            const int off1 = m.cell_dof_off[s];
+            const int bd1 = restrict_ ? m.cell_bdry[s] : 0;
+            const int n1 = (F & 1) ? nx1 : ny1;   // neighbour trace length
+            const int n0 = (F & 1) ? nx : ny;     // own trace length
+            __syncthreads();                      // tr reuse
+            for (int t = lane; t < n1 * nt1; t += 64) {
+                const int m1 = t / nt1, a = t % nt1;
+                int i1, j1;
+                if (F == 0) { i1 = 0; j1 = m1; }
+                else if (F == 2) { i1 = nx1 - 1; j1 = m1; }
+                else if (F == 1) { i1 = m1; j1 = 0; }
+                else { i1 = m1; j1 = ny1 - 1; }
+                double v = double(x[off1 + (i1 * ny1 + j1) * nt1 + a]);
+                if (bd1 && on_bdry(bd1, i1, j1, nx1, ny1)) v = 0.0;
+                tr[t] = v;
+            }
+            __syncthreads();
+            const double *Es = m.Es_pool + m.link_Es_off[l];
+            const double *Et = m.Eth_pool + m.link_Eth_off[l];
+            const double cf = ((F & 1) ? dx : dy) * dth * 0.25;
+            for (int t = lane; t < n0 * nth; t += 64) {
+                const int m0 = t / nth, r = t % nth;
+                double acc = 0.0;
+                for (int m1 = 0; m1 < n1; ++m1) {
+                    double inner = 0.0;
+                    for (int a = 0; a < nt1; ++a) inner += Et[a * nth + r] * tr[m1 * nt1 + a];
+                    acc += Es[m1 * n0 + m0] * inner;
+                }
+                int node;
+                if (F == 0) node = (nx - 1) * nyt + m0 * nth + r;
+                else if (F == 2) node = m0 * nth + r;
+                else if (F == 1) node = m0 * nyt + (ny - 1) * nth + r;
+                else node = m0 * nyt + r;
+                ys[node] += cf * acc;   // one lane per node: no conflict
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = lane; t < n; t += 64) {
+        const int i = t / nyt, j = (t / nth) % ny;
+        const bool masked = restrict_ && on_bdry(m.cell_bdry[k], i, j, nx, ny);
+        double v = masked ? 0.0 : ys[t];
+        if (accumulate) {
+            if (!masked) y[off + t] = T(double(y[off + t]) + v);
+        } else {
+            y[off + t] = T(v);
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(SCAT_THREADS) void k_scatter(DevMesh m, uint32_t ops, int accumulate,
+                                                          const T *__restrict__ x,
+                                                          T *__restrict__ y) {
+    __shared__ double us[SCAT_LDS_DOUBLES];
+    __shared__ int a_off[MAX_NANG];    // first DOF of the cell owning angle A
+    __shared__ int a_nt[MAX_NANG];     // that cell's nth
+    __shared__ int a_loc[MAX_NANG];    // local angular index within the cell
+    __shared__ int a_bd[MAX_NANG];     // that cell's masked faces
+    const int c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nx = m.col_nx[c], ny = m.col_ny[c], nsp = nx * ny;
+    const int na = m.col_nang[c];
+    const int k0 = m.col_cell_start[c], k1 = m.col_cell_start[c + 1];
+    const bool restrict_ = (ops & RTX_OP_RESTRICT) != 0;
+    for (int k = k0 + tid; k < k1; k += SCAT_THREADS) {
+        const int nt = m.cell_nth[k], ao = m.cell_ang_off[k], dof = m.cell_dof_off[k];
+        const int bd = restrict_ ? m.cell_bdry[k] : 0;
+        for (int a = 0; a < nt; ++a) {
+            a_off[ao + a] = dof;
+            a_nt[ao + a] = nt;
+            a_loc[ao + a] = a;
+            a_bd[ao + a] = bd;
+        }
+    }
+    const double x0 = m.col_geom[4 * c + 0], y0 = m.col_geom[4 * c + 1];
+    const double dxdy = (m.col_geom[4 * c + 2] - x0) * (m.col_geom[4 * c + 3] - y0);
+    const double *wx = m.lgl_w + nx * NM, *wy = m.lgl_w + ny * NM;
+    const double *G = m.G + m.col_G_off[c];
+    const double *sig = m.sigma + m.col_sp_off[c];
+    const int ts = SCAT_LDS_DOUBLES / na < nsp ? SCAT_LDS_DOUBLES / na : nsp;
+    __syncthreads();
+    for (int s0 = 0; s0 < nsp; s0 += ts) {
+        const int sn = (nsp - s0) < ts ? (nsp - s0) : ts;
+        __syncthreads();
+        for (int t = tid; t < sn * na; t += SCAT_THREADS) {
+            const int sl = t / na, A = t % na, s = s0 + sl;
+            const int i = s / ny, j = s % ny;
+            double v = double(x[a_off[A] + s * a_nt[A] + a_loc[A]]);
+            if (a_bd[A] && on_bdry(a_bd[A], i, j, nx, ny)) v = 0.0;
+            us[t] = v;
+        }
+        __syncthreads();
+        for (int t = tid; t < sn * na; t += SCAT_THREADS) {
+            const int sl = t / na, R = t % na, s = s0 + sl;
+            const int i = s / ny, j = s % ny;
+            const double *u = us + sl * na;
+            double acc = 0.0;
+            for (int A = 0; A < na; ++A) acc += G[size_t(A) * na + R] * u[A];
+            const double v = -dxdy * wx[i] * wy[j] * sig[s] * acc;
+            const int addr = a_off[R] + s * a_nt[R] + a_loc[R];
+            const bool masked = restrict_ && a_bd[R] && on_bdry(a_bd[R], i, j, nx, ny);
+            if (accumulate) {
+                if (!masked) y[addr] = T(double(y[addr]) + v);
+            } else {
+                y[addr] = T(masked ? 0.0 : v);
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_zero(T *y, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i < n; i += int64_t(gridDim.x) * blockDim.x) y[i] = T(0);
+}
+
+template <typename T>
+static int launch_apply(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t st) {
+    const T *x = static_cast<const T *>(dx);
+    T *y = static_cast<T *>(dy);
+    const bool acc = (ops & RTX_OP_ACCUM) != 0;
+    const bool tp = (ops & (RTX_OP_MASS | RTX_OP_ICONV | RTX_OP_BCONV)) != 0;
+    const bool sc = (ops & RTX_OP_SCAT) != 0;
+    if (!tp && !sc) {
+        if (!acc) k_zero<T><<<1024, 256, 0, st>>>(y, p->m.ndof);
+    }
+    if (tp)
+        k_transport<T><<<p->m.ncell, 64, 0, st>>>(p->m, ops, acc ? 1 : 0, x, y);
+    if (sc)
+        k_scatter<T><<<p->m.ncol, SCAT_THREADS, 0, st>>>(p->m, ops, (acc || tp) ? 1 : 0, x, y);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+}  // namespace rtx
+
+using namespace rtx;
+
+extern "C" int rtx_apply(rtx_plan *p, uint32_t op_mask, int precision, const void *d_x,
+                         void *d_y, void *stream) {
+    RTX_CHECK(p && d_x && d_y, "rtx_apply: null argument");
+    if (!p->has_coeffs) {
+        set_error("rtx_apply: coefficients not set (call rtx_plan_set_coeffs)");
+        return RTX_ERR_STATE;
+    }
+    RTX_CHECK(d_x != d_y, "rtx_apply: x and y must not alias");
+    RTX_CHECK(p->max_nang <= MAX_NANG, "rtx_apply: a column has %d angular nodes (max %d)",
+              p->max_nang, MAX_NANG);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    switch (precision) {
+        case RTX_PREC_F64: return launch_apply<double>(p, op_mask, d_x, d_y, st);
+        case RTX_PREC_F32: return launch_apply<float>(p, op_mask, d_x, d_y, st);
+        default: set_error("rtx_apply: unsupported precision %d", precision); return RTX_ERR_ARG;
+    }
+}

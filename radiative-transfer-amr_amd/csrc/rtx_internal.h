@@ -1,0 +1,63 @@
+// Internal definitions shared by the rtx translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/rtx.h"
+
+namespace rtx {
+
+void set_error(const char *fmt, ...);
+
+#define RTX_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            ::rtx::set_error("%s failed: %s (%s:%d)", #call,                   \
+                             hipGetErrorString(e_), __FILE__, __LINE__);       \
+            return RTX_ERR_HIP;                                                \
+        }                                                                      \
+    } while (0)
+
+#define RTX_CHECK(cond, ...)                                                   \
+    do {                                                                       \
+        if (!(cond)) {                                                         \
+            ::rtx::set_error(__VA_ARGS__);                                     \
+            return RTX_ERR_ARG;                                                \
+        }                                                                      \
+    } while (0)
+
+// Device-side view of the plan (passed by value to kernels).
+struct DevMesh {
+    int32_t ncol, ncell, nlink;
+    int64_t ndof;
+    const int32_t *col_cell_start, *col_nx, *col_ny, *col_sp_off, *col_nang;
+    const int64_t *col_G_off;
+    const double *col_geom;
+    const int32_t *cell_dof_off, *cell_col, *cell_nth, *cell_quad, *cell_ang_off,
+        *cell_bdry;
+    const double *cell_geom;
+    const int32_t *link_start, *link_src, *link_face, *link_Es_off, *link_Eth_off;
+    const double *Es_pool, *Eth_pool;
+    const double *lgl_x, *lgl_w, *lg_x, *lg_w, *lgl_D;
+    // coefficients
+    const double *kappa, *sigma, *G;
+    const float *kappa_f, *sigma_f, *G_f;
+    const uint16_t *G_bf16;
+};
+
+}  // namespace rtx
+
+struct rtx_plan {
+    int device = 0;
+    rtx::DevMesh m{};
+    void *arena = nullptr;       // one device allocation for all tables
+    size_t arena_bytes = 0;
+    void *coef = nullptr;        // kappa, sigma, G (+ reduced-precision copies)
+    size_t coef_bytes = 0;
+    int64_t nsp = 0, nG = 0;
+    int32_t max_nang = 0, max_cell_ndof = 0, max_nsp = 0;
+    bool has_coeffs = false;
+};

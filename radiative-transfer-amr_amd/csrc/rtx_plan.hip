@@ -1,0 +1,282 @@
+// Plan creation: copy the flattened hp mesh and coefficient tables to HBM.
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "rtx_internal.h"
+
+namespace rtx {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+namespace {
+
+struct Packer {
+    struct Item {
+        const void *src;
+        size_t bytes;
+        size_t off;
+    };
+    std::vector<Item> items;
+    size_t total = 0;
+    size_t add(const void *src, size_t bytes) {
+        size_t off = total;
+        items.push_back({src, bytes, off});
+        total += (bytes + 255) & ~size_t(255);
+        return off;
+    }
+};
+
+}  // namespace
+}  // namespace rtx
+
+using namespace rtx;
+
+extern "C" const char *rtx_last_error(void) { return g_err; }
+
+extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
+    RTX_CHECK(d != nullptr && out != nullptr, "rtx_plan_create: null argument");
+    RTX_CHECK(d->ncol > 0 && d->ncell > 0 && d->nlink >= 0,
+              "rtx_plan_create: empty mesh (ncol=%d ncell=%d)", d->ncol, d->ncell);
+    RTX_CHECK(d->ndof > 0 && d->ndof < (int64_t(1) << 31),
+              "rtx_plan_create: ndof=%lld outside [1, 2^31)", (long long)d->ndof);
+    const int nc = d->ncol, nk = d->ncell, nl = d->nlink;
+    // Validate the tables the kernels index with.
+    int32_t max_nang = 0, max_cell = 0, max_nsp = 0;
+    for (int c = 0; c < nc; ++c) {
+        int nx = d->col_nx[c], ny = d->col_ny[c];
+        RTX_CHECK(nx >= 1 && nx <= RTX_NMAX && ny >= 1 && ny <= RTX_NMAX,
+                  "column %d: orders (%d, %d) outside [1, %d]", c, nx, ny, RTX_NMAX);
+        int k0 = d->col_cell_start[c], k1 = d->col_cell_start[c + 1];
+        RTX_CHECK(k0 >= 0 && k1 > k0 && k1 <= nk, "column %d: bad cell range", c);
+        int nang = 0;
+        for (int k = k0; k < k1; ++k) {
+            RTX_CHECK(d->cell_col[k] == c, "cell %d: owner mismatch", k);
+            int nth = d->cell_nth[k];
+            RTX_CHECK(nth >= 1 && nth <= RTX_NMAX, "cell %d: nth=%d outside [1, %d]", k,
+                      nth, RTX_NMAX);
+            RTX_CHECK(d->cell_ang_off[k] == nang, "cell %d: angular offset", k);
+            nang += nth;
+            int64_t cd = int64_t(nx) * ny * nth;
+            RTX_CHECK(d->cell_dof_off[k] >= 0 && d->cell_dof_off[k] + cd <= d->ndof,
+                      "cell %d: DOF range outside the vector", k);
+            if (cd > max_cell) max_cell = int32_t(cd);
+        }
+        RTX_CHECK(d->col_nang[c] == nang, "column %d: nang mismatch", c);
+        if (nang > max_nang) max_nang = nang;
+        if (nx * ny > max_nsp) max_nsp = nx * ny;
+    }
+    RTX_CHECK(d->col_cell_start[0] == 0 && d->col_cell_start[nc] == nk,
+              "cell ranges do not cover the cells");
+    RTX_CHECK(d->link_start[0] == 0 && d->link_start[nk] == nl, "bad link_start");
+    for (int k = 0; k < nk; ++k)
+        RTX_CHECK(d->link_start[k] <= d->link_start[k + 1], "link_start not sorted");
+    for (int k = 0; k < nk; ++k) {
+        const int c0 = d->cell_col[k];
+        for (int l = d->link_start[k]; l < d->link_start[k + 1]; ++l) {
+            int s = d->link_src[l], F = d->link_face[l];
+            RTX_CHECK(s >= 0 && s < nk && F >= 0 && F < 4, "link %d: bad source/face", l);
+            const int c1 = d->cell_col[s];
+            // E^s is n1 x n0 (trace lengths), E^theta is nth1 x nth0
+            int64_t n0 = (F % 2 == 0) ? d->col_ny[c0] : d->col_nx[c0];
+            int64_t n1 = (F % 2 == 0) ? d->col_ny[c1] : d->col_nx[c1];
+            int64_t es = d->link_Es_off[l], et = d->link_Eth_off[l];
+            RTX_CHECK(es >= 0 && es + n0 * n1 <= d->nEs, "link %d: E^s outside its pool", l);
+            RTX_CHECK(et >= 0 && et + int64_t(d->cell_nth[k]) * d->cell_nth[s] <= d->nEth,
+                      "link %d: E^theta outside its pool", l);
+        }
+    }
+
+    auto *p = new rtx_plan();
+    hipError_t he = hipGetDevice(&p->device);
+    if (he != hipSuccess) {
+        set_error("hipGetDevice: %s", hipGetErrorString(he));
+        delete p;
+        return RTX_ERR_HIP;
+    }
+    Packer pk;
+    const size_t I = sizeof(int32_t), D = sizeof(double);
+    const int NT = (RTX_NMAX + 1) * RTX_NMAX;
+    size_t o_ccs = pk.add(d->col_cell_start, (nc + 1) * I);
+    size_t o_cnx = pk.add(d->col_nx, nc * I);
+    size_t o_cny = pk.add(d->col_ny, nc * I);
+    size_t o_csp = pk.add(d->col_sp_off, nc * I);
+    size_t o_cna = pk.add(d->col_nang, nc * I);
+    size_t o_cgo = pk.add(d->col_G_off, nc * sizeof(int64_t));
+    size_t o_cge = pk.add(d->col_geom, 4 * nc * D);
+    size_t o_kdo = pk.add(d->cell_dof_off, nk * I);
+    size_t o_kco = pk.add(d->cell_col, nk * I);
+    size_t o_knt = pk.add(d->cell_nth, nk * I);
+    size_t o_kqu = pk.add(d->cell_quad, nk * I);
+    size_t o_kao = pk.add(d->cell_ang_off, nk * I);
+    size_t o_kbd = pk.add(d->cell_bdry, nk * I);
+    size_t o_kge = pk.add(d->cell_geom, 2 * nk * D);
+    size_t o_lst = pk.add(d->link_start, (nk + 1) * I);
+    size_t o_lsr = pk.add(d->link_src, (nl > 0 ? nl : 1) * I);
+    size_t o_lfa = pk.add(d->link_face, (nl > 0 ? nl : 1) * I);
+    size_t o_les = pk.add(d->link_Es_off, (nl > 0 ? nl : 1) * I);
+    size_t o_let = pk.add(d->link_Eth_off, (nl > 0 ? nl : 1) * I);
+    size_t o_esp = pk.add(d->Es_pool, (d->nEs > 0 ? d->nEs : 1) * D);
+    size_t o_etp = pk.add(d->Eth_pool, (d->nEth > 0 ? d->nEth : 1) * D);
+    size_t o_lx = pk.add(d->lgl_x, NT * D);
+    size_t o_lw = pk.add(d->lgl_w, NT * D);
+    size_t o_gx = pk.add(d->lg_x, NT * D);
+    size_t o_gw = pk.add(d->lg_w, NT * D);
+    size_t o_ld = pk.add(d->lgl_D, NT * RTX_NMAX * D);
+
+    he = hipMalloc(&p->arena, pk.total);
+    if (he != hipSuccess) {
+        set_error("hipMalloc(%zu): %s", pk.total, hipGetErrorString(he));
+        delete p;
+        return RTX_ERR_HIP;
+    }
+    p->arena_bytes = pk.total;
+    std::vector<char> staging(pk.total, 0);
+    for (auto &it : pk.items)
+        if (it.src != nullptr && it.bytes > 0) memcpy(staging.data() + it.off, it.src, it.bytes);
+    he = hipMemcpy(p->arena, staging.data(), pk.total, hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+        set_error("hipMemcpy(plan): %s", hipGetErrorString(he));
+        (void)hipFree(p->arena);
+        delete p;
+        return RTX_ERR_HIP;
+    }
+    char *b = static_cast<char *>(p->arena);
+    DevMesh &m = p->m;
+    m.ncol = nc;
+    m.ncell = nk;
+    m.nlink = nl;
+    m.ndof = d->ndof;
+    m.col_cell_start = (const int32_t *)(b + o_ccs);
+    m.col_nx = (const int32_t *)(b + o_cnx);
+    m.col_ny = (const int32_t *)(b + o_cny);
+    m.col_sp_off = (const int32_t *)(b + o_csp);
+    m.col_nang = (const int32_t *)(b + o_cna);
+    m.col_G_off = (const int64_t *)(b + o_cgo);
+    m.col_geom = (const double *)(b + o_cge);
+    m.cell_dof_off = (const int32_t *)(b + o_kdo);
+    m.cell_col = (const int32_t *)(b + o_kco);
+    m.cell_nth = (const int32_t *)(b + o_knt);
+    m.cell_quad = (const int32_t *)(b + o_kqu);
+    m.cell_ang_off = (const int32_t *)(b + o_kao);
+    m.cell_bdry = (const int32_t *)(b + o_kbd);
+    m.cell_geom = (const double *)(b + o_kge);
+    m.link_start = (const int32_t *)(b + o_lst);
+    m.link_src = (const int32_t *)(b + o_lsr);
+    m.link_face = (const int32_t *)(b + o_lfa);
+    m.link_Es_off = (const int32_t *)(b + o_les);
+    m.link_Eth_off = (const int32_t *)(b + o_let);
+    m.Es_pool = (const double *)(b + o_esp);
+    m.Eth_pool = (const double *)(b + o_etp);
+    m.lgl_x = (const double *)(b + o_lx);
+    m.lgl_w = (const double *)(b + o_lw);
+    m.lg_x = (const double *)(b + o_gx);
+    m.lg_w = (const double *)(b + o_gw);
+    m.lgl_D = (const double *)(b + o_ld);
+    p->max_nang = max_nang;
+    p->max_cell_ndof = max_cell;
+    p->max_nsp = max_nsp;
+    // sizes of the column G blocks are checked in set_coeffs
+    *out = p;
+    return RTX_OK;
+}
+
+extern "C" void rtx_plan_destroy(rtx_plan *p) {
+    if (!p) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(p->device);
+    if (p->arena) (void)hipFree(p->arena);
+    if (p->coef) (void)hipFree(p->coef);
+    (void)hipSetDevice(cur);
+    delete p;
+}
+
+// Conversion kernels for reduced-precision copies of the coefficients.
+__global__ void k_to_f32(const double *__restrict__ s, float *__restrict__ d, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t st = int64_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += st) d[i] = float(s[i]);
+}
+
+__device__ inline uint16_t f32_to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return uint16_t(u >> 16);
+}
+
+__global__ void k_to_bf16(const double *__restrict__ s, uint16_t *__restrict__ d, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t st = int64_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += st) d[i] = f32_to_bf16_rne(float(s[i]));
+}
+
+extern "C" int rtx_plan_set_coeffs(rtx_plan *p, int64_t nsp, const double *kappa,
+                                   const double *sigma, int64_t nG, const double *G) {
+    RTX_CHECK(p && kappa && sigma && G, "rtx_plan_set_coeffs: null argument");
+    RTX_CHECK(nsp > 0 && nG > 0, "rtx_plan_set_coeffs: empty coefficient arrays");
+    // The caller's tables must cover every column's spatial nodes and G block.
+    {
+        std::vector<int32_t> nx(p->m.ncol), ny(p->m.ncol), sp(p->m.ncol), na(p->m.ncol);
+        std::vector<int64_t> go(p->m.ncol);
+        RTX_HIP(hipMemcpy(nx.data(), p->m.col_nx, nx.size() * 4, hipMemcpyDeviceToHost));
+        RTX_HIP(hipMemcpy(ny.data(), p->m.col_ny, ny.size() * 4, hipMemcpyDeviceToHost));
+        RTX_HIP(hipMemcpy(sp.data(), p->m.col_sp_off, sp.size() * 4, hipMemcpyDeviceToHost));
+        RTX_HIP(hipMemcpy(na.data(), p->m.col_nang, na.size() * 4, hipMemcpyDeviceToHost));
+        RTX_HIP(hipMemcpy(go.data(), p->m.col_G_off, go.size() * 8, hipMemcpyDeviceToHost));
+        for (int c = 0; c < p->m.ncol; ++c) {
+            RTX_CHECK(sp[c] >= 0 && int64_t(sp[c]) + nx[c] * ny[c] <= nsp,
+                      "column %d: spatial nodes outside kappa/sigma (nsp=%lld)", c,
+                      (long long)nsp);
+            RTX_CHECK(go[c] >= 0 && go[c] + int64_t(na[c]) * na[c] <= nG,
+                      "column %d: G block outside the G pool (nG=%lld)", c, (long long)nG);
+        }
+    }
+    int cur = 0;
+    RTX_HIP(hipGetDevice(&cur));
+    RTX_CHECK(cur == p->device, "rtx_plan_set_coeffs: plan lives on device %d, current %d",
+              p->device, cur);
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    size_t o_k = 0, o_s = al(nsp * 8), o_g = o_s + al(nsp * 8);
+    size_t o_kf = o_g + al(nG * 8), o_sf = o_kf + al(nsp * 4), o_gf = o_sf + al(nsp * 4);
+    size_t o_gb = o_gf + al(nG * 4);
+    size_t total = o_gb + al(nG * 2);
+    if (p->coef && p->coef_bytes < total) {
+        RTX_HIP(hipFree(p->coef));
+        p->coef = nullptr;
+    }
+    if (!p->coef) {
+        RTX_HIP(hipMalloc(&p->coef, total));
+        p->coef_bytes = total;
+    }
+    char *b = static_cast<char *>(p->coef);
+    RTX_HIP(hipMemcpy(b + o_k, kappa, nsp * 8, hipMemcpyHostToDevice));
+    RTX_HIP(hipMemcpy(b + o_s, sigma, nsp * 8, hipMemcpyHostToDevice));
+    RTX_HIP(hipMemcpy(b + o_g, G, nG * 8, hipMemcpyHostToDevice));
+    p->m.kappa = (const double *)(b + o_k);
+    p->m.sigma = (const double *)(b + o_s);
+    p->m.G = (const double *)(b + o_g);
+    p->m.kappa_f = (const float *)(b + o_kf);
+    p->m.sigma_f = (const float *)(b + o_sf);
+    p->m.G_f = (const float *)(b + o_gf);
+    p->m.G_bf16 = (const uint16_t *)(b + o_gb);
+    int grid = 1024;
+    k_to_f32<<<grid, 256>>>(p->m.kappa, (float *)p->m.kappa_f, nsp);
+    k_to_f32<<<grid, 256>>>(p->m.sigma, (float *)p->m.sigma_f, nsp);
+    k_to_f32<<<grid, 256>>>(p->m.G, (float *)p->m.G_f, nG);
+    k_to_bf16<<<grid, 256>>>(p->m.G, (uint16_t *)p->m.G_bf16, nG);
+    RTX_HIP(hipGetLastError());
+    RTX_HIP(hipDeviceSynchronize());
+    p->nsp = nsp;
+    p->nG = nG;
+    p->has_coeffs = true;
+    return RTX_OK;
+}

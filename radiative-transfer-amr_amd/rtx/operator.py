@@ -1,0 +1,101 @@
+"""Device-resident matrix-free transport operator.
+
+``TransportOperator(mesh, kappa, sigma, Phi)`` flattens the mesh
+(``plan.MeshTables``), creates the device plan, evaluates the coefficient
+callables at the nodes and uploads them.  ``apply`` is the matrix-free
+counterpart of PETSc ``MatMult`` with the reference's assembled operator
+(``src/rt/solve.py:49-52``); ``RESTRICT`` gives the interior block A_II in
+full-length vectors (zeros at inflow-boundary DOFs).
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import (OP_MASS, OP_SCAT, OP_ICONV, OP_BCONV, OP_RESTRICT, OP_ACCUM,
+                   OP_SYSTEM, PREC_F64, PREC_F32, PREC_MIXED, check, ptr)
+from .plan import MeshTables
+
+__all__ = ["TransportOperator", "OP_MASS", "OP_SCAT", "OP_ICONV", "OP_BCONV",
+           "OP_RESTRICT", "OP_ACCUM", "OP_SYSTEM", "PREC_F64", "PREC_F32",
+           "PREC_MIXED"]
+
+
+def stream_handle(stream=None):
+    s = torch.cuda.current_stream() if stream is None else stream
+    return C.c_void_p(s.cuda_stream)
+
+
+class TransportOperator:
+    def __init__(self, mesh, kappa, sigma, Phi, device=None, tables=None):
+        if not torch.cuda.is_available():
+            raise _lib.RtxError("TransportOperator needs a GPU (torch.cuda is not available)")
+        self.lib = _lib.load()
+        self.device = torch.device("cuda" if device is None else device)
+        self.t = MeshTables(mesh) if tables is None else tables
+        t = self.t
+        self.ndof = t.ndof
+        self._keep = []
+        d = _lib.MeshDesc()
+        d.ncol, d.ncell, d.nlink, d.ndof = t.ncol, t.ncell, t.nlink, t.ndof
+
+        def P(arr, ct):
+            a = np.ascontiguousarray(arr)
+            self._keep.append(a)
+            return ptr(a, ct)
+
+        i32, i64, f64 = C.c_int32, C.c_int64, C.c_double
+        G_pool, G_off = t.G_pool(Phi)
+        d.col_cell_start = P(t.col_cell_start, i32)
+        d.col_nx, d.col_ny = P(t.col_nx, i32), P(t.col_ny, i32)
+        d.col_sp_off, d.col_nang = P(t.col_sp_off, i32), P(t.col_nang, i32)
+        d.col_G_off = P(G_off.astype(np.int64), i64)
+        d.col_geom = P(t.col_geom.ravel(), f64)
+        d.cell_dof_off, d.cell_col = P(t.cell_dof_off, i32), P(t.cell_col, i32)
+        d.cell_nth, d.cell_quad = P(t.cell_nth, i32), P(t.cell_quad, i32)
+        d.cell_ang_off, d.cell_bdry = P(t.cell_ang_off, i32), P(t.cell_bdry, i32)
+        d.cell_geom = P(t.cell_geom.ravel(), f64)
+        d.link_start, d.link_src = P(t.link_start, i32), P(t.link_src, i32)
+        d.link_face = P(t.link_face, i32)
+        d.link_Es_off, d.link_Eth_off = P(t.link_Es_off, i32), P(t.link_Eth_off, i32)
+        d.nEs, d.nEth = t.Es_pool.size, t.Eth_pool.size
+        d.Es_pool, d.Eth_pool = P(t.Es_pool, f64), P(t.Eth_pool, f64)
+        d.lgl_x, d.lgl_w = P(t.lgl_x, f64), P(t.lgl_w, f64)
+        d.lg_x, d.lg_w, d.lgl_D = P(t.lg_x, f64), P(t.lg_w, f64), P(t.lgl_D, f64)
+        with torch.cuda.device(self.device):
+            h = C.c_void_p()
+            check(self.lib.rtx_plan_create(C.byref(d), C.byref(h)), "rtx_plan_create")
+            self.plan = h
+            self.kappa_nodes = np.ascontiguousarray(t.eval_spatial(kappa))
+            self.sigma_nodes = np.ascontiguousarray(t.eval_spatial(sigma))
+            self.G_pool = np.ascontiguousarray(G_pool)
+            check(self.lib.rtx_plan_set_coeffs(
+                self.plan, t.nsp, ptr(self.kappa_nodes, f64), ptr(self.sigma_nodes, f64),
+                self.G_pool.size, ptr(self.G_pool, f64)), "rtx_plan_set_coeffs")
+        self._keep = []
+
+    def __del__(self):
+        try:
+            if getattr(self, "plan", None) is not None and self.plan.value:
+                self.lib.rtx_plan_destroy(self.plan)
+                self.plan = None
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------
+    def _check_vec(self, v, dtype):
+        if not (v.is_cuda and v.dtype == dtype and v.is_contiguous() and v.numel() == self.ndof):
+            raise ValueError("expected a contiguous {} CUDA tensor of {} entries, got {} {} {}"
+                             .format(dtype, self.ndof, v.dtype, tuple(v.shape), v.device))
+
+    def apply(self, x, y=None, ops=OP_SYSTEM, precision=PREC_F64, stream=None):
+        """y = op(x) on the device (y += op(x) with OP_ACCUM)."""
+        dtype = torch.float64 if precision == PREC_F64 else torch.float32
+        self._check_vec(x, dtype)
+        if y is None:
+            y = torch.empty_like(x)
+        self._check_vec(y, dtype)
+        check(self.lib.rtx_apply(self.plan, ops, precision, C.c_void_p(x.data_ptr()),
+                                 C.c_void_p(y.data_ptr()), stream_handle(stream)), "rtx_apply")
+        return y

@@ -1,0 +1,142 @@
+"""Workload definitions: meshes and coefficient callables of the benchmark
+configurations (the analogue of the reference's
+``experiments/*/problem.py`` and ``experiments/*/mesh.py``).
+
+* ``henyey_greenstein(g)``: the reference's scattering kernel form
+  (``experiments/two-strong-scatterers/problem.py:36-43``), renormalised on
+  [0, 2 pi); the normalisation uses the closed form
+  N_g = 4 E(m) / (1 - g), m = 4 g / (1 + g)^2 (complete elliptic integral of
+  the second kind) instead of adaptive quadrature.
+* ``uniform_mesh``: the reference's experiment meshes
+  (``experiments/*/mesh.py``: ``nref_ang`` angular then ``nref_spt`` spatial
+  uniform h-refinements).
+* ``manufactured``: smooth manufactured solution with consistent forcing and
+  inflow data.
+* ``absorber_c2``: kappa = 1 + 0.5 sin(2 pi x) sin(2 pi y), sigma = 0.7 kappa,
+  HG g = 0.8, unit isotropic inflow on the left wall, no forcing.
+* ``two_scatterers``: two Gaussian strong-scattering blobs in a weakly
+  scattering background.
+"""
+import math
+
+import numpy as np
+from scipy import integrate, special
+
+from dg.mesh import Mesh
+
+
+def hg_norm(g: float) -> float:
+    """int_0^{2 pi} (1 - g^2) / (1 + g^2 - 2 g cos t)^{3/2} dt."""
+    if g == 0.0:
+        return 2.0 * math.pi
+    m = 4.0 * g / (1.0 + g) ** 2
+    return 4.0 * special.ellipe(m) / (1.0 - g)
+
+
+def henyey_greenstein(g: float):
+    norm = hg_norm(g)
+
+    def Phi(th, phi):
+        return ((1.0 - g * g) / norm) / (1.0 + g * g - 2.0 * g * np.cos(th - phi)) ** 1.5
+    Phi.g = g
+    Phi.norm = norm
+    return Phi
+
+
+def hg_first_moment(g: float) -> float:
+    """int_0^{2 pi} Phi(t, 0) cos t dt for the normalised HG kernel."""
+    Phi = henyey_greenstein(g)
+    v, _ = integrate.quad(lambda t: Phi(t, 0.0) * math.cos(t), 0.0, 2.0 * math.pi,
+                          epsabs=1e-14, epsrel=1e-14, limit=400)
+    return v
+
+
+def uniform_mesh(Ls, ndofs, nref_ang, nref_spt, pbcs=(False, False)):
+    mesh = Mesh(Ls=list(Ls), pbcs=list(pbcs), ndofs=list(ndofs), has_th=True)
+    for _ in range(nref_ang):
+        mesh.ref_mesh(kind="ang", form="h")
+    for _ in range(nref_spt):
+        mesh.ref_mesh(kind="spt", form="h")
+    return mesh
+
+
+class Workload:
+    """kappa(x, y), sigma(x, y), Phi(th, phi), f(x, y, th), bcs(x, y, th)."""
+
+    def __init__(self, name, kappa, sigma, Phi, f, bcs, u=None):
+        self.name, self.kappa, self.sigma, self.Phi = name, kappa, sigma, Phi
+        self.f, self.bcs, self.u = f, bcs, u
+
+    @property
+    def bcs_dirac(self):
+        return [self.bcs, [None, None, None]]
+
+
+def manufactured(kappa0=1.1, sigma0=0.5, g=0.6, width=0.1, c=(0.5, 0.5)):
+    """u = exp(-|x - c|^2 / width) (1 + 0.5 cos th) with constant kappa, sigma
+    and HG scattering; f = s . grad u + kappa u - sigma int Phi u."""
+    Phi = henyey_greenstein(g)
+    m1 = hg_first_moment(g)
+
+    def U(x, y):
+        return np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2) / width)
+
+    def u(x, y, th):
+        return U(x, y) * (1.0 + 0.5 * np.cos(th))
+
+    def f(x, y, th):
+        Ux = -2.0 * (x - c[0]) / width * U(x, y)
+        Uy = -2.0 * (y - c[1]) / width * U(x, y)
+        stream = (1.0 + 0.5 * np.cos(th)) * (np.cos(th) * Ux + np.sin(th) * Uy)
+        return stream + kappa0 * u(x, y, th) - sigma0 * U(x, y) * (1.0 + 0.5 * m1 * np.cos(th))
+
+    def kappa(x, y):
+        return kappa0 + 0.0 * (x + y)
+
+    def sigma(x, y):
+        return sigma0 + 0.0 * (x + y)
+
+    return Workload("manufactured", kappa, sigma, Phi, f, u, u=u)
+
+
+def absorber_c2(g=0.8, ratio=0.7):
+    Phi = henyey_greenstein(g)
+
+    def kappa(x, y):
+        return 1.0 + 0.5 * np.sin(2.0 * np.pi * x) * np.sin(2.0 * np.pi * y)
+
+    def sigma(x, y):
+        return ratio * kappa(x, y)
+
+    def f(x, y, th):
+        return 0.0 * (x + y + th)
+
+    def bcs(x, y, th):
+        return np.where(np.asarray(x) == 0.0, 1.0, 0.0) + 0.0 * (y + th)
+
+    return Workload("absorber", kappa, sigma, Phi, f, bcs)
+
+
+def two_scatterers(g=0.9, centres=((0.3, 0.6), (0.7, 0.4)), radius=0.1,
+                   kappa_blob=10.0, ratio_blob=0.99, ratio_bg=0.1):
+    Phi = henyey_greenstein(g)
+
+    def blob(x, y):
+        v = 0.0
+        for cx, cy in centres:
+            v = np.maximum(v, np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / radius ** 2))
+        return v
+
+    def kappa(x, y):
+        return 1.0 + (kappa_blob - 1.0) * blob(x, y)
+
+    def sigma(x, y):
+        return (ratio_bg + (ratio_blob - ratio_bg) * blob(x, y)) * kappa(x, y)
+
+    def f(x, y, th):
+        return 0.0 * (x + y + th)
+
+    def bcs(x, y, th):
+        return np.where(np.asarray(x) == 0.0, 1.0, 0.0) + 0.0 * (y + th)
+
+    return Workload("two_scatterers", kappa, sigma, Phi, f, bcs)
