@@ -117,6 +117,46 @@ int rtx_plan_set_coeffs(rtx_plan *plan, int64_t nsp, const double *kappa,
 int rtx_apply(rtx_plan *plan, uint32_t op_mask, int precision, const void *d_x,
               void *d_y, void *stream);
 
+/* ---- per-cell block-Jacobi (PETSc PCBJACOBI of src/rt/solve.py:93-95; the
+   reference's discarded dense column inverse, preconditioner_matrix.py:259-262,
+   is the closest dense analogue).  Blocks: each cell's diagonal block of the
+   interior system (identity on inflow-boundary DOFs when restrict_interior). */
+
+/* Number of doubles of all cell inverses (sum of n_K^2). */
+int rtx_bjacobi_size(rtx_plan *plan, int64_t *n_entries);
+/* Assemble and invert every cell block (FP64, Gauss-Jordan with partial
+   pivoting) into d_inv (column-major per cell); optional FP32 copy into
+   d_inv_f32.  d_scratch (same size as d_inv) is needed only when a block does
+   not fit LDS.  Synchronises the stream; RTX_ERR_NUMERIC on a singular block. */
+int rtx_bjacobi_factor(rtx_plan *plan, int restrict_interior, double *d_inv, float *d_inv_f32,
+                       double *d_scratch, void *stream);
+/* z = blockdiag(inv) r, precision RTX_PREC_F64 (double inverses, vectors) or
+   RTX_PREC_F32 (float inverses, vectors). */
+int rtx_bjacobi_apply(rtx_plan *plan, int precision, const void *d_inv, const void *d_r,
+                      void *d_z, void *stream);
+
+/* ---- GMRES building blocks (PETSc KSPGMRES of src/rt/solve.py:81-100). */
+
+/* Workspace (doubles) needed by rtx_arnoldi_step / rtx_mdot. */
+int rtx_arnoldi_workspace(int max_vecs, int64_t *n_doubles);
+/* Orthogonalise d_w against the nv columns of d_V (length n, stride ld) by
+   classical Gram-Schmidt with one conditional re-orthogonalisation, store
+   w / |w| as column nv of d_V and the Hessenberg column (nv + 1 values) in
+   d_hcol.  Asynchronous. */
+int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n, double *d_w,
+                     double *d_hcol, double *d_work, void *stream);
+/* d_out[j] = V_j . w, j < nv. */
+int rtx_mdot(const double *d_V, int64_t ld, int nv, int64_t n, const double *d_w,
+             double *d_out, double *d_work, void *stream);
+/* d_x = (accumulate ? d_x : 0) + sum_j y_j V_j, y on the device. */
+int rtx_combine(const double *d_V, int64_t ld, int nv, const double *d_y, int64_t n,
+                double *d_x, int accumulate, void *stream);
+/* y = a x + b y (precision RTX_PREC_F64 or RTX_PREC_F32). */
+int rtx_axpby(int precision, int64_t n, double a, const void *d_x, double b, void *d_y,
+              void *stream);
+/* direction 0: FP64 -> FP32, 1: FP32 -> FP64. */
+int rtx_convert(int direction, int64_t n, const void *d_src, void *d_dst, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

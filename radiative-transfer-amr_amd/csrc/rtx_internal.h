@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/rtx.h"
 
@@ -60,4 +61,8 @@ struct rtx_plan {
     int64_t nsp = 0, nG = 0;
     int32_t max_nang = 0, max_cell_ndof = 0, max_nsp = 0;
     bool has_coeffs = false;
+    // block-Jacobi
+    int64_t *bj_off = nullptr;   // ncell + 1 offsets of the column-major inverses
+    int64_t bj_entries = 0;
+    bool bj_ready = false;
 };

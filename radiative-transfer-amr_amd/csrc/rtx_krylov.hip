@@ -1,0 +1,253 @@
+// Krylov building blocks for the device-resident GMRES.
+//
+// The reference hands the assembled interior system to PETSc KSP
+// (src/rt/solve.py:81-100: GMRES(50), which orthogonalises with classical
+// Gram-Schmidt).  Here one Arnoldi step (classical Gram-Schmidt with a
+// device-side "twice is enough" re-orthogonalisation test, then
+// normalisation) is a fixed sequence of fused FP64 kernels over the tall
+// basis V; reductions are two-phase and deterministic (per-block partial
+// sums, then one reduction per output).
+#include "rtx_internal.h"
+
+namespace rtx {
+
+constexpr int KT = 256;          // threads per block
+constexpr int KCH = 8;           // basis vectors per mdot block (gridDim.y chunks)
+constexpr int KNB = 1024;        // row blocks of the tall-skinny kernels
+constexpr int KMAXV = 256;       // largest basis handled by one Arnoldi step
+constexpr double REORTH = 0.5;   // re-orthogonalise when |w'|^2 < 0.5 |w|^2
+
+template <int N>
+__device__ inline void block_sum(double (&v)[N], double *sh) {
+    // sh: KT * N doubles; result in sh[j * KT]
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < N; ++j) sh[j * KT + tid] = v[j];
+    __syncthreads();
+    for (int s = KT / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+#pragma unroll
+            for (int j = 0; j < N; ++j) sh[j * KT + tid] += sh[j * KT + tid + s];
+        }
+        __syncthreads();
+    }
+}
+
+// partial[j * nb + b] = sum over block b's rows of V_j[i] w[i], j < nv;
+// if with_ww, partial[nv * nb + b] = sum w[i]^2 (computed by chunk 0).
+__global__ __launch_bounds__(KT) void k_mdot(const double *__restrict__ V, int64_t ld, int nv,
+                                             const double *__restrict__ w, int64_t n,
+                                             double *__restrict__ partial, int with_ww,
+                                             const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double sh[KT * (KCH + 1)];
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int j0 = blockIdx.y * KCH;
+    const int64_t per = (n + nb - 1) / nb;
+    const int64_t i0 = int64_t(b) * per, i1 = (i0 + per < n) ? i0 + per : n;
+    double acc[KCH + 1];
+#pragma unroll
+    for (int j = 0; j <= KCH; ++j) acc[j] = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += KT) {
+        const double wi = w[i];
+#pragma unroll
+        for (int j = 0; j < KCH; ++j)
+            if (j0 + j < nv) acc[j] += V[int64_t(j0 + j) * ld + i] * wi;
+        acc[KCH] += wi * wi;
+    }
+    block_sum<KCH + 1>(acc, sh);
+    if (threadIdx.x == 0) {
+        for (int j = 0; j < KCH; ++j)
+            if (j0 + j < nv) partial[int64_t(j0 + j) * nb + b] = sh[j * KT];
+        if (with_ww && blockIdx.y == 0) partial[int64_t(nv) * nb + b] = sh[KCH * KT];
+    }
+}
+
+// out[j] (+)= sum_b partial[j * nb + b]; one block per j.
+__global__ __launch_bounds__(KT) void k_reduce(const double *__restrict__ partial, int nb,
+                                               double *__restrict__ out, int accumulate,
+                                               const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double sh[KT];
+    const int j = blockIdx.x;
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nb; b += KT) v[0] += partial[int64_t(j) * nb + b];
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0) out[j] = accumulate ? out[j] + sh[0] : sh[0];
+}
+
+// w -= V h (nv vectors); partial[b] = sum over block b of the new w^2.
+__global__ __launch_bounds__(KT) void k_update(const double *__restrict__ V, int64_t ld, int nv,
+                                               const double *__restrict__ h,
+                                               double *__restrict__ w, int64_t n,
+                                               double *__restrict__ partial,
+                                               const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double hs[KMAXV];
+    __shared__ double sh[KT];
+    for (int j = threadIdx.x; j < nv; j += KT) hs[j] = h[j];
+    __syncthreads();
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int64_t per = (n + nb - 1) / nb;
+    const int64_t i0 = int64_t(b) * per, i1 = (i0 + per < n) ? i0 + per : n;
+    double ss[1] = {0.0};
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += KT) {
+        double acc = w[i];
+        int j = 0;
+        for (; j + 4 <= nv; j += 4) {
+            acc -= V[int64_t(j) * ld + i] * hs[j] + V[int64_t(j + 1) * ld + i] * hs[j + 1] +
+                   V[int64_t(j + 2) * ld + i] * hs[j + 2] + V[int64_t(j + 3) * ld + i] * hs[j + 3];
+        }
+        for (; j < nv; ++j) acc -= V[int64_t(j) * ld + i] * hs[j];
+        w[i] = acc;
+        ss[0] += acc * acc;
+    }
+    block_sum<1>(ss, sh);
+    if (threadIdx.x == 0) partial[b] = sh[0];
+}
+
+// state layout (doubles): [0] |w|^2 before projection, [1] |w'|^2 after;
+// flag (int) = 1 when the second pass is skipped.
+__global__ void k_reorth_test(const double *state, int *flag) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *flag = (state[1] >= REORTH * state[0]) ? 1 : 0;
+}
+
+// v_out = w / sqrt(nrm2); hcol[nv] = sqrt(nrm2).  nrm2 comes from state.
+__global__ void k_normalize(const double *__restrict__ w, double *__restrict__ v_out, int64_t n,
+                            const double *__restrict__ nrm2, double *__restrict__ hcol, int nv) {
+    const double beta = sqrt(*nrm2);
+    const double s = beta > 0.0 ? 1.0 / beta : 0.0;
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i < n; i += int64_t(gridDim.x) * blockDim.x) v_out[i] = w[i] * s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) hcol[nv] = beta;
+}
+
+// x += sum_j y_j V_j  (y on the device)
+__global__ __launch_bounds__(KT) void k_combine(const double *__restrict__ V, int64_t ld, int nv,
+                                                const double *__restrict__ y,
+                                                double *__restrict__ x, int64_t n, int accumulate) {
+    __shared__ double ys[KMAXV];
+    for (int j = threadIdx.x; j < nv; j += KT) ys[j] = y[j];
+    __syncthreads();
+    int64_t i = int64_t(blockIdx.x) * KT + threadIdx.x;
+    for (; i < n; i += int64_t(gridDim.x) * KT) {
+        double acc = accumulate ? x[i] : 0.0;
+        for (int j = 0; j < nv; ++j) acc += V[int64_t(j) * ld + i] * ys[j];
+        x[i] = acc;
+    }
+}
+
+// y = a x + b y (device scalars optional)
+template <typename T>
+__global__ void k_axpby(int64_t n, double a, const T *__restrict__ x, double b,
+                        T *__restrict__ y) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        y[i] = T(a * double(x[i]) + (b == 0.0 ? 0.0 : b * double(y[i])));
+}
+
+template <typename S, typename D>
+__global__ void k_convert(int64_t n, const S *__restrict__ s, D *__restrict__ d) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i < n; i += int64_t(gridDim.x) * blockDim.x) d[i] = D(s[i]);
+}
+
+}  // namespace rtx
+
+using namespace rtx;
+
+extern "C" int rtx_arnoldi_workspace(int max_vecs, int64_t *n_doubles) {
+    RTX_CHECK(n_doubles && max_vecs >= 1 && max_vecs < KMAXV,
+              "rtx_arnoldi_workspace: max_vecs must be in [1, %d)", KMAXV);
+    // dot partials ((KMAXV + 1) rows, the last one also holding the second-pass
+    // coefficients), norm partials, and a small state block (norms + flag)
+    *n_doubles = int64_t(KMAXV + 2) * KNB + 16;
+    return RTX_OK;
+}
+
+/* One Arnoldi step: orthogonalise w against V_0..V_{nv-1} (columns of length n,
+   stride ld), re-orthogonalise once when the norm dropped by more than the
+   "twice is enough" factor, normalise into V_nv and write the Hessenberg
+   column h_0..h_{nv-1}, h_nv = |w| to d_hcol (device, nv + 1 doubles).
+   w is overwritten.  Everything is enqueued on `stream`; no host sync. */
+extern "C" int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n, double *d_w,
+                                double *d_hcol, double *d_work, void *stream) {
+    RTX_CHECK(d_V && d_w && d_hcol && d_work, "rtx_arnoldi_step: null argument");
+    RTX_CHECK(nv >= 1 && nv < KMAXV - 1, "rtx_arnoldi_step: nv=%d outside [1, %d)", nv,
+              KMAXV - 1);
+    RTX_CHECK(n >= 1 && ld >= n, "rtx_arnoldi_step: bad n/ld");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    double *part = d_work;                                  // (nv + 1) * KNB
+    double *part2 = d_work + int64_t(KMAXV + 1) * KNB;      // KNB
+    double *state = part2 + KNB;                            // [ww0, ww1, h2...]
+    int *flag = reinterpret_cast<int *>(state + 4);
+    const int nchunk = (nv + KCH - 1) / KCH;
+    double *V_out = const_cast<double *>(d_V) + int64_t(nv) * ld;
+    // pass 1: h = V^T w, |w|^2
+    k_mdot<<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, part, 1, nullptr);
+    k_reduce<<<nv, KT, 0, st>>>(part, KNB, d_hcol, 0, nullptr);
+    k_reduce<<<1, KT, 0, st>>>(part + int64_t(nv) * KNB, KNB, state + 0, 0, nullptr);
+    // w -= V h, |w'|^2
+    k_update<<<KNB, KT, 0, st>>>(d_V, ld, nv, d_hcol, d_w, n, part2, nullptr);
+    k_reduce<<<1, KT, 0, st>>>(part2, KNB, state + 1, 0, nullptr);
+    k_reorth_test<<<1, 1, 0, st>>>(state, flag);
+    // optional pass 2 (skipped on the device when flag == 1)
+    k_mdot<<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, part, 0, flag);
+    // accumulate the correction into hcol, keep it separately for the update
+    double *hcorr = part + int64_t(KMAXV) * KNB;            // nv doubles, after the partials
+    k_reduce<<<nv, KT, 0, st>>>(part, KNB, hcorr, 0, flag);
+    k_update<<<KNB, KT, 0, st>>>(d_V, ld, nv, hcorr, d_w, n, part2, flag);
+    k_reduce<<<1, KT, 0, st>>>(part2, KNB, state + 1, 0, flag);
+    k_reduce<<<nv, KT, 0, st>>>(hcorr, 1, d_hcol, 1, flag);  // hcol += hcorr
+    k_normalize<<<2048, KT, 0, st>>>(d_w, V_out, n, state + 1, d_hcol, nv);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+/* d_out[j] = V_j . w for j < nv (device), workspace as for rtx_arnoldi_step. */
+extern "C" int rtx_mdot(const double *d_V, int64_t ld, int nv, int64_t n, const double *d_w,
+                        double *d_out, double *d_work, void *stream) {
+    RTX_CHECK(d_V && d_w && d_out && d_work && nv >= 1 && nv < KMAXV, "rtx_mdot: bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int nchunk = (nv + KCH - 1) / KCH;
+    k_mdot<<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, d_work, 0, nullptr);
+    k_reduce<<<nv, KT, 0, st>>>(d_work, KNB, d_out, 0, nullptr);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+/* d_x (+)= sum_j y_j V_j with y on the device. */
+extern "C" int rtx_combine(const double *d_V, int64_t ld, int nv, const double *d_y, int64_t n,
+                           double *d_x, int accumulate, void *stream) {
+    RTX_CHECK(d_V && d_y && d_x && nv >= 1 && nv < KMAXV, "rtx_combine: bad argument");
+    k_combine<<<2048, KT, 0, static_cast<hipStream_t>(stream)>>>(d_V, ld, nv, d_y, d_x, n,
+                                                               accumulate);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+/* y = a x + b y for FP64 (precision 0) or FP32 (precision 1) vectors. */
+extern "C" int rtx_axpby(int precision, int64_t n, double a, const void *d_x, double b,
+                         void *d_y, void *stream) {
+    RTX_CHECK(d_x && d_y && n >= 0, "rtx_axpby: bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (precision == RTX_PREC_F64)
+        k_axpby<double><<<2048, 256, 0, st>>>(n, a, (const double *)d_x, b, (double *)d_y);
+    else
+        k_axpby<float><<<2048, 256, 0, st>>>(n, a, (const float *)d_x, b, (float *)d_y);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+/* Precision conversion FP64 <-> FP32 (direction 0: f64 -> f32, 1: f32 -> f64). */
+extern "C" int rtx_convert(int direction, int64_t n, const void *d_src, void *d_dst,
+                           void *stream) {
+    RTX_CHECK(d_src && d_dst && n >= 0, "rtx_convert: bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (direction == 0)
+        k_convert<double, float><<<2048, 256, 0, st>>>(n, (const double *)d_src, (float *)d_dst);
+    else
+        k_convert<float, double><<<2048, 256, 0, st>>>(n, (const float *)d_src, (double *)d_dst);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}

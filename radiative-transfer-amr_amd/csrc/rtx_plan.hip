@@ -195,6 +195,7 @@ extern "C" void rtx_plan_destroy(rtx_plan *p) {
     (void)hipSetDevice(p->device);
     if (p->arena) (void)hipFree(p->arena);
     if (p->coef) (void)hipFree(p->coef);
+    if (p->bj_off) (void)hipFree(p->bj_off);
     (void)hipSetDevice(cur);
     delete p;
 }

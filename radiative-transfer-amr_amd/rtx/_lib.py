@@ -51,6 +51,21 @@ SIGNATURES = [
     ("rtx_plan_destroy", None, [C.c_void_p]),
     ("rtx_plan_set_coeffs", C.c_int, [C.c_void_p, C.c_int64, _f64p, _f64p, C.c_int64, _f64p]),
     ("rtx_apply", C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtx_bjacobi_size", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("rtx_bjacobi_factor", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
+    ("rtx_bjacobi_apply", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]),
+    ("rtx_arnoldi_workspace", C.c_int, [C.c_int, C.POINTER(C.c_int64)]),
+    ("rtx_arnoldi_step", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_int64, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtx_mdot", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_int64, C.c_void_p, C.c_void_p,
+                           C.c_void_p, C.c_void_p]),
+    ("rtx_combine", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64,
+                              C.c_void_p, C.c_int, C.c_void_p]),
+    ("rtx_axpby", C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_void_p, C.c_double,
+                            C.c_void_p, C.c_void_p]),
+    ("rtx_convert", C.c_int, [C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
 ]
 
 

@@ -1,0 +1,279 @@
+"""Device-resident GMRES, per-cell block-Jacobi and FP64 iterative refinement.
+
+Counterpart of the PETSc KSP solve of ``src/rt/solve.py:81-117`` (GMRES with
+restart 50 and a block-Jacobi preconditioner).  Differences that are design
+choices, not accidents:
+
+* right preconditioning, so the GMRES residual is the true residual of the
+  interior system (PETSc's default is left preconditioning);
+* the Krylov basis, the Arnoldi orthogonalisation (classical Gram-Schmidt
+  with one conditional re-orthogonalisation), the solution update and the
+  true residual are FP64 on the device; only the small Hessenberg least
+  squares problem (at most (m + 1) x m) is solved on the host;
+* the block-Jacobi blocks are the angular cells (``BlockJacobi``);
+* ``refine`` wraps an inexact inner GMRES that applies the operator and the
+  preconditioner in FP32 / BF16 inside an FP64 residual loop.
+
+Vectors are full-length (every DOF); inflow-boundary DOFs are kept at zero,
+which makes the iteration identical to one on the compressed interior
+system A_II of ``split_matrix`` (``src/dg/matrix/split_matrix.py:46-47``).
+"""
+import ctypes as C
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, PREC_F64, PREC_F32, PREC_MIXED, OP_SYSTEM, OP_RESTRICT
+from .operator import stream_handle
+
+
+def _vp(t):
+    return C.c_void_p(t.data_ptr())
+
+
+class BlockJacobi:
+    """Dense inverses of every cell's interior diagonal block, on the device."""
+
+    def __init__(self, op, restrict=True, keep_f32=False, keep_f64=True):
+        self.op, self.lib = op, op.lib
+        n = C.c_int64()
+        check(self.lib.rtx_bjacobi_size(op.plan, C.byref(n)), "rtx_bjacobi_size")
+        self.entries = int(n.value)
+        dev = op.device
+        self.inv = torch.empty(self.entries, dtype=torch.float64, device=dev)
+        self.inv32 = torch.empty(self.entries, dtype=torch.float32, device=dev) if keep_f32 else None
+        st = stream_handle()
+        rc = self.lib.rtx_bjacobi_factor(op.plan, 1 if restrict else 0, _vp(self.inv),
+                                         _vp(self.inv32) if keep_f32 else None, None, st)
+        if rc == -1 and b"LDS" in self.lib.rtx_last_error():
+            scratch = torch.empty_like(self.inv)
+            rc = self.lib.rtx_bjacobi_factor(op.plan, 1 if restrict else 0, _vp(self.inv),
+                                             _vp(self.inv32) if keep_f32 else None,
+                                             _vp(scratch), st)
+            del scratch
+        check(rc, "rtx_bjacobi_factor")
+        if not keep_f64:
+            self.inv = None
+
+    @property
+    def bytes_per_apply(self):
+        return self.entries * (8 if self.inv is not None else 4)
+
+    def apply(self, r, z, precision=PREC_F64, stream=None):
+        inv = self.inv if precision == PREC_F64 else self.inv32
+        if inv is None:
+            raise ValueError("block-Jacobi inverses for this precision were not kept")
+        check(self.lib.rtx_bjacobi_apply(self.op.plan, PREC_F64 if precision == PREC_F64 else PREC_F32,
+                                         _vp(inv), _vp(r), _vp(z), stream_handle(stream)),
+              "rtx_bjacobi_apply")
+        return z
+
+
+class InteriorSystem:
+    """y = A_II x in full-length FP64 vectors, the apply done in ``precision``."""
+
+    def __init__(self, op, precision=PREC_F64, pc=None, pc_precision=None):
+        self.op, self.lib, self.precision = op, op.lib, precision
+        self.pc = pc
+        self.pc_precision = precision if pc_precision is None else pc_precision
+        n = op.ndof
+        dev = op.device
+        if precision != PREC_F64 or self.pc_precision != PREC_F64:
+            self.x32 = torch.empty(n, dtype=torch.float32, device=dev)
+            self.y32 = torch.empty(n, dtype=torch.float32, device=dev)
+
+    def matvec(self, x, y):
+        if self.precision == PREC_F64:
+            self.op.apply(x, y, ops=OP_SYSTEM | OP_RESTRICT, precision=PREC_F64)
+        else:
+            st = stream_handle()
+            check(self.lib.rtx_convert(0, x.numel(), _vp(x), _vp(self.x32), st), "convert")
+            self.op.apply(self.x32, self.y32, ops=OP_SYSTEM | OP_RESTRICT, precision=self.precision)
+            check(self.lib.rtx_convert(1, x.numel(), _vp(self.y32), _vp(y), st), "convert")
+        return y
+
+    def precondition(self, r, z):
+        if self.pc is None:
+            z.copy_(r)
+            return z
+        if self.pc_precision == PREC_F64:
+            return self.pc.apply(r, z, PREC_F64)
+        st = stream_handle()
+        check(self.lib.rtx_convert(0, r.numel(), _vp(r), _vp(self.x32), st), "convert")
+        self.pc.apply(self.x32, self.y32, PREC_F32)
+        check(self.lib.rtx_convert(1, r.numel(), _vp(self.y32), _vp(z), st), "convert")
+        return z
+
+    def residual(self, b, x, r):
+        """r = b - A_II x (always FP64)."""
+        self.op.apply(x, r, ops=OP_SYSTEM | OP_RESTRICT, precision=PREC_F64)
+        check(self.lib.rtx_axpby(PREC_F64, b.numel(), 1.0, _vp(b), -1.0, _vp(r), stream_handle()),
+              "axpby")
+        return r
+
+
+class GMRESResult(dict):
+    pass
+
+
+def _norm(lib, v, work, out):
+    check(lib.rtx_mdot(_vp(v), v.numel(), 1, v.numel(), _vp(v), _vp(out), _vp(work),
+                       stream_handle()), "rtx_mdot")
+    return float(out[0].item()) ** 0.5
+
+
+def gmres(system, b, x=None, rtol=1e-8, atol=0.0, restart=50, max_it=5000,
+          flexible=False, callback=None, true_residual_every_restart=True):
+    """Right-preconditioned restarted GMRES on ``system`` (InteriorSystem).
+
+    Converged when |b - A x| <= max(rtol |b|, atol) for the FP64 true
+    residual.  Returns (x, info) with PETSc-like convergence information."""
+    lib = system.lib
+    n = b.numel()
+    dev = b.device
+    m = int(restart)
+    x = torch.zeros_like(b) if x is None else x
+    V = torch.empty((m + 1, n), dtype=torch.float64, device=dev)
+    Z = torch.empty((m, n), dtype=torch.float64, device=dev) if flexible else None
+    w = torch.empty(n, dtype=torch.float64, device=dev)
+    z = torch.empty(n, dtype=torch.float64, device=dev)
+    r = torch.empty(n, dtype=torch.float64, device=dev)
+    ws = C.c_int64()
+    check(lib.rtx_arnoldi_workspace(m + 1, C.byref(ws)), "workspace")
+    work = torch.empty(int(ws.value), dtype=torch.float64, device=dev)
+    hcol_d = torch.zeros(m + 2, dtype=torch.float64, device=dev)
+    hcol_h = torch.zeros(m + 2, dtype=torch.float64).pin_memory()
+    ydev = torch.empty(m, dtype=torch.float64, device=dev)
+    scal = torch.empty(2, dtype=torch.float64, device=dev)
+    st = stream_handle()
+    stream = torch.cuda.current_stream()
+
+    bnorm = _norm(lib, b, work, scal)
+    target = max(rtol * bnorm, atol)
+    history, true_hist = [], []
+    its = 0
+    reason = 0
+    H_last = None
+    t0 = time.perf_counter()
+    while True:
+        system.residual(b, x, r)
+        beta = _norm(lib, r, work, scal)
+        true_hist.append(beta)
+        if its == 0:
+            history.append(beta)
+        if beta <= target:
+            reason = 2 if beta <= rtol * bnorm else 3     # KSP_CONVERGED_RTOL / ATOL
+            break
+        if its >= max_it:
+            reason = -3                                   # KSP_DIVERGED_ITS
+            break
+        if not np.isfinite(beta):
+            reason = -9                                   # KSP_DIVERGED_NANORINF
+            break
+        check(lib.rtx_axpby(PREC_F64, n, 1.0 / beta, _vp(r), 0.0, _vp(V[0]), st), "axpby")
+        H = np.zeros((m + 1, m))          # raw Arnoldi (Hessenberg) matrix
+        Rf = np.zeros((m, m))             # its triangular factor
+        cs, sn = np.zeros(m), np.zeros(m)
+        g = np.zeros(m + 1)
+        g[0] = beta
+        k = 0
+        for j in range(m):
+            zj = Z[j] if flexible else z
+            system.precondition(V[j], zj)
+            system.matvec(zj, w)
+            check(lib.rtx_arnoldi_step(_vp(V), n, j + 1, n, _vp(w), _vp(hcol_d), _vp(work), st),
+                  "rtx_arnoldi_step")
+            hcol_h[:j + 2].copy_(hcol_d[:j + 2], non_blocking=True)
+            stream.synchronize()
+            h = hcol_h[:j + 2].numpy().copy()
+            H[:j + 2, j] = h
+            for i in range(j):
+                t = cs[i] * h[i] + sn[i] * h[i + 1]
+                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1]
+                h[i] = t
+            den = np.hypot(h[j], h[j + 1])
+            if den == 0.0:
+                cs[j], sn[j] = 1.0, 0.0
+            else:
+                cs[j], sn[j] = h[j] / den, h[j + 1] / den
+            h[j] = den
+            h[j + 1] = 0.0
+            Rf[:j + 1, j] = h[:j + 1]
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            k = j + 1
+            its += 1
+            est = abs(g[j + 1])
+            history.append(est)
+            if callback is not None:
+                callback(its, est)
+            if est <= target or its >= max_it or hcol_h[j + 1] == 0.0:
+                break
+        H_last = (H, k)
+        # solve R y = g and update x
+        y = np.linalg.solve(np.triu(Rf[:k, :k]), g[:k]) if k > 0 else np.zeros(0)
+        ydev[:k].copy_(torch.from_numpy(y))
+        if flexible:
+            check(lib.rtx_combine(_vp(Z), n, k, _vp(ydev), n, _vp(x), 1, st), "combine")
+        else:
+            check(lib.rtx_combine(_vp(V), n, k, _vp(ydev), n, _vp(w), 0, st), "combine")
+            system.precondition(w, z)
+            check(lib.rtx_axpby(PREC_F64, n, 1.0, _vp(z), 1.0, _vp(x), st), "axpby")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    sv = hessenberg_singular_values(*H_last) if H_last is not None else [float("nan")] * 2
+    info = GMRESResult(converged_reason=reason, convergence_history=history,
+                       iteration_number=its, res_f=true_hist[-1], res_best=min(true_hist),
+                       true_residual_history=true_hist, rtol=rtol, atol=atol,
+                       divtol=float(np.sqrt(np.finfo(np.float64).max)), max_it=max_it,
+                       bnorm=bnorm, rel_residual=true_hist[-1] / bnorm if bnorm > 0 else 0.0,
+                       seconds=elapsed, restart=m, extreme_singular_values=sv)
+    return x, info
+
+
+def hessenberg_singular_values(H, k):
+    """Extreme singular values of the (k+1) x k Arnoldi matrix (PETSc's
+    KSPComputeExtremeSingularValues estimates them the same way)."""
+    if k == 0:
+        return [float("nan"), float("nan")]
+    s = np.linalg.svd(H[:k + 1, :k], compute_uv=False)
+    return [float(s.max()), float(s.min())]
+
+
+def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=50,
+           max_outer=50, max_inner=200):
+    """FP64 iterative refinement: r = b - A x in FP64 (``system_hi``), inner
+    correction solve A d = r by GMRES whose operator and preconditioner run in
+    reduced precision (``system_lo``; its Arnoldi is still FP64)."""
+    lib = system_hi.lib
+    n = b.numel()
+    x = torch.zeros_like(b) if x is None else x
+    r = torch.empty_like(b)
+    work_n = C.c_int64()
+    check(lib.rtx_arnoldi_workspace(2, C.byref(work_n)), "workspace")
+    work = torch.empty(int(work_n.value), dtype=torch.float64, device=b.device)
+    scal = torch.empty(2, dtype=torch.float64, device=b.device)
+    bnorm = _norm(lib, b, work, scal)
+    hist, inner_its = [], []
+    t0 = time.perf_counter()
+    reason = -3
+    for outer in range(max_outer + 1):
+        system_hi.residual(b, x, r)
+        rn = _norm(lib, r, work, scal)
+        hist.append(rn)
+        if rn <= rtol * bnorm:
+            reason = 2
+            break
+        if outer == max_outer or not np.isfinite(rn):
+            reason = -3 if np.isfinite(rn) else -9
+            break
+        d, info = gmres(system_lo, r, None, rtol=inner_rtol, restart=restart, max_it=max_inner)
+        inner_its.append(info["iteration_number"])
+        check(lib.rtx_axpby(PREC_F64, n, 1.0, _vp(d), 1.0, _vp(x), stream_handle()), "axpby")
+    torch.cuda.synchronize()
+    return x, dict(converged_reason=reason, residual_history=hist, inner_iterations=inner_its,
+                   iteration_number=int(sum(inner_its)), res_f=hist[-1], bnorm=bnorm,
+                   rel_residual=hist[-1] / bnorm if bnorm > 0 else 0.0,
+                   seconds=time.perf_counter() - t0)
