@@ -1,0 +1,214 @@
+"""Benchmark: DG transport operator apply and time-to-1e-8 solve on one GPU.
+
+Workload (uniform absorber, "C2"): [0,1]^2 x [0, 2 pi), 32 x 32 spatial
+columns x 16 angular cells, 5 nodes per direction (p = 4): 2,048,000 DOF;
+kappa = 1 + 0.5 sin(2 pi x) sin(2 pi y), sigma = 0.7 kappa, Henyey-Greenstein
+g = 0.8, unit isotropic inflow on the left wall, no forcing.  Data are
+synthetic (seeded random operand vectors) on the deterministic mesh.
+
+A "step" is one application of the interior transport operator A_II
+(mass + interior convection + upwind boundary convection + scattering,
+FP64) to a vector already resident in HBM.  ``value`` is DOF/s per apply.
+After the timed steps the script also solves A_II x = rhs to a true FP64
+relative residual of 1e-8 with (a) plain FP64 GMRES + block-Jacobi and
+(b) FP64 iterative refinement around reduced-precision applies, and times
+the oracle's assembled CSR SpMV on a bounded sample on the host CPU.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve]
+                       [--no-cpu] [--calib]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "radiative-transfer-amr_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="run one plain 1 GiB device copy after the timed region "
+                         "(counter calibration for rocprofv3 --pmc)")
+    return ap.parse_args()
+
+
+def workload():
+    from rtx import problems
+    mesh = problems.uniform_mesh([1.0, 1.0], [5, 5, 5], nref_ang=4, nref_spt=5)
+    return mesh, problems.absorber_c2()
+
+
+def algorithmic(op):
+    """Bytes and flops of one FP64 apply (read x, write y, read kappa and
+    sigma at the spatial nodes; scattering 2 N_sp Nth^2 per column, volume
+    and face terms)."""
+    t = op.t
+    import numpy as np
+    nsp = int(t.nsp)
+    bytes_ = 16 * t.ndof + 16 * nsp
+    nang = t.col_nang.astype(np.int64)
+    nsp_c = (t.col_nx * t.col_ny).astype(np.int64)
+    scat = int((2 * nsp_c * nang * nang).sum())
+    cell_n = (t.col_nx[t.cell_col] * t.col_ny[t.cell_col] * t.cell_nth).astype(np.int64)
+    vol = int((cell_n * (2 * (t.col_nx[t.cell_col] + t.col_ny[t.cell_col]) + 8)).sum())
+    face = 0
+    for l in range(0, t.nlink):
+        k, s, F = t.link_target[l], t.link_src[l], t.link_face[l]
+        n0 = t.col_nx[t.cell_col[k]] if F % 2 else t.col_ny[t.cell_col[k]]
+        n1 = t.col_nx[t.cell_col[s]] if F % 2 else t.col_ny[t.cell_col[s]]
+        face += 2 * (int(n1) * int(t.cell_nth[s]) * int(t.cell_nth[k]) + int(n1) * int(n0) * int(t.cell_nth[k]))
+    return bytes_, scat + vol + face
+
+
+def cpu_baseline(mesh, wl, target_s=12.0):
+    """Oracle assembled CSR SpMV (scipy, one thread) on the rows of a sample
+    of columns; DOF/s = rows / time per SpMV."""
+    import numpy as np
+    from oracle import assemble as asm
+    keys = sorted(mesh.cols)
+    sample = keys[:: max(1, len(keys) // 32)][:32]
+    t0 = time.perf_counter()
+    V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi, cols=sample)
+    A = asm.system_matrix(V)
+    rows = np.unique(A.nonzero()[0])
+    A = A[rows].tocsr()
+    t_asm = time.perf_counter() - t0
+    x = np.random.default_rng(0).uniform(-1, 1, A.shape[1])
+    A @ x
+    reps, t = 0, 0.0
+    t0 = time.perf_counter()
+    while t < target_s * 0.25 and reps < 1000:
+        A @ x
+        reps += 1
+        t = time.perf_counter() - t0
+    per = t / reps
+    return {"value": rows.size / per, "unit": "DOF/s", "cores": 1, "kind": "port",
+            "sample": "rows of {} of {} columns ({} rows, {} nnz) of the oracle's assembled "
+                      "FP64 CSR A = (BC - IC) + M + S, scipy SpMV, {} reps; assembly of the "
+                      "sample took {:.1f} s".format(len(sample), len(keys), rows.size, A.nnz,
+                                                    reps, t_asm)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(--gpus {} requested)".format(args.gpus), file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from rtx.operator import TransportOperator, OP_SYSTEM, OP_RESTRICT, PREC_F64, PREC_F32, PREC_MIXED
+    from rtx import krylov
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    mesh, wl = workload()
+    t0 = time.perf_counter()
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi, device=dev)
+    t_plan = time.perf_counter() - t0
+    n = op.ndof
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    x = torch.rand(n, dtype=torch.float64, device=dev, generator=gen) * 2 - 1
+    op.mask(x)
+    y = torch.empty_like(x)
+    ops = OP_SYSTEM | OP_RESTRICT
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        op.apply(x, y, ops=ops)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_wall0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        op.apply(x, y, ops=ops)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall0
+    ms = ev0.elapsed_time(ev1) / args.steps
+    bytes_, flops = algorithmic(op)
+    achieved = bytes_ / (ms * 1e-3) / 1e9
+    if args.calib:
+        big = torch.empty(1 << 27, dtype=torch.float64, device=dev)   # 1 GiB
+        big2 = torch.empty_like(big)
+        big2.copy_(big)
+        torch.cuda.synchronize()
+
+    out = {
+        "metric": "time-to-1e-8 residual & DOF/s per apply: learned+mixed vs plain FP64, 1 GPU",
+        "value": n / (ms * 1e-3), "unit": "DOF/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "uniform absorber 32x32x16 cells, p=4 (2,048,000 DOF), HG g=0.8, "
+                               "sigma/kappa=0.7", "ndof": n, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "operator apply (A_II, FP64)",
+                     "algorithmic_bytes": bytes_, "algorithmic_flops": flops,
+                     "fp64_tflops": flops / (ms * 1e-3) / 1e12,
+                     "fp64_frac": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
+        "plan_seconds": t_plan, "timed_wall_seconds": t_wall,
+    }
+    if not args.no_solve:
+        out["solve"] = solve_section(op, mesh, wl)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(mesh, wl)
+    print(json.dumps(out), flush=True)
+
+
+def solve_section(op, mesh, wl):
+    """Time-to-1e-8 true FP64 relative residual on the workload's interior
+    system, rhs = f_I - A_IB b_B as in rt.Problem.solve."""
+    import torch
+    from rtx.operator import OP_SYSTEM, PREC_F64, PREC_F32, PREC_MIXED
+    from rtx import krylov
+    dev = op.device
+    res = {}
+    f = torch.from_numpy(op.nodal(wl.f)).to(dev)
+    from rtx.operator import OP_QUADW
+    f = op.apply(f, ops=OP_QUADW)
+    b = torch.from_numpy(op.nodal(wl.bcs)).to(dev)
+    bb = op.mask(b.clone(), keep_interior=False)
+    rhs = op.mask(f - op.apply(bb, ops=OP_SYSTEM), keep_interior=True)
+    for name, prec in (("fp64", PREC_F64), ("mixed", PREC_MIXED)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pc = krylov.BlockJacobi(op, keep_f32=prec != PREC_F64, keep_f64=prec == PREC_F64)
+        torch.cuda.synchronize()
+        t_pc = time.perf_counter() - t0
+        hi = krylov.InteriorSystem(op, PREC_F64, pc if prec == PREC_F64 else None, PREC_F64)
+        if prec == PREC_F64:
+            x, info = krylov.gmres(hi, rhs, None, rtol=1e-8, restart=50, max_it=3000)
+        else:
+            lo = krylov.InteriorSystem(op, prec, pc, PREC_F32)
+            x, info = krylov.refine(hi, lo, rhs, None, rtol=1e-8, inner_rtol=1e-3,
+                                    restart=50, max_inner=3000)
+        torch.cuda.synchronize()
+        total = time.perf_counter() - t0
+        res[name] = {"seconds": total, "pc_setup_seconds": t_pc,
+                     "iterations": int(info["iteration_number"]),
+                     "rel_residual": float(info["rel_residual"]),
+                     "converged": bool(info["converged_reason"] > 0)}
+        if name == "fp64":
+            x64 = x.clone()
+        else:
+            res[name]["rel_l2_vs_fp64"] = float((x - x64).norm() / x64.norm())
+        del pc
+    res["speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
+    return res
+
+
+if __name__ == "__main__":
+    main()

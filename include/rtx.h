@@ -37,7 +37,9 @@ enum {
     RTX_OP_BCONV = 8u,     /* +BC    : src/rt/boundary_convection_matrix.py:258-537  */
     RTX_OP_RESTRICT = 16u, /* zero x and y on inflow-boundary DOFs: P_I A P_I^T,
                               src/dg/matrix/get_masks.py:80-110, split_matrix.py:46 */
-    RTX_OP_ACCUM = 32u     /* y += op(x) instead of y = op(x)                      */
+    RTX_OP_ACCUM = 32u,    /* y += op(x) instead of y = op(x)                      */
+    RTX_OP_QUADW = 64u     /* quadrature weights (dx dy dth / 8) w_i w_j w_a: the
+                              forcing-vector weighting, src/rt/forcing_vector.py:141-153 */
 };
 #define RTX_OP_SYSTEM (RTX_OP_MASS | RTX_OP_SCAT | RTX_OP_ICONV | RTX_OP_BCONV)
 
@@ -85,6 +87,7 @@ typedef struct {
     const int32_t *link_face;       /* face F of the target cell                   */
     const int32_t *link_Es_off;     /* E^x or E^y (n1 x n0, row-major) in Es pool  */
     const int32_t *link_Eth_off;    /* E^theta (nth1 x nth0, row-major)            */
+    const int32_t *link_target;     /* target cell of each link                    */
     int64_t nEs, nEth;
     const double *Es_pool, *Eth_pool;
     /* 1-D rules: for n = 1..RTX_NMAX, index (n * RTX_NMAX + k):
@@ -116,6 +119,29 @@ int rtx_plan_set_coeffs(rtx_plan *plan, int64_t nsp, const double *kappa,
    (BC - IC) + (M + S_ret) of src/rt/solve.py:52. */
 int rtx_apply(rtx_plan *plan, uint32_t op_mask, int precision, const void *d_x,
               void *d_y, void *stream);
+
+/* Apply path: -1 automatic (default: the fused per-column kernel whenever
+   every column fits it, else the generic per-cell/per-column kernels), 0
+   generic, 1 fused (rtx_apply then fails if a column does not fit). */
+int rtx_set_path(rtx_plan *plan, int mode);
+
+/* In-place mask of a full-length vector: keep = 0 zeroes the inflow-boundary
+   DOFs (keeps the interior, split_vector.py:39-46), keep = 1 zeroes the
+   interior DOFs (keeps the boundary part). */
+int rtx_mask(rtx_plan *plan, int keep, int precision, void *d_x, void *stream);
+
+/* Values of one operator part at given positions (the assembled-matrix
+   methods of src/rt/*.py).  part: 0 mass (mass_matrix.py), 1 returned
+   scattering -S (scattering_matrix.py), 2 interior convection IC
+   (interior_convection_matrix.py), 3 boundary convection BC
+   (boundary_convection_matrix.py), 4 M - S (preconditioner_matrix.py).
+   Entry e sits at (row cell d_kr[e], row local DOF d_lr[e]) x (column cell
+   d_kc[e], column local DOF d_lc[e]); for BC, d_link[e] is the upwind link
+   or -1 for the outflow own-trace term.  The reference's drop thresholds
+   are applied (dropped entries are written as 0). */
+int rtx_assemble_entries(rtx_plan *plan, int part, int64_t n, const int32_t *d_kr,
+                         const int32_t *d_lr, const int32_t *d_kc, const int32_t *d_lc,
+                         const int32_t *d_link, double *d_val, void *stream);
 
 /* ---- per-cell block-Jacobi (PETSc PCBJACOBI of src/rt/solve.py:93-95; the
    reference's discarded dense column inverse, preconditioner_matrix.py:259-262,

@@ -79,6 +79,7 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
         if (ops & RTX_OP_MASS)
             acc += (dx * dy * dth * 0.125) * wx[p] * wy[q] * wt[r] * m.kappa[sp0 + p * ny + q] *
                    xs[t];
+        if (ops & RTX_OP_QUADW) acc += (dx * dy * dth * 0.125) * wx[p] * wy[q] * wt[r] * xs[t];
         if (ops & RTX_OP_ICONV) {
             double sx = 0.0, sy = 0.0;
             for (int i = 0; i < nx; ++i) sx += wx[i] * Dx[p * NM + i] * xs[i * nyt + q * nth + r];
@@ -225,12 +226,27 @@ __global__ void k_zero(T *y, int64_t n) {
     for (; i < n; i += int64_t(gridDim.x) * blockDim.x) y[i] = T(0);
 }
 
+// keep == 0: zero inflow-boundary DOFs; keep == 1: zero interior DOFs.
+template <typename T>
+__global__ __launch_bounds__(64) void k_mask(DevMesh m, int keep, T *__restrict__ x) {
+    const int k = blockIdx.x;
+    const int c = m.cell_col[k];
+    const int nx = m.col_nx[c], ny = m.col_ny[c], nth = m.cell_nth[k];
+    const int bd = m.cell_bdry[k];
+    const int off = m.cell_dof_off[k];
+    for (int t = threadIdx.x; t < nx * ny * nth; t += 64) {
+        const int i = t / (ny * nth), j = (t / nth) % ny;
+        const bool b = bd && on_bdry(bd, i, j, nx, ny);
+        if (b == (keep == 0)) x[off + t] = T(0);
+    }
+}
+
 template <typename T>
 static int launch_apply(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t st) {
     const T *x = static_cast<const T *>(dx);
     T *y = static_cast<T *>(dy);
     const bool acc = (ops & RTX_OP_ACCUM) != 0;
-    const bool tp = (ops & (RTX_OP_MASS | RTX_OP_ICONV | RTX_OP_BCONV)) != 0;
+    const bool tp = (ops & (RTX_OP_MASS | RTX_OP_ICONV | RTX_OP_BCONV | RTX_OP_QUADW)) != 0;
     const bool sc = (ops & RTX_OP_SCAT) != 0;
     if (!tp && !sc) {
         if (!acc) k_zero<T><<<1024, 256, 0, st>>>(y, p->m.ndof);
@@ -243,9 +259,27 @@ static int launch_apply(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hip
     return RTX_OK;
 }
 
+// The fused per-column kernel holds a whole column (values and accumulators)
+// in LDS and tiles its spatial nodes by 16 up to 7 tiles.
+constexpr size_t FUSED_LDS_CAP = 96 * 1024;
+
+static bool fused_fits(const rtx_plan *p, int precision) {
+    return p->max_col_cells <= 64 && p->max_nsp <= 112 && p->max_col_links <= 256 &&
+           fused_lds_bytes(p->max_col_dofs, p->max_nang, p->max_col_trace, precision) <=
+               FUSED_LDS_CAP;
+}
+
 }  // namespace rtx
 
 using namespace rtx;
+
+/* Select the apply path: -1 automatic (fused when every column fits), 0 the
+   generic per-cell + per-column kernels, 1 the fused per-column kernel. */
+extern "C" int rtx_set_path(rtx_plan *p, int mode) {
+    RTX_CHECK(p && mode >= -1 && mode <= 1, "rtx_set_path: bad argument");
+    p->fused_mode = mode;
+    return RTX_OK;
+}
 
 extern "C" int rtx_apply(rtx_plan *p, uint32_t op_mask, int precision, const void *d_x,
                          void *d_y, void *stream) {
@@ -258,9 +292,28 @@ extern "C" int rtx_apply(rtx_plan *p, uint32_t op_mask, int precision, const voi
     RTX_CHECK(p->max_nang <= MAX_NANG, "rtx_apply: a column has %d angular nodes (max %d)",
               p->max_nang, MAX_NANG);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    RTX_CHECK(precision >= RTX_PREC_F64 && precision <= RTX_PREC_MIXED,
+              "rtx_apply: unsupported precision %d", precision);
+    if (p->fused_mode != 0 && fused_fits(p, precision))
+        return launch_fused(p, op_mask, precision, d_x, d_y, st, (op_mask & RTX_OP_ACCUM) ? 1 : 0);
+    RTX_CHECK(p->fused_mode != 1, "rtx_apply: the fused per-column path was requested but a "
+                                  "column does not fit it (DOFs %d, cells %d, nodes %d)",
+              p->max_col_dofs, p->max_col_cells, p->max_nsp);
     switch (precision) {
         case RTX_PREC_F64: return launch_apply<double>(p, op_mask, d_x, d_y, st);
         case RTX_PREC_F32: return launch_apply<float>(p, op_mask, d_x, d_y, st);
+        case RTX_PREC_MIXED: return launch_apply<float>(p, op_mask, d_x, d_y, st);
         default: set_error("rtx_apply: unsupported precision %d", precision); return RTX_ERR_ARG;
     }
+}
+
+extern "C" int rtx_mask(rtx_plan *p, int keep, int precision, void *d_x, void *stream) {
+    RTX_CHECK(p && d_x && (keep == 0 || keep == 1), "rtx_mask: bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (precision == RTX_PREC_F64)
+        k_mask<double><<<p->m.ncell, 64, 0, st>>>(p->m, keep, static_cast<double *>(d_x));
+    else
+        k_mask<float><<<p->m.ncell, 64, 0, st>>>(p->m, keep, static_cast<float *>(d_x));
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
 }

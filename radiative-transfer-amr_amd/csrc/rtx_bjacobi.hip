@@ -6,8 +6,12 @@
 // Here the blocks are the angular cells: the cell's diagonal block of the
 // interior system A_II (mass, interior convection, outflow faces, in-cell
 // scattering and any self-coupling link), identity on inflow-boundary DOFs.
-// Each block is assembled on the device, inverted by Gauss-Jordan with
-// partial pivoting (in LDS when it fits, else in place in HBM), and stored
+//
+// Build: one workgroup per cell assembles the block from its structure
+// (per-node angular scattering blocks, the diagonal, the x/y derivative
+// lines) in LDS (or in a global scratch block when it does not fit), inverts
+// it by Gauss-Jordan with partial pivoting (column-parallel elimination, one
+// pivot search per wavefront-shuffle reduction), and stores the inverse
 // column-major so the apply is a coalesced batched GEMV.
 #include "rtx_internal.h"
 
@@ -15,169 +19,201 @@ namespace rtx {
 
 constexpr int NMq = RTX_NMAX;
 constexpr int BJ_THREADS = 256;
+constexpr int BJ_COLS = 128;     // column threads in the elimination
 
 __device__ inline bool bj_on_bdry(int bd, int i, int j, int nx, int ny) {
     return ((bd & 1) && i == nx - 1) || ((bd & 2) && j == ny - 1) || ((bd & 4) && i == 0) ||
            ((bd & 8) && j == 0);
 }
 
-// Entry (row (p, q, r), column (i, j, a)) of the cell's diagonal block.
-__device__ double block_entry(const DevMesh &m, int k, int row, int col, const double *cth,
-                              const double *sth) {
+// Assemble the cell's diagonal block into row-major A (n x n).
+__device__ void bj_assemble(const DevMesh &m, int k, int restrict_, double *A, const double *cth,
+                            const double *sth) {
+    const int tid = threadIdx.x;
     const int c = m.cell_col[k];
     const int nx = m.col_nx[c], ny = m.col_ny[c], nth = m.cell_nth[k];
-    const int nyt = ny * nth;
-    const int p = row / nyt, q = (row / nth) % ny, r = row % nth;
-    const int i = col / nyt, j = (col / nth) % ny, a = col % nth;
-    const double x0 = m.col_geom[4 * c + 0], y0 = m.col_geom[4 * c + 1];
-    const double dx = m.col_geom[4 * c + 2] - x0, dy = m.col_geom[4 * c + 3] - y0;
-    const double th0 = m.cell_geom[2 * k + 0], dth = m.cell_geom[2 * k + 1] - th0;
+    const int n = nx * ny * nth, nyt = ny * nth, nsp = nx * ny;
+    const double dx = m.col_geom[4 * c + 2] - m.col_geom[4 * c + 0];
+    const double dy = m.col_geom[4 * c + 3] - m.col_geom[4 * c + 1];
+    const double th0 = m.cell_geom[2 * k], dth = m.cell_geom[2 * k + 1] - th0;
     const double *wx = m.lgl_w + nx * NMq, *wy = m.lgl_w + ny * NMq, *wt = m.lg_w + nth * NMq;
-    double v = 0.0;
-    if (p == i && q == j) {
-        // in-cell scattering: -dx dy w_p w_q sigma G'[(K, r), (K, a)]
-        const int ao = m.cell_ang_off[k], na = m.col_nang[c];
-        const double g = m.G[m.col_G_off[c] + size_t(ao + a) * na + (ao + r)];
-        v -= dx * dy * wx[p] * wy[q] * m.sigma[m.col_sp_off[c] + p * ny + q] * g;
-        if (r == a) {
-            v += (dx * dy * dth * 0.125) * wx[p] * wy[q] * wt[r] *
-                 m.kappa[m.col_sp_off[c] + p * ny + q];
-            const int quad = m.cell_quad[k];
-            if (quad >= 0) {
-                for (int h = 0; h < 2; ++h) {
-                    const int F = (quad + h) & 3;
-                    bool on = (F == 0 && p == nx - 1) || (F == 2 && p == 0) ||
-                              (F == 1 && q == ny - 1) || (F == 3 && q == 0);
-                    if (on) {
-                        const double tf =
-                            F == 0 ? cth[r] : (F == 1 ? sth[r] : (F == 2 ? -cth[r] : -sth[r]));
-                        v += ((F & 1) ? dx : dy) * dth * 0.25 * ((F & 1) ? wx[p] : wy[q]) *
-                             wt[r] * tf;
-                    }
+    const double *Dx = m.lgl_D + size_t(nx) * NMq * NMq, *Dy = m.lgl_D + size_t(ny) * NMq * NMq;
+    const int ao = m.cell_ang_off[k], na = m.col_nang[c];
+    const double *G = m.G + m.col_G_off[c];
+    const double *kap = m.kappa + m.col_sp_off[c], *sig = m.sigma + m.col_sp_off[c];
+    for (int e = tid; e < n * n; e += BJ_THREADS) A[e] = 0.0;
+    __syncthreads();
+    // in-cell scattering: row (s, r), column (s, a)
+    for (int e = tid; e < nsp * nth * nth; e += BJ_THREADS) {
+        const int s = e / (nth * nth), r = (e / nth) % nth, a = e % nth;
+        const int p = s / ny, q = s % ny;
+        A[size_t(s * nth + r) * n + s * nth + a] =
+            -dx * dy * wx[p] * wy[q] * sig[s] * G[size_t(ao + a) * na + ao + r];
+    }
+    __syncthreads();
+    // mass and outflow faces on the diagonal
+    const int quad = m.cell_quad[k];
+    for (int t = tid; t < n; t += BJ_THREADS) {
+        const int p = t / nyt, q = (t / nth) % ny, r = t % nth;
+        double v = (dx * dy * dth * 0.125) * wx[p] * wy[q] * wt[r] * kap[p * ny + q];
+        if (quad >= 0)
+            for (int h = 0; h < 2; ++h) {
+                const int F = (quad + h) & 3;
+                const bool on = (F == 0 && p == nx - 1) || (F == 2 && p == 0) ||
+                                (F == 1 && q == ny - 1) || (F == 3 && q == 0);
+                if (on) {
+                    const double tf = F == 0 ? cth[r] : (F == 1 ? sth[r] : (F == 2 ? -cth[r] : -sth[r]));
+                    v += ((F & 1) ? dx : dy) * dth * 0.25 * ((F & 1) ? wx[p] : wy[q]) * wt[r] * tf;
                 }
             }
-        }
+        A[size_t(t) * n + t] += v;
     }
-    if (r == a) {
-        // -IC: x-derivative couples (i, q) -> (p, q); y-derivative (p, j) -> (p, q)
-        if (q == j)
-            v -= (dy * dth * 0.25) * wy[q] * wt[r] * cth[r] * wx[i] *
-                 m.lgl_D[(size_t(nx) * NMq + p) * NMq + i];
-        if (p == i)
-            v -= (dx * dth * 0.25) * wx[p] * wt[r] * sth[r] * wy[j] *
-                 m.lgl_D[(size_t(ny) * NMq + q) * NMq + j];
+    __syncthreads();
+    // -IC, x-derivative: row (p, q, r), column (i, q, r)
+    for (int e = tid; e < n * nx; e += BJ_THREADS) {
+        const int t = e / nx, i = e % nx;
+        const int p = t / nyt, q = (t / nth) % ny, r = t % nth;
+        A[size_t(t) * n + i * nyt + q * nth + r] -=
+            (dy * dth * 0.25) * wy[q] * wt[r] * cth[r] * wx[i] * Dx[p * NMq + i];
     }
+    __syncthreads();
+    // -IC, y-derivative: row (p, q, r), column (p, j, r)
+    for (int e = tid; e < n * ny; e += BJ_THREADS) {
+        const int t = e / ny, j = e % ny;
+        const int p = t / nyt, q = (t / nth) % ny, r = t % nth;
+        A[size_t(t) * n + p * nyt + j * nth + r] -=
+            (dx * dth * 0.25) * wx[p] * wt[r] * sth[r] * wy[j] * Dy[q * NMq + j];
+    }
+    __syncthreads();
     // a cell coupled to itself through a periodic face
     for (int l = m.link_start[k]; l < m.link_start[k + 1]; ++l) {
         if (m.link_src[l] != k) continue;
         const int F = m.link_face[l];
-        int m0, m1;
-        bool row_on, col_on;
-        if (F == 0) { row_on = p == nx - 1; col_on = i == 0; m0 = q; m1 = j; }
-        else if (F == 2) { row_on = p == 0; col_on = i == nx - 1; m0 = q; m1 = j; }
-        else if (F == 1) { row_on = q == ny - 1; col_on = j == 0; m0 = p; m1 = i; }
-        else { row_on = q == 0; col_on = j == ny - 1; m0 = p; m1 = i; }
-        if (!(row_on && col_on)) continue;
         const int n0 = (F & 1) ? nx : ny;
-        v += ((F & 1) ? dx : dy) * dth * 0.25 * m.Es_pool[m.link_Es_off[l] + m1 * n0 + m0] *
-             m.Eth_pool[m.link_Eth_off[l] + a * nth + r];
+        const double cf = ((F & 1) ? dx : dy) * dth * 0.25;
+        const double *Es = m.Es_pool + m.link_Es_off[l];
+        const double *Et = m.Eth_pool + m.link_Eth_off[l];
+        for (int e = tid; e < n0 * nth * n0 * nth; e += BJ_THREADS) {
+            const int m0 = e / (nth * n0 * nth), r = (e / (n0 * nth)) % nth;
+            const int m1 = (e / nth) % n0, a = e % nth;
+            int row, col;
+            if (F == 0) { row = (nx - 1) * nyt + m0 * nth + r; col = m1 * nth + a; }
+            else if (F == 2) { row = m0 * nth + r; col = (nx - 1) * nyt + m1 * nth + a; }
+            else if (F == 1) { row = m0 * nyt + (ny - 1) * nth + r; col = m1 * nyt + a; }
+            else { row = m0 * nyt + r; col = m1 * nyt + (ny - 1) * nth + a; }
+            A[size_t(row) * n + col] += cf * Es[m1 * n0 + m0] * Et[a * nth + r];
+        }
+        __syncthreads();
     }
-    return v;
+    // identity rows/columns on inflow-boundary DOFs
+    const int bd = restrict_ ? m.cell_bdry[k] : 0;
+    if (bd) {
+        for (int e = tid; e < n * n; e += BJ_THREADS) {
+            const int row = e / n, col = e % n;
+            const bool rb = bj_on_bdry(bd, row / nyt, (row / nth) % ny, nx, ny);
+            const bool cb = bj_on_bdry(bd, col / nyt, (col / nth) % ny, nx, ny);
+            if (rb || cb) A[e] = (row == col) ? 1.0 : 0.0;
+        }
+        __syncthreads();
+    }
 }
 
-// Gauss-Jordan inversion with partial pivoting of the n x n row-major matrix
-// A (LDS or global), in place.  Returns nonzero in *bad on a zero pivot.
-__device__ void gauss_jordan(double *A, int n, int *perm, double *red_v, int *red_i,
-                             int *bad) {
-    const int tid = threadIdx.x, nt = blockDim.x;
+// In-place Gauss-Jordan inversion with partial pivoting of row-major A.
+// Returns false on a zero pivot.  colk: n doubles of LDS.
+__device__ bool bj_invert(double *A, int n, double *colk, int *perm, int *piv_row) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int k = 0; k < n; ++k) {
-        // pivot search in column k, rows k..n-1
-        double best = -1.0;
-        int bi = k;
-        for (int i = k + tid; i < n; i += nt) {
-            double v = fabs(A[size_t(i) * n + k]);
-            if (v > best) { best = v; bi = i; }
-        }
-        red_v[tid] = best;
-        red_i[tid] = bi;
-        __syncthreads();
-        for (int s = nt / 2; s > 0; s >>= 1) {
-            if (tid < s) {
-                double o = red_v[tid + s];
-                int oi = red_i[tid + s];
-                if (o > red_v[tid] || (o == red_v[tid] && oi < red_i[tid])) {
-                    red_v[tid] = o;
-                    red_i[tid] = oi;
-                }
+        if (wave == 0) {
+            double best = -1.0;
+            int bi = k;
+            for (int i = k + lane; i < n; i += 64) {
+                const double v = fabs(A[size_t(i) * n + k]);
+                if (v > best) { best = v; bi = i; }
             }
-            __syncthreads();
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ov = __shfl_xor(best, o);
+                const int oi = __shfl_xor(bi, o);
+                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            }
+            if (lane == 0) {
+                *piv_row = (best > 0.0) ? bi : -1;
+                perm[k] = bi;
+            }
         }
-        const int pr = red_i[0];
-        const bool zero = !(red_v[0] > 0.0);
         __syncthreads();
-        if (zero) {
-            if (tid == 0) *bad = 1;
-            return;
-        }
-        if (tid == 0) perm[k] = pr;
+        const int pr = *piv_row;
+        if (pr < 0) return false;
         if (pr != k)
-            for (int c = tid; c < n; c += nt) {
-                double t = A[size_t(k) * n + c];
+            for (int c = tid; c < n; c += BJ_THREADS) {
+                const double t = A[size_t(k) * n + c];
                 A[size_t(k) * n + c] = A[size_t(pr) * n + c];
                 A[size_t(pr) * n + c] = t;
             }
         __syncthreads();
-        const double inv = 1.0 / A[size_t(k) * n + k];
+        for (int i = tid; i < n; i += BJ_THREADS) colk[i] = A[size_t(i) * n + k];
         __syncthreads();
-        for (int c = tid; c < n; c += nt)
+        const double inv = 1.0 / colk[k];
+        for (int c = tid; c < n; c += BJ_THREADS)
             A[size_t(k) * n + c] = (c == k) ? inv : A[size_t(k) * n + c] * inv;
         __syncthreads();
-        // eliminate column k from every other row
-        for (int e = tid; e < n * n; e += nt) {
-            const int i = e / n, c = e % n;
-            if (i == k) continue;
-            const double f = A[size_t(i) * n + k];
-            if (c == k) continue;
-            A[size_t(i) * n + c] -= f * A[size_t(k) * n + c];
+        // Rows are processed in register-staged batches of 8 so that the LDS
+        // loads of a batch are in flight together (the row updates of one
+        // column are independent).
+        const int g = tid / BJ_COLS, ng = BJ_THREADS / BJ_COLS;
+        for (int c = tid % BJ_COLS; c < n; c += BJ_COLS) {
+            const double akc = A[size_t(k) * n + c];
+            const bool pivcol = (c == k);
+            int i = g;
+            for (; i + 7 * ng < n; i += 8 * ng) {
+                double v[8], f[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    v[u] = A[size_t(i + u * ng) * n + c];
+                    f[u] = colk[i + u * ng];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    v[u] = (i + u * ng == k) ? v[u] : (pivcol ? -f[u] * inv : v[u] - f[u] * akc);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) A[size_t(i + u * ng) * n + c] = v[u];
+            }
+            for (; i < n; i += ng)
+                if (i != k) A[size_t(i) * n + c] = pivcol ? -colk[i] * inv : A[size_t(i) * n + c] - colk[i] * akc;
         }
-        __syncthreads();
-        for (int i = tid; i < n; i += nt)
-            if (i != k) A[size_t(i) * n + k] = -A[size_t(i) * n + k] * inv;
         __syncthreads();
     }
     // undo the row interchanges as column interchanges, in reverse order
     for (int k = n - 1; k >= 0; --k) {
         const int pr = perm[k];
         if (pr != k)
-            for (int i = tid; i < n; i += nt) {
-                double t = A[size_t(i) * n + k];
+            for (int i = tid; i < n; i += BJ_THREADS) {
+                const double t = A[size_t(i) * n + k];
                 A[size_t(i) * n + k] = A[size_t(i) * n + pr];
                 A[size_t(i) * n + pr] = t;
             }
         __syncthreads();
     }
+    return true;
 }
 
-// One workgroup per cell.  USE_LDS: factor in dynamic LDS, then write the
-// inverse transposed (column-major) to out; otherwise factor in place in a
-// global scratch block and transpose into out.
+// One workgroup per cell.  USE_LDS: the block lives in dynamic LDS; else in
+// the global scratch block of the cell.  The inverse goes to out,
+// column-major.
 template <bool USE_LDS>
-__global__ __launch_bounds__(BJ_THREADS) void k_bj_build(DevMesh m, int restrict_, const int64_t *inv_off,
+__global__ __launch_bounds__(BJ_THREADS) void k_bj_build(DevMesh m, int restrict_,
+                                                        const int64_t *inv_off,
                                                         double *__restrict__ out,
                                                         double *__restrict__ scratch,
                                                         int *__restrict__ status) {
     extern __shared__ double lds[];
     __shared__ double cth[NMq], sth[NMq];
-    __shared__ double red_v[BJ_THREADS];
-    __shared__ int red_i[BJ_THREADS];
+    __shared__ double colk[RTX_NMAX * RTX_NMAX * RTX_NMAX];
     __shared__ int perm[RTX_NMAX * RTX_NMAX * RTX_NMAX];
-    __shared__ int bad;
+    __shared__ int piv_row;
     const int k = blockIdx.x, tid = threadIdx.x;
     const int c = m.cell_col[k];
-    const int nx = m.col_nx[c], ny = m.col_ny[c], nth = m.cell_nth[k];
-    const int n = nx * ny * nth, nyt = ny * nth;
-    const int bd = restrict_ ? m.cell_bdry[k] : 0;
-    if (tid == 0) bad = 0;
+    const int nth = m.cell_nth[k];
+    const int n = m.col_nx[c] * m.col_ny[c] * nth;
     if (tid < nth) {
         const double th0 = m.cell_geom[2 * k], dth = m.cell_geom[2 * k + 1] - th0;
         double s, cc;
@@ -187,35 +223,15 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_build(DevMesh m, int restrict
     }
     __syncthreads();
     double *A = USE_LDS ? lds : scratch + inv_off[k];
-    for (int e = tid; e < n * n; e += BJ_THREADS) {
-        const int row = e / n, col = e % n;
-        const bool rb = bd && bj_on_bdry(bd, row / nyt, (row / nth) % ny, nx, ny);
-        const bool cb = bd && bj_on_bdry(bd, col / nyt, (col / nth) % ny, nx, ny);
-        double v;
-        if (rb || cb) v = (row == col) ? 1.0 : 0.0;
-        else v = block_entry(m, k, row, col, cth, sth);
-        A[e] = v;
-    }
-    __syncthreads();
-    gauss_jordan(A, n, perm, red_v, red_i, &bad);
-    __syncthreads();
-    if (bad) {
+    bj_assemble(m, k, restrict_, A, cth, sth);
+    const bool ok = bj_invert(A, n, colk, perm, &piv_row);
+    if (!ok) {
         if (tid == 0) atomicMax(status, k + 1);
         return;
     }
     double *o = out + inv_off[k];
-    if (USE_LDS) {
-        for (int e = tid; e < n * n; e += BJ_THREADS) {
-            const int col = e / n, row = e % n;     // o is column-major
-            o[e] = A[size_t(row) * n + col];
-        }
-    } else {
-        // out and scratch are distinct buffers
-        for (int e = tid; e < n * n; e += BJ_THREADS) {
-            const int col = e / n, row = e % n;
-            o[e] = A[size_t(row) * n + col];
-        }
-    }
+    for (int col = 0; col < n; ++col)
+        for (int row = tid; row < n; row += BJ_THREADS) o[size_t(col) * n + row] = A[size_t(row) * n + col];
 }
 
 __global__ void k_to_f32_bj(const double *__restrict__ s, float *__restrict__ d, int64_t n) {
@@ -223,13 +239,15 @@ __global__ void k_to_f32_bj(const double *__restrict__ s, float *__restrict__ d,
     for (; i < n; i += int64_t(gridDim.x) * blockDim.x) d[i] = float(s[i]);
 }
 
-// z_K = inv_K r_K, inv column-major.  One workgroup per cell.
+// z_K = inv_K r_K, inv column-major.  One workgroup per cell; the columns
+// are split across the four wavefronts' partial sums when n is small.
 template <typename T, typename M>
 __global__ __launch_bounds__(BJ_THREADS) void k_bj_apply(DevMesh m, const int64_t *inv_off,
                                                         const M *__restrict__ inv,
                                                         const T *__restrict__ r,
                                                         T *__restrict__ z) {
     __shared__ double rs[RTX_NMAX * RTX_NMAX * RTX_NMAX];
+    __shared__ double part[BJ_THREADS];
     const int k = blockIdx.x, tid = threadIdx.x;
     const int c = m.cell_col[k];
     const int n = m.col_nx[c] * m.col_ny[c] * m.cell_nth[k];
@@ -237,10 +255,25 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_apply(DevMesh m, const int64_
     for (int i = tid; i < n; i += BJ_THREADS) rs[i] = double(r[off + i]);
     __syncthreads();
     const M *B = inv + inv_off[k];
-    for (int i = tid; i < n; i += BJ_THREADS) {
+    if (n <= BJ_THREADS / 2) {
+        // two column halves per output row
+        const int half = tid / (BJ_THREADS / 2), i = tid % (BJ_THREADS / 2);
         double acc = 0.0;
-        for (int j = 0; j < n; ++j) acc += double(B[size_t(j) * n + i]) * rs[j];
-        z[off + i] = T(acc);
+        if (i < n) {
+            const int j0 = half * ((n + 1) / 2), j1 = half ? n : (n + 1) / 2;
+#pragma unroll 4
+            for (int j = j0; j < j1; ++j) acc += double(B[size_t(j) * n + i]) * rs[j];
+        }
+        part[tid] = acc;
+        __syncthreads();
+        if (tid < n) z[off + tid] = T(part[tid] + part[tid + BJ_THREADS / 2]);
+    } else {
+        for (int i = tid; i < n; i += BJ_THREADS) {
+            double acc = 0.0;
+#pragma unroll 4
+            for (int j = 0; j < n; ++j) acc += double(B[size_t(j) * n + i]) * rs[j];
+            z[off + i] = T(acc);
+        }
     }
 }
 
@@ -288,7 +321,7 @@ extern "C" int rtx_bjacobi_factor(rtx_plan *p, int restrict_interior, double *d_
     int dev = 0, max_lds = 0;
     RTX_HIP(hipGetDevice(&dev));
     RTX_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    const size_t static_lds = 2 * NMq * 8 + BJ_THREADS * 12 + RTX_NMAX * RTX_NMAX * RTX_NMAX * 4 + 16;
+    const size_t static_lds = 2 * NMq * 8 + RTX_NMAX * RTX_NMAX * RTX_NMAX * 12 + 16;
     int *d_status = nullptr;
     RTX_HIP(hipMallocAsync((void **)&d_status, sizeof(int), st));
     RTX_HIP(hipMemsetAsync(d_status, 0, sizeof(int), st));
@@ -298,10 +331,13 @@ extern "C" int rtx_bjacobi_factor(rtx_plan *p, int restrict_interior, double *d_
         k_bj_build<true><<<p->m.ncell, BJ_THREADS, lds, st>>>(p->m, restrict_interior, p->bj_off,
                                                             d_inv, nullptr, d_status);
     } else {
-        RTX_CHECK(d_scratch != nullptr,
-                  "rtx_bjacobi_factor: blocks of %lld DOFs do not fit LDS; a scratch buffer of "
-                  "the same size as the inverses is required",
-                  (long long)nmax);
+        if (d_scratch == nullptr) {
+            (void)hipFreeAsync(d_status, st);
+            set_error("rtx_bjacobi_factor: blocks of %lld DOFs do not fit LDS; a scratch buffer "
+                      "of the same size as the inverses is required",
+                      (long long)nmax);
+            return RTX_ERR_ARG;
+        }
         k_bj_build<false><<<p->m.ncell, BJ_THREADS, 0, st>>>(p->m, restrict_interior, p->bj_off,
                                                            d_inv, d_scratch, d_status);
     }

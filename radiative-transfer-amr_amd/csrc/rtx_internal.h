@@ -40,7 +40,7 @@ struct DevMesh {
     const int32_t *cell_dof_off, *cell_col, *cell_nth, *cell_quad, *cell_ang_off,
         *cell_bdry;
     const double *cell_geom;
-    const int32_t *link_start, *link_src, *link_face, *link_Es_off, *link_Eth_off;
+    const int32_t *link_start, *link_src, *link_face, *link_Es_off, *link_Eth_off, *link_target;
     const double *Es_pool, *Eth_pool;
     const double *lgl_x, *lgl_w, *lg_x, *lg_w, *lgl_D;
     // coefficients
@@ -49,6 +49,13 @@ struct DevMesh {
     const uint16_t *G_bf16;
 };
 
+}  // namespace rtx
+
+struct rtx_plan;
+namespace rtx {
+size_t fused_lds_bytes(int ncd, int na, int ntrace, int prec);
+int launch_fused(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
+                 int accumulate);
 }  // namespace rtx
 
 struct rtx_plan {
@@ -60,6 +67,8 @@ struct rtx_plan {
     size_t coef_bytes = 0;
     int64_t nsp = 0, nG = 0;
     int32_t max_nang = 0, max_cell_ndof = 0, max_nsp = 0;
+    int32_t max_col_dofs = 0, max_col_cells = 0, max_col_links = 0, max_col_trace = 0;
+    int fused_mode = -1;   // -1 auto, 0 generic kernels, 1 fused per-column kernel
     bool has_coeffs = false;
     // block-Jacobi
     int64_t *bj_off = nullptr;   // ncell + 1 offsets of the column-major inverses

@@ -49,7 +49,7 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
               "rtx_plan_create: ndof=%lld outside [1, 2^31)", (long long)d->ndof);
     const int nc = d->ncol, nk = d->ncell, nl = d->nlink;
     // Validate the tables the kernels index with.
-    int32_t max_nang = 0, max_cell = 0, max_nsp = 0;
+    int32_t max_nang = 0, max_cell = 0, max_nsp = 0, max_col_dofs = 0, max_col_cells = 0;
     for (int c = 0; c < nc; ++c) {
         int nx = d->col_nx[c], ny = d->col_ny[c];
         RTX_CHECK(nx >= 1 && nx <= RTX_NMAX && ny >= 1 && ny <= RTX_NMAX,
@@ -70,7 +70,12 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
             if (cd > max_cell) max_cell = int32_t(cd);
         }
         RTX_CHECK(d->col_nang[c] == nang, "column %d: nang mismatch", c);
+        for (int k = k0 + 1; k < k1; ++k)
+            RTX_CHECK(d->cell_dof_off[k] == d->cell_dof_off[k - 1] + nx * ny * d->cell_nth[k - 1],
+                      "column %d: cell DOF ranges are not contiguous", c);
         if (nang > max_nang) max_nang = nang;
+        if (nx * ny * nang > max_col_dofs) max_col_dofs = nx * ny * nang;
+        if (k1 - k0 > max_col_cells) max_col_cells = k1 - k0;
         if (nx * ny > max_nsp) max_nsp = nx * ny;
     }
     RTX_CHECK(d->col_cell_start[0] == 0 && d->col_cell_start[nc] == nk,
@@ -78,10 +83,13 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
     RTX_CHECK(d->link_start[0] == 0 && d->link_start[nk] == nl, "bad link_start");
     for (int k = 0; k < nk; ++k)
         RTX_CHECK(d->link_start[k] <= d->link_start[k + 1], "link_start not sorted");
+    std::vector<int32_t> col_links(nc, 0), col_trace(nc, 0);
     for (int k = 0; k < nk; ++k) {
         const int c0 = d->cell_col[k];
         for (int l = d->link_start[k]; l < d->link_start[k + 1]; ++l) {
             int s = d->link_src[l], F = d->link_face[l];
+            RTX_CHECK(d->link_target[l] == k, "link %d: target %d is not cell %d", l,
+                      d->link_target[l], k);
             RTX_CHECK(s >= 0 && s < nk && F >= 0 && F < 4, "link %d: bad source/face", l);
             const int c1 = d->cell_col[s];
             // E^s is n1 x n0 (trace lengths), E^theta is nth1 x nth0
@@ -91,7 +99,14 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
             RTX_CHECK(es >= 0 && es + n0 * n1 <= d->nEs, "link %d: E^s outside its pool", l);
             RTX_CHECK(et >= 0 && et + int64_t(d->cell_nth[k]) * d->cell_nth[s] <= d->nEth,
                       "link %d: E^theta outside its pool", l);
+            col_links[c0] += 1;
+            col_trace[c0] += int32_t(n1) * d->cell_nth[s];
         }
+    }
+    int32_t max_col_links = 0, max_col_trace = 0;
+    for (int c = 0; c < nc; ++c) {
+        if (col_links[c] > max_col_links) max_col_links = col_links[c];
+        if (col_trace[c] > max_col_trace) max_col_trace = col_trace[c];
     }
 
     auto *p = new rtx_plan();
@@ -123,6 +138,7 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
     size_t o_lfa = pk.add(d->link_face, (nl > 0 ? nl : 1) * I);
     size_t o_les = pk.add(d->link_Es_off, (nl > 0 ? nl : 1) * I);
     size_t o_let = pk.add(d->link_Eth_off, (nl > 0 ? nl : 1) * I);
+    size_t o_ltg = pk.add(d->link_target, (nl > 0 ? nl : 1) * I);
     size_t o_esp = pk.add(d->Es_pool, (d->nEs > 0 ? d->nEs : 1) * D);
     size_t o_etp = pk.add(d->Eth_pool, (d->nEth > 0 ? d->nEth : 1) * D);
     size_t o_lx = pk.add(d->lgl_x, NT * D);
@@ -173,6 +189,7 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
     m.link_face = (const int32_t *)(b + o_lfa);
     m.link_Es_off = (const int32_t *)(b + o_les);
     m.link_Eth_off = (const int32_t *)(b + o_let);
+    m.link_target = (const int32_t *)(b + o_ltg);
     m.Es_pool = (const double *)(b + o_esp);
     m.Eth_pool = (const double *)(b + o_etp);
     m.lgl_x = (const double *)(b + o_lx);
@@ -183,6 +200,10 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
     p->max_nang = max_nang;
     p->max_cell_ndof = max_cell;
     p->max_nsp = max_nsp;
+    p->max_col_dofs = max_col_dofs;
+    p->max_col_cells = max_col_cells;
+    p->max_col_links = max_col_links;
+    p->max_col_trace = max_col_trace;
     // sizes of the column G blocks are checked in set_coeffs
     *out = p;
     return RTX_OK;

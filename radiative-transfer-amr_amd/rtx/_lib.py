@@ -9,7 +9,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtx.so")
 
-OP_MASS, OP_SCAT, OP_ICONV, OP_BCONV, OP_RESTRICT, OP_ACCUM = 1, 2, 4, 8, 16, 32
+OP_MASS, OP_SCAT, OP_ICONV, OP_BCONV, OP_RESTRICT, OP_ACCUM, OP_QUADW = 1, 2, 4, 8, 16, 32, 64
 OP_SYSTEM = OP_MASS | OP_SCAT | OP_ICONV | OP_BCONV
 PREC_F64, PREC_F32, PREC_MIXED = 0, 1, 2
 NMAX = 10
@@ -30,7 +30,7 @@ class MeshDesc(C.Structure):
         ("cell_quad", _i32p), ("cell_ang_off", _i32p), ("cell_bdry", _i32p),
         ("cell_geom", _f64p),
         ("link_start", _i32p), ("link_src", _i32p), ("link_face", _i32p),
-        ("link_Es_off", _i32p), ("link_Eth_off", _i32p),
+        ("link_Es_off", _i32p), ("link_Eth_off", _i32p), ("link_target", _i32p),
         ("nEs", C.c_int64), ("nEth", C.c_int64),
         ("Es_pool", _f64p), ("Eth_pool", _f64p),
         ("lgl_x", _f64p), ("lgl_w", _f64p), ("lg_x", _f64p), ("lg_w", _f64p),
@@ -51,6 +51,9 @@ SIGNATURES = [
     ("rtx_plan_destroy", None, [C.c_void_p]),
     ("rtx_plan_set_coeffs", C.c_int, [C.c_void_p, C.c_int64, _f64p, _f64p, C.c_int64, _f64p]),
     ("rtx_apply", C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtx_mask", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    ("rtx_set_path", C.c_int, [C.c_void_p, C.c_int]),
+    ("rtx_assemble_entries", C.c_int, [C.c_void_p, C.c_int, C.c_int64] + [C.c_void_p] * 7),
     ("rtx_bjacobi_size", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("rtx_bjacobi_factor", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p]),

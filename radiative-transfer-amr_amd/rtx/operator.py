@@ -14,7 +14,7 @@ import torch
 
 from . import _lib
 from ._lib import (OP_MASS, OP_SCAT, OP_ICONV, OP_BCONV, OP_RESTRICT, OP_ACCUM,
-                   OP_SYSTEM, PREC_F64, PREC_F32, PREC_MIXED, check, ptr)
+                   OP_QUADW, OP_SYSTEM, PREC_F64, PREC_F32, PREC_MIXED, check, ptr)
 from .plan import MeshTables
 
 __all__ = ["TransportOperator", "OP_MASS", "OP_SCAT", "OP_ICONV", "OP_BCONV",
@@ -59,6 +59,7 @@ class TransportOperator:
         d.link_start, d.link_src = P(t.link_start, i32), P(t.link_src, i32)
         d.link_face = P(t.link_face, i32)
         d.link_Es_off, d.link_Eth_off = P(t.link_Es_off, i32), P(t.link_Eth_off, i32)
+        d.link_target = P(t.link_target, i32)
         d.nEs, d.nEth = t.Es_pool.size, t.Eth_pool.size
         d.Es_pool, d.Eth_pool = P(t.Es_pool, f64), P(t.Eth_pool, f64)
         d.lgl_x, d.lgl_w = P(t.lgl_x, f64), P(t.lgl_w, f64)
@@ -99,3 +100,19 @@ class TransportOperator:
         check(self.lib.rtx_apply(self.plan, ops, precision, C.c_void_p(x.data_ptr()),
                                  C.c_void_p(y.data_ptr()), stream_handle(stream)), "rtx_apply")
         return y
+
+    def set_path(self, mode):
+        """-1 automatic, 0 generic per-cell/per-column kernels, 1 fused per-column kernel."""
+        check(self.lib.rtx_set_path(self.plan, int(mode)), "rtx_set_path")
+
+    def mask(self, x, keep_interior=True, stream=None):
+        """Zero the inflow-boundary DOFs (keep_interior) or the interior ones."""
+        prec = PREC_F64 if x.dtype == torch.float64 else PREC_F32
+        self._check_vec(x, x.dtype)
+        check(self.lib.rtx_mask(self.plan, 0 if keep_interior else 1, prec,
+                                C.c_void_p(x.data_ptr()), stream_handle(stream)), "rtx_mask")
+        return x
+
+    def nodal(self, func):
+        """func(x, y, th) at every DOF node in the global DOF order (host)."""
+        return self.t.eval_nodal_dofs(func)

@@ -307,6 +307,27 @@ class MeshTables:
             out[o:o + nx * ny] = eval_nodal(func, xf, yf).ravel()
         return out
 
+    def eval_nodal_dofs(self, func):
+        """func(x, y, th) at every DOF in the global order; one broadcast call
+        per column over (x nodes, y nodes, all angular nodes of the column)."""
+        out = np.zeros(self.ndof)
+        for ci in range(self.ncol):
+            x0, y0, x1, y1 = self.col_geom[ci]
+            nx, ny = int(self.col_nx[ci]), int(self.col_ny[ci])
+            xf = push_forward(x0, x1, _lgl(nx)[0])
+            yf = push_forward(y0, y1, _lgl(ny)[0])
+            k0, k1 = self.col_cell_start[ci], self.col_cell_start[ci + 1]
+            ths = [push_forward(self.cell_geom[k, 0], self.cell_geom[k, 1],
+                                _lg(int(self.cell_nth[k]))[0]) for k in range(k0, k1)]
+            vals = eval_nodal(func, xf, yf, np.concatenate(ths))
+            a = 0
+            for k, th in zip(range(k0, k1), ths):
+                nt = th.size
+                o = self.cell_dof_off[k]
+                out[o:o + nx * ny * nt] = vals[:, :, a:a + nt].ravel()
+                a += nt
+        return out
+
     def G_pool(self, Phi):
         """G^T[A, R] = (dth_K / 8)(dth_K' / 2) w_r w_a Phi(th_R, th_A) per
         angular signature; returns (pool, per-column offsets)."""

@@ -36,8 +36,9 @@ def _dev(x):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("path", [0, 1], ids=["generic", "fused"])
 @pytest.mark.parametrize("case", ["hp", "c1"])
-def test_apply_parts_match_oracle(cuda, case):
+def test_apply_parts_match_oracle(cuda, case, path):
     from rtx.operator import (TransportOperator, OP_MASS, OP_SCAT, OP_ICONV, OP_BCONV,
                               OP_SYSTEM, OP_RESTRICT)
     if case == "hp":
@@ -46,10 +47,11 @@ def test_apply_parts_match_oracle(cuda, case):
     else:
         mesh = problems.uniform_mesh([1.0, 1.0], [4, 4, 4], 2, 2)
         wl = problems.manufactured()
-    print("case", case, "ndof", mesh.get_ndof(), flush=True)
+    print("case", case, "path", path, "ndof", mesh.get_ndof(), flush=True)
     V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi)
     A = (V["BC"] - V["IC"] + V["M"] + V["S"]).tocsr()
     op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    op.set_path(path)
     x = np.random.default_rng(0).uniform(-1.0, 1.0, op.ndof)
     xd = _dev(x)
     for name, ops, ref in [("M", OP_MASS, V["M"] @ x), ("S", OP_SCAT, V["S"] @ x),
@@ -63,22 +65,38 @@ def test_apply_parts_match_oracle(cuda, case):
     assert _rel(y, ref) < TOL64
 
 
+TOLMIX = 5e-3     # relative 2-norm, BF16 scattering operands with FP32 accumulation
+
+
 @pytest.mark.gpu
-def test_apply_accumulate_and_f32(cuda):
+@pytest.mark.parametrize("path", [0, 1], ids=["generic", "fused"])
+def test_apply_accumulate_and_reduced_precision(cuda, path):
     import torch
-    from rtx.operator import TransportOperator, OP_SYSTEM, OP_ACCUM, PREC_F32
+    from rtx.operator import (TransportOperator, OP_SYSTEM, OP_ACCUM, OP_SCAT, OP_RESTRICT,
+                              PREC_F32, PREC_MIXED)
     mesh = hp_mesh()
     wl = problems.two_scatterers()
     V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi)
     A = (V["BC"] - V["IC"] + V["M"] + V["S"]).tocsr()
     op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    op.set_path(path)
     rng = np.random.default_rng(1)
     x, y0 = rng.uniform(-1, 1, op.ndof), rng.uniform(-1, 1, op.ndof)
     yd = _dev(y0)
     op.apply(_dev(x), yd, ops=OP_SYSTEM | OP_ACCUM)
     assert _rel(yd.cpu().numpy(), y0 + A @ x) < TOL64
-    y32 = op.apply(_dev(x).to(torch.float32), precision=PREC_F32).cpu().double().numpy()
+    x32 = _dev(x).to(torch.float32)
+    y32 = op.apply(x32, precision=PREC_F32).cpu().double().numpy()
     assert _rel(y32, A @ x) < TOL32
+    if path == 1:
+        ym = op.apply(x32, precision=PREC_MIXED).cpu().double().numpy()
+        assert _rel(ym, A @ x) < TOLMIX, _rel(ym, A @ x)
+        ys = op.apply(x32, ops=OP_SCAT, precision=PREC_MIXED).cpu().double().numpy()
+        assert _rel(ys, V["S"] @ x) < 2e-2, _rel(ys, V["S"] @ x)
+        mask = asm.intr_mask(mesh)
+        yr = op.apply(x32, ops=OP_SYSTEM | OP_RESTRICT, precision=PREC_MIXED).cpu().double().numpy()
+        assert np.all(yr[~mask] == 0.0)
+        assert _rel(yr, mask * (A @ (mask * x))) < TOLMIX
 
 
 @pytest.mark.gpu
