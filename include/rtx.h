@@ -88,6 +88,24 @@ typedef struct {
     const int32_t *link_Es_off;     /* E^x or E^y (n1 x n0, row-major) in Es pool  */
     const int32_t *link_Eth_off;    /* E^theta (nth1 x nth0, row-major)            */
     const int32_t *link_target;     /* target cell of each link                    */
+    /* Packed per-column metadata for the fused per-column kernel (derived from
+       the tables above; checked against them in rtx_plan_create):
+       col_ang_start[ncol + 1]: first global angular node of each column
+         (angular nodes numbered cell by cell in DOF order);
+       ang_pack[nang]: column-local first DOF of the node's cell | nth << 20 |
+         local index << 25;
+       ang_trig[3 * nang]: cos and sin of the node's physical angle and its
+         Gauss weight;
+       link_meta[4 * nlink]: source cell's first DOF;
+         F | nx1 << 2 | ny1 << 6 | nth1 << 10 | masked faces of the source << 14 |
+         column-local target cell << 18;
+         column-local offset of the link's trace (nx1 or ny1 times nth1 values);
+         column-local offset of the link's face nodes (nx or ny times nth);
+       col_ntrace[ncol], col_nitems[ncol]: totals of the last two per column. */
+    const int32_t *col_ang_start, *ang_pack;
+    const double *ang_trig;
+    const int32_t *link_meta, *col_ntrace, *col_nitems;
+    int64_t nang;
     int64_t nEs, nEth;
     const double *Es_pool, *Eth_pool;
     /* 1-D rules: for n = 1..RTX_NMAX, index (n * RTX_NMAX + k):
@@ -99,6 +117,11 @@ typedef struct {
 typedef struct rtx_plan rtx_plan;
 
 const char *rtx_last_error(void);
+
+/* Byte offsets of the rtx_mesh_desc fields in declaration order, followed by
+   sizeof(rtx_mesh_desc); writes min(n, count) values and returns count.
+   Lets a binding verify its mirror of the struct. */
+int rtx_mesh_desc_layout(int64_t *offsets, int n);
 
 /* Copy the flattened mesh to the device (replaces the per-call assembly set-up
    of every src/rt/*_matrix.py).  Uses the current HIP device. */

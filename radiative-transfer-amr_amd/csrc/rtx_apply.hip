@@ -264,9 +264,9 @@ static int launch_apply(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hip
 constexpr size_t FUSED_LDS_CAP = 96 * 1024;
 
 static bool fused_fits(const rtx_plan *p, int precision) {
-    return p->max_col_cells <= 64 && p->max_nsp <= 112 && p->max_col_links <= 256 &&
-           fused_lds_bytes(p->max_col_dofs, p->max_nang, p->max_col_trace, precision) <=
-               FUSED_LDS_CAP;
+    const int H = fused_parts(p, precision);
+    return p->max_col_cells <= 64 && p->max_nsp <= 112 && p->max_part_links[H] <= 256 &&
+           fused_lds_bytes(p, H, precision) <= FUSED_LDS_CAP;
 }
 
 }  // namespace rtx
@@ -276,7 +276,7 @@ using namespace rtx;
 /* Select the apply path: -1 automatic (fused when every column fits), 0 the
    generic per-cell + per-column kernels, 1 the fused per-column kernel. */
 extern "C" int rtx_set_path(rtx_plan *p, int mode) {
-    RTX_CHECK(p && mode >= -1 && mode <= 1, "rtx_set_path: bad argument");
+    RTX_CHECK(p && mode >= -1 && mode <= 2, "rtx_set_path: bad argument");
     p->fused_mode = mode;
     return RTX_OK;
 }
@@ -294,8 +294,15 @@ extern "C" int rtx_apply(rtx_plan *p, uint32_t op_mask, int precision, const voi
     hipStream_t st = static_cast<hipStream_t>(stream);
     RTX_CHECK(precision >= RTX_PREC_F64 && precision <= RTX_PREC_MIXED,
               "rtx_apply: unsupported precision %d", precision);
-    if (p->fused_mode != 0 && fused_fits(p, precision))
-        return launch_fused(p, op_mask, precision, d_x, d_y, st, (op_mask & RTX_OP_ACCUM) ? 1 : 0);
+    const int acc = (op_mask & RTX_OP_ACCUM) ? 1 : 0;
+    if ((p->fused_mode == -1 || p->fused_mode == 2) && p->uni_ok) {
+        const int rc = launch_uniform(p, op_mask, precision, d_x, d_y, st, acc);
+        if (rc <= 0) return rc;
+    }
+    RTX_CHECK(p->fused_mode != 2, "rtx_apply: the uniform-bin path was requested but the mesh "
+                                  "is not a compiled uniform bin");
+    if ((p->fused_mode == -1 || p->fused_mode == 1) && fused_fits(p, precision))
+        return launch_fused(p, op_mask, precision, d_x, d_y, st, acc);
     RTX_CHECK(p->fused_mode != 1, "rtx_apply: the fused per-column path was requested but a "
                                   "column does not fit it (DOFs %d, cells %d, nodes %d)",
               p->max_col_dofs, p->max_col_cells, p->max_nsp);

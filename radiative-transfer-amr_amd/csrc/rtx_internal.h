@@ -43,6 +43,9 @@ struct DevMesh {
     const int32_t *link_start, *link_src, *link_face, *link_Es_off, *link_Eth_off, *link_target;
     const double *Es_pool, *Eth_pool;
     const double *lgl_x, *lgl_w, *lg_x, *lg_w, *lgl_D;
+    // packed per-column tables of the fused kernel
+    const int32_t *col_ang_start, *ang_pack, *link_meta, *col_ntrace, *col_nitems;
+    const double *ang_trig;
     // coefficients
     const double *kappa, *sigma, *G;
     const float *kappa_f, *sigma_f, *G_f;
@@ -53,7 +56,11 @@ struct DevMesh {
 
 struct rtx_plan;
 namespace rtx {
-size_t fused_lds_bytes(int ncd, int na, int ntrace, int prec);
+size_t fused_lds_bytes(const rtx_plan *p, int H, int prec);
+int fused_parts(const rtx_plan *p, int prec);
+int launch_uniform(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
+                   int accumulate);
+bool uniform_shape_compiled(int N, int NTH, int NC);
 int launch_fused(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
                  int accumulate);
 }  // namespace rtx
@@ -68,6 +75,15 @@ struct rtx_plan {
     int64_t nsp = 0, nG = 0;
     int32_t max_nang = 0, max_cell_ndof = 0, max_nsp = 0;
     int32_t max_col_dofs = 0, max_col_cells = 0, max_col_links = 0, max_col_trace = 0;
+    int32_t min_col_cells = 0;
+    // uniform bin: every column nx = ny = uni_n, every cell nth = uni_nth,
+    // uni_ncells cells per column, one shared angular mesh (G block)
+    bool uni_ok = false;
+    int32_t uni_n = 0, uni_nth = 0, uni_ncells = 0;
+    // largest own-DOF count and own-trace count of a column part when the
+    // fused kernel splits every column into H = 1, 2, 4 cell ranges
+    int32_t max_part_dofs[5] = {0, 0, 0, 0, 0}, max_part_trace[5] = {0, 0, 0, 0, 0};
+    int32_t max_part_links[5] = {0, 0, 0, 0, 0};
     int fused_mode = -1;   // -1 auto, 0 generic kernels, 1 fused per-column kernel
     bool has_coeffs = false;
     // block-Jacobi

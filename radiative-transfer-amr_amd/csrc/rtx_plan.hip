@@ -1,5 +1,6 @@
 // Plan creation: copy the flattened hp mesh and coefficient tables to HBM.
 #include <cstdarg>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 
@@ -40,6 +41,27 @@ struct Packer {
 using namespace rtx;
 
 extern "C" const char *rtx_last_error(void) { return g_err; }
+
+extern "C" int rtx_mesh_desc_layout(int64_t *o, int n) {
+#define RTX_OFF(f) int64_t(offsetof(rtx_mesh_desc, f))
+    const int64_t v[] = {
+        RTX_OFF(ncol), RTX_OFF(ncell), RTX_OFF(nlink), RTX_OFF(ndof),
+        RTX_OFF(col_cell_start), RTX_OFF(col_nx), RTX_OFF(col_ny), RTX_OFF(col_sp_off),
+        RTX_OFF(col_nang), RTX_OFF(col_G_off), RTX_OFF(col_geom),
+        RTX_OFF(cell_dof_off), RTX_OFF(cell_col), RTX_OFF(cell_nth), RTX_OFF(cell_quad),
+        RTX_OFF(cell_ang_off), RTX_OFF(cell_bdry), RTX_OFF(cell_geom),
+        RTX_OFF(link_start), RTX_OFF(link_src), RTX_OFF(link_face), RTX_OFF(link_Es_off),
+        RTX_OFF(link_Eth_off), RTX_OFF(link_target),
+        RTX_OFF(col_ang_start), RTX_OFF(ang_pack), RTX_OFF(ang_trig), RTX_OFF(link_meta),
+        RTX_OFF(col_ntrace), RTX_OFF(col_nitems), RTX_OFF(nang),
+        RTX_OFF(nEs), RTX_OFF(nEth), RTX_OFF(Es_pool), RTX_OFF(Eth_pool),
+        RTX_OFF(lgl_x), RTX_OFF(lgl_w), RTX_OFF(lg_x), RTX_OFF(lg_w), RTX_OFF(lgl_D),
+        int64_t(sizeof(rtx_mesh_desc))};
+#undef RTX_OFF
+    const int count = int(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; i < count && i < n && o; ++i) o[i] = v[i];
+    return count;
+}
 
 extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
     RTX_CHECK(d != nullptr && out != nullptr, "rtx_plan_create: null argument");
@@ -108,6 +130,39 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
         if (col_links[c] > max_col_links) max_col_links = col_links[c];
         if (col_trace[c] > max_col_trace) max_col_trace = col_trace[c];
     }
+    // The packed fused-kernel tables must agree with the primary tables.
+    {
+        int64_t g = 0;
+        for (int c = 0; c < nc; ++c) {
+            const int k0 = d->col_cell_start[c], k1 = d->col_cell_start[c + 1];
+            const int nsp = d->col_nx[c] * d->col_ny[c];
+            RTX_CHECK(d->col_ang_start[c] == g, "column %d: col_ang_start", c);
+            for (int k = k0; k < k1; ++k)
+                for (int a = 0; a < d->cell_nth[k]; ++a, ++g) {
+                    const int32_t want = (d->cell_dof_off[k] - d->cell_dof_off[k0]) |
+                                         (d->cell_nth[k] << 20) | (a << 25);
+                    RTX_CHECK(g < d->nang && d->ang_pack[g] == want, "angle %lld: ang_pack",
+                              (long long)g);
+                }
+            (void)nsp;
+            int tr = 0, it = 0;
+            for (int l = d->link_start[k0]; l < d->link_start[k1]; ++l) {
+                const int s = d->link_src[l], F = d->link_face[l], k = d->link_target[l];
+                const int c1 = d->cell_col[s];
+                const int32_t w1 = F | (d->col_nx[c1] << 2) | (d->col_ny[c1] << 6) |
+                                   (d->cell_nth[s] << 10) | (d->cell_bdry[s] << 14) |
+                                   ((k - k0) << 18);
+                const int32_t *lm = d->link_meta + 4 * int64_t(l);
+                RTX_CHECK(lm[0] == d->cell_dof_off[s] && lm[1] == w1 && lm[2] == tr && lm[3] == it,
+                          "link %d: link_meta", l);
+                tr += ((F & 1) ? d->col_nx[c1] : d->col_ny[c1]) * d->cell_nth[s];
+                it += ((F & 1) ? d->col_nx[c] : d->col_ny[c]) * d->cell_nth[k];
+            }
+            RTX_CHECK(d->col_ntrace[c] == tr && d->col_nitems[c] == it,
+                      "column %d: trace/item totals", c);
+        }
+        RTX_CHECK(d->col_ang_start[nc] == g && d->nang == g, "nang mismatch");
+    }
 
     auto *p = new rtx_plan();
     hipError_t he = hipGetDevice(&p->device);
@@ -139,6 +194,12 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
     size_t o_les = pk.add(d->link_Es_off, (nl > 0 ? nl : 1) * I);
     size_t o_let = pk.add(d->link_Eth_off, (nl > 0 ? nl : 1) * I);
     size_t o_ltg = pk.add(d->link_target, (nl > 0 ? nl : 1) * I);
+    size_t o_cas = pk.add(d->col_ang_start, (nc + 1) * I);
+    size_t o_apk = pk.add(d->ang_pack, d->nang * I);
+    size_t o_atr = pk.add(d->ang_trig, 3 * d->nang * D);
+    size_t o_lme = pk.add(d->link_meta, 4 * (nl > 0 ? nl : 1) * I);
+    size_t o_cnt = pk.add(d->col_ntrace, nc * I);
+    size_t o_cni = pk.add(d->col_nitems, nc * I);
     size_t o_esp = pk.add(d->Es_pool, (d->nEs > 0 ? d->nEs : 1) * D);
     size_t o_etp = pk.add(d->Eth_pool, (d->nEth > 0 ? d->nEth : 1) * D);
     size_t o_lx = pk.add(d->lgl_x, NT * D);
@@ -190,6 +251,12 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
     m.link_Es_off = (const int32_t *)(b + o_les);
     m.link_Eth_off = (const int32_t *)(b + o_let);
     m.link_target = (const int32_t *)(b + o_ltg);
+    m.col_ang_start = (const int32_t *)(b + o_cas);
+    m.ang_pack = (const int32_t *)(b + o_apk);
+    m.ang_trig = (const double *)(b + o_atr);
+    m.link_meta = (const int32_t *)(b + o_lme);
+    m.col_ntrace = (const int32_t *)(b + o_cnt);
+    m.col_nitems = (const int32_t *)(b + o_cni);
     m.Es_pool = (const double *)(b + o_esp);
     m.Eth_pool = (const double *)(b + o_etp);
     m.lgl_x = (const double *)(b + o_lx);
@@ -204,6 +271,48 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
     p->max_col_cells = max_col_cells;
     p->max_col_links = max_col_links;
     p->max_col_trace = max_col_trace;
+    p->min_col_cells = max_col_cells;
+    {
+        // uniform bin detection
+        bool ok = true;
+        const int N = d->col_nx[0], NTH = d->cell_nth[0];
+        const int NC = d->col_cell_start[1] - d->col_cell_start[0];
+        for (int c = 0; c < nc && ok; ++c) {
+            ok = d->col_nx[c] == N && d->col_ny[c] == N &&
+                 d->col_cell_start[c + 1] - d->col_cell_start[c] == NC &&
+                 d->col_G_off[c] == d->col_G_off[0];
+        }
+        for (int k = 0; k < nk && ok; ++k) ok = d->cell_nth[k] == NTH;
+        // every link must couple equal shapes (trace of N x NTH values)
+        for (int l = 0; l < nl && ok; ++l) {
+            const int s = d->link_src[l];
+            ok = d->cell_nth[s] == NTH && d->col_nx[d->cell_col[s]] == N;
+        }
+        p->uni_ok = ok && uniform_shape_compiled(N, NTH, NC);
+        p->uni_n = N;
+        p->uni_nth = NTH;
+        p->uni_ncells = NC;
+    }
+    for (int c = 0; c < nc; ++c) {
+        const int k0 = d->col_cell_start[c], k1 = d->col_cell_start[c + 1], ncl = k1 - k0;
+        if (ncl < p->min_col_cells) p->min_col_cells = ncl;
+        const int nsp = d->col_nx[c] * d->col_ny[c];
+        for (int H : {1, 2, 4}) {
+            for (int h = 0; h < H; ++h) {
+                const int lo = k0 + (h * ncl) / H, hi = k0 + ((h + 1) * ncl) / H;
+                int dofs = 0, tr = 0;
+                for (int k = lo; k < hi; ++k) dofs += nsp * d->cell_nth[k];
+                const int l0 = d->link_start[lo], l1 = d->link_start[hi];
+                for (int l = l0; l < l1; ++l) {
+                    const int s = d->link_src[l], F = d->link_face[l], c1 = d->cell_col[s];
+                    tr += ((F & 1) ? d->col_nx[c1] : d->col_ny[c1]) * d->cell_nth[s];
+                }
+                if (dofs > p->max_part_dofs[H]) p->max_part_dofs[H] = dofs;
+                if (tr > p->max_part_trace[H]) p->max_part_trace[H] = tr;
+                if (l1 - l0 > p->max_part_links[H]) p->max_part_links[H] = l1 - l0;
+            }
+        }
+    }
     // sizes of the column G blocks are checked in set_coeffs
     *out = p;
     return RTX_OK;

@@ -1,0 +1,426 @@
+// Order-binned fused apply for uniform column bins.
+//
+// When every column has nx = ny = N, every cell nth = NTH angular nodes and
+// every column NC cells with one shared angular mesh (one G^T block), the
+// per-column work has compile-time shape: NSP = N^2 spatial nodes,
+// NA = NC NTH angles, NCD = NSP NA DOFs.  The kernel is the same five-phase
+// computation as rtx_fused.hip (same reference formulas), specialised:
+// no searches, constant divisions, batched register prefetch of the column,
+// the part's neighbour traces and the MFMA B operands, and XCD-aware
+// (column, part) blocks.
+#include <type_traits>
+
+#include "rtx_internal.h"
+
+namespace rtx {
+
+namespace {
+
+typedef double u_dbl4 __attribute__((ext_vector_type(4)));
+typedef float u_flt4 __attribute__((ext_vector_type(4)));
+typedef short u_sht4 __attribute__((ext_vector_type(4)));
+
+constexpr int UNM = RTX_NMAX;
+constexpr int UXCD = 8;
+constexpr int UTHREADS = 256;
+
+__device__ inline bool u_on_bdry(int bd, int i, int j, int n) {
+    return ((bd & 1) && i == n - 1) || ((bd & 2) && j == n - 1) || ((bd & 4) && i == 0) ||
+           ((bd & 8) && j == 0);
+}
+
+__device__ inline uint16_t u_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+template <typename T, typename A, int PREC, int N, int NTH, int NC, int H>
+__global__ __launch_bounds__(UTHREADS) void k_uniform(DevMesh m, uint32_t ops, int accumulate,
+                                                     const T *__restrict__ x,
+                                                     T *__restrict__ y) {
+    constexpr int NSP = N * N, NA = NC * NTH, NCD = NSP * NA, CD = NSP * NTH;
+    constexpr int OC = NC / H;                 // cells per part
+    constexpr int NOWN = OC * CD, NOA = OC * NTH;
+    constexpr int MT = (NSP + 15) / 16;
+    constexpr int NTILE = (NOA + 15) / 16;     // output-angle tiles of the part
+    constexpr int NKS = (NA + 3) / 4;          // f64/f32 k-steps
+    constexpr int NKB = (NA + 15) / 16;        // bf16 k-steps
+    constexpr int MAXL = 4 * OC;               // links of the part (<= 2 per cell per
+                                               // face pair; 4 leaves room for hp-free meshes)
+    constexpr int TRL = N * NTH;               // trace values per link
+    constexpr int XU = (NCD + UTHREADS - 1) / UTHREADS;
+    static_assert(NTILE <= UTHREADS / 64, "one output-angle tile per wavefront");
+    static_assert(NC % H == 0, "parts must split the cells evenly");
+    extern __shared__ __align__(16) unsigned char u_lds[];
+    A *xs = reinterpret_cast<A *>(u_lds);      // NCD
+    A *ys = xs + NCD;                          // NOWN
+    A *tr = ys + NOWN;                         // MAXL * TRL
+    __shared__ A cth[NA], sth[NA], cw[NA];
+    __shared__ A kap[NSP], sgw[NSP];
+    __shared__ A wx_[N], WD[N * N];
+    __shared__ A dth_[NC];
+    __shared__ int cbd[NC], cqd[NC];
+    __shared__ int l_src[MAXL], l_meta[MAXL], l_es[MAXL], l_et[MAXL];
+
+    const int b = blockIdx.x;
+    const int grp = b / (UXCD * H), rem = b % (UXCD * H);
+    const int h = rem / UXCD, c = grp * UXCD + rem % UXCD;
+    if (c >= m.ncol) return;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int k0 = m.col_cell_start[c];
+    const int cdof = m.cell_dof_off[k0];
+    const int kk_lo = h * OC;
+    const int own0 = kk_lo * CD;
+    const int L0 = m.link_start[k0 + kk_lo];
+    const int nl = m.link_start[k0 + kk_lo + OC] - L0;
+    const bool restrict_ = (ops & RTX_OP_RESTRICT) != 0;
+    const bool bconv = (ops & RTX_OP_BCONV) != 0;
+    const double dx = m.col_geom[4 * c + 2] - m.col_geom[4 * c + 0];
+    const double dy = m.col_geom[4 * c + 3] - m.col_geom[4 * c + 1];
+    const int ga0 = m.col_ang_start[c];
+
+    // ---- batched loads issued before any use
+    typedef typename std::conditional<PREC == RTX_PREC_F64, double, float>::type EB;
+    EB bpre[(PREC == RTX_PREC_MIXED) ? 1 : NKS];
+    const int l16 = lane & 15, lg = lane >> 4;
+    const int Rl = wave * 16 + l16;            // part-local output angle of this lane
+    const bool rok = (wave < NTILE) && (Rl < NOA);
+    const int R = kk_lo * NTH + Rl;
+    if constexpr (PREC != RTX_PREC_MIXED) {
+        const EB *Gp = (PREC == RTX_PREC_F64) ? (const EB *)(m.G + m.col_G_off[c])
+                                              : (const EB *)(m.G_f + m.col_G_off[c]);
+        const bool scat = (ops & RTX_OP_SCAT) != 0;
+#pragma unroll
+        for (int u = 0; u < NKS; ++u) {
+            const int Ak = u * 4 + lg;
+            bpre[u] = (scat && rok && Ak < NA) ? Gp[size_t(Ak) * NA + R] : EB(0);
+        }
+    }
+    A xv[XU];
+#pragma unroll
+    for (int u = 0; u < XU; ++u) {
+        const int t = tid + u * UTHREADS;
+        xv[u] = (t < NCD) ? A(x[cdof + t]) : A(0);
+    }
+    double r_c = 0, r_s = 0, r_w = 0, r_kap = 0, r_sig = 0, r_wd = 0, r_g = 0;
+    int r_bd = 0, r_q = -1, r_src = 0, r_meta = 0, r_es = 0, r_et = 0;
+    if (tid < NA) {
+        r_c = m.ang_trig[3 * (ga0 + tid)];
+        r_s = m.ang_trig[3 * (ga0 + tid) + 1];
+        r_w = m.ang_trig[3 * (ga0 + tid) + 2];
+    }
+    if (tid < NSP) {
+        r_kap = m.kappa[m.col_sp_off[c] + tid];
+        r_sig = m.sigma[m.col_sp_off[c] + tid];
+    }
+    if (tid < N * N) {
+        const int p = tid / N, i = tid % N;
+        r_wd = m.lgl_w[N * UNM + i] * m.lgl_D[(size_t(N) * UNM + p) * UNM + i];
+    }
+    if (tid < NC) {
+        const int k = k0 + tid;
+        r_bd = m.cell_bdry[k];
+        r_q = m.cell_quad[k];
+        r_g = m.cell_geom[2 * k + 1] - m.cell_geom[2 * k];
+    }
+    if (bconv && tid < nl) {
+        const int l = L0 + tid;
+        r_src = m.link_meta[4 * l + 0];
+        r_meta = m.link_meta[4 * l + 1];
+        r_es = m.link_Es_off[l];
+        r_et = m.link_Eth_off[l];
+    }
+    if (tid < NA) {
+        cth[tid] = A(r_c);
+        sth[tid] = A(r_s);
+        cw[tid] = A(r_w);
+    }
+    if (tid < NSP) {
+        kap[tid] = A(r_kap);
+        sgw[tid] = A(dx * dy * m.lgl_w[N * UNM + tid / N] * m.lgl_w[N * UNM + tid % N] * r_sig);
+    }
+    if (tid < N * N) WD[tid] = A(r_wd);
+    if (tid < N) wx_[tid] = A(m.lgl_w[N * UNM + tid]);
+    if (tid < NC) {
+        cbd[tid] = restrict_ ? r_bd : 0;
+        cqd[tid] = r_q;
+        dth_[tid] = A(r_g);
+    }
+    if (bconv && tid < nl) {
+        l_src[tid] = r_src;
+        l_meta[tid] = r_meta;
+        l_es[tid] = r_es;
+        l_et[tid] = r_et;
+    }
+    __syncthreads();
+
+    // ---- phase 1: column input (masked) and the part's neighbour traces
+#pragma unroll
+    for (int u = 0; u < XU; ++u) {
+        const int t = tid + u * UTHREADS;
+        if (t < NCD) {
+            A v = xv[u];
+            if (restrict_) {
+                const int kk = t / CD, lt = t % CD;
+                if (cbd[kk] && u_on_bdry(cbd[kk], lt / (N * NTH), (lt / NTH) % N, N)) v = A(0);
+            }
+            xs[t] = v;
+        }
+    }
+    if (bconv) {
+        constexpr int TU = (MAXL * TRL + UTHREADS - 1) / UTHREADS;
+        int addr[TU];
+        bool zero[TU];
+        A v[TU];
+#pragma unroll
+        for (int u = 0; u < TU; ++u) {
+            const int e = tid + u * UTHREADS;
+            const int li = e / TRL, le = e % TRL;
+            addr[u] = -1;
+            zero[u] = false;
+            if (li < nl) {
+                const int w1 = l_meta[li];
+                const int F = w1 & 3, bd1 = restrict_ ? ((w1 >> 14) & 15) : 0;
+                const int m1 = le / NTH, a = le % NTH;
+                int i1, j1;
+                if (F == 0) { i1 = 0; j1 = m1; }
+                else if (F == 2) { i1 = N - 1; j1 = m1; }
+                else if (F == 1) { i1 = m1; j1 = 0; }
+                else { i1 = m1; j1 = N - 1; }
+                addr[u] = l_src[li] + (i1 * N + j1) * NTH + a;
+                zero[u] = bd1 && u_on_bdry(bd1, i1, j1, N);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TU; ++u) v[u] = (addr[u] >= 0) ? A(x[addr[u]]) : A(0);
+#pragma unroll
+        for (int u = 0; u < TU; ++u) {
+            const int e = tid + u * UTHREADS;
+            if (e < MAXL * TRL) tr[e] = zero[u] ? A(0) : v[u];
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: cell-local terms on the part's DOFs
+    for (int e = tid; e < NOWN; e += UTHREADS) {
+        const int kk = kk_lo + e / CD, lt = e % CD;
+        const int p = lt / (N * NTH), q = (lt / NTH) % N, r = lt % NTH;
+        const int ca = kk * NTH + r;
+        const A dth = dth_[kk], wr = cw[ca];
+        const A *xc = xs + kk * CD;
+        A acc = A(0);
+        if (ops & RTX_OP_MASS) acc += A(dx * dy * 0.125) * dth * wx_[p] * wx_[q] * wr * kap[p * N + q] * xc[lt];
+        if (ops & RTX_OP_QUADW) acc += A(dx * dy * 0.125) * dth * wx_[p] * wx_[q] * wr * xc[lt];
+        if (ops & RTX_OP_ICONV) {
+            A sx = A(0), sy = A(0);
+#pragma unroll
+            for (int i = 0; i < N; ++i) sx += WD[p * N + i] * xc[(i * N + q) * NTH + r];
+#pragma unroll
+            for (int j = 0; j < N; ++j) sy += WD[q * N + j] * xc[(p * N + j) * NTH + r];
+            acc -= A(dy * 0.25) * dth * wx_[q] * wr * cth[ca] * sx +
+                   A(dx * 0.25) * dth * wx_[p] * wr * sth[ca] * sy;
+        }
+        const int quad = cqd[kk];
+        if (bconv && quad >= 0) {
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const int F = (quad + hh) & 3;
+                const bool on = (F == 0 && p == N - 1) || (F == 2 && p == 0) ||
+                                (F == 1 && q == N - 1) || (F == 3 && q == 0);
+                if (on) {
+                    const A c_ = cth[ca], s_ = sth[ca];
+                    const A tf = F == 0 ? c_ : (F == 1 ? s_ : (F == 2 ? -c_ : -s_));
+                    acc += A(((F & 1) ? dx : dy) * 0.25) * dth * ((F & 1) ? wx_[p] : wx_[q]) * wr * tf * xc[lt];
+                }
+            }
+        }
+        ys[e] = acc;
+    }
+    __syncthreads();
+
+    // ---- phase 3: upwind inflow couplings (one (link, face node) per thread)
+    if (bconv) {
+        for (int e = tid; e < nl * N * NTH; e += UTHREADS) {
+            const int li = e / (N * NTH), le = e % (N * NTH);
+            const int w1 = l_meta[li];
+            const int F = w1 & 3, kk = w1 >> 18;
+            const int m0 = le / NTH, r = le % NTH;
+            const double *Es = m.Es_pool + l_es[li];
+            const double *Et = m.Eth_pool + l_et[li];
+            A et[NTH], es[N];
+#pragma unroll
+            for (int a = 0; a < NTH; ++a) et[a] = A(Et[a * NTH + r]);
+#pragma unroll
+            for (int a = 0; a < N; ++a) es[a] = A(Es[a * N + m0]);
+            const A *trl = tr + li * TRL;
+            A acc = A(0);
+#pragma unroll
+            for (int m1 = 0; m1 < N; ++m1) {
+                A inner = A(0);
+#pragma unroll
+                for (int a = 0; a < NTH; ++a) inner += et[a] * trl[m1 * NTH + a];
+                acc += es[m1] * inner;
+            }
+            int node;
+            if (F == 0) node = ((N - 1) * N + m0) * NTH + r;
+            else if (F == 2) node = m0 * NTH + r;
+            else if (F == 1) node = (m0 * N + N - 1) * NTH + r;
+            else node = (m0 * N) * NTH + r;
+            atomicAdd(&ys[(kk - kk_lo) * CD + node], A(((F & 1) ? dx : dy) * 0.25) * dth_[kk] * acc);
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 4: scattering for the part's output angles (MFMA)
+    if ((ops & RTX_OP_SCAT) && wave < NTILE) {
+        if constexpr (PREC == RTX_PREC_F64 || PREC == RTX_PREC_F32) {
+            typedef typename std::conditional<PREC == RTX_PREC_F64, u_dbl4, u_flt4>::type acc4;
+            acc4 acc[MT];
+#pragma unroll
+            for (int t = 0; t < MT; ++t) acc[t] = acc4{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks) {
+                const int Ak = ks * 4 + lg;
+                const int kcell = Ak / NTH, kloc = Ak % NTH;
+#pragma unroll
+                for (int t = 0; t < MT; ++t) {
+                    const int s = t * 16 + l16;
+                    const EB a = (Ak < NA && s < NSP) ? EB(xs[kcell * CD + s * NTH + kloc]) : EB(0);
+                    if constexpr (PREC == RTX_PREC_F64)
+                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bpre[ks], acc[t], 0, 0, 0);
+                    else
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bpre[ks], acc[t], 0, 0, 0);
+                }
+            }
+            if (rok) {
+                const int ocell = Rl / NTH, oloc = Rl % NTH;
+#pragma unroll
+                for (int t = 0; t < MT; ++t) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int s = t * 16 + ((PREC == RTX_PREC_F64) ? (lg + 4 * v) : (4 * lg + v));
+                        if (s < NSP) ys[ocell * CD + s * NTH + oloc] -= sgw[s] * A(acc[t][v]);
+                    }
+                }
+            }
+        } else {   // MIXED: BF16 operands, FP32 accumulation
+            const uint16_t *G = m.G_bf16 + m.col_G_off[c];
+            u_flt4 acc[MT];
+#pragma unroll
+            for (int t = 0; t < MT; ++t) acc[t] = u_flt4{0, 0, 0, 0};
+            u_sht4 bv[NKB];
+#pragma unroll
+            for (int ks = 0; ks < NKB; ++ks)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int Ak = ks * 16 + 4 * lg + u;
+                    bv[ks][u] = (rok && Ak < NA) ? short(G[size_t(Ak) * NA + R]) : short(0);
+                }
+#pragma unroll
+            for (int ks = 0; ks < NKB; ++ks) {
+#pragma unroll
+                for (int t = 0; t < MT; ++t) {
+                    const int s = t * 16 + l16;
+                    u_sht4 av;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int Ak = ks * 16 + 4 * lg + u;
+                        av[u] = (Ak < NA && s < NSP)
+                                    ? short(u_bf16(float(xs[(Ak / NTH) * CD + s * NTH + Ak % NTH])))
+                                    : short(0);
+                    }
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[ks], acc[t], 0, 0, 0);
+                }
+            }
+            if (rok) {
+                const int ocell = Rl / NTH, oloc = Rl % NTH;
+#pragma unroll
+                for (int t = 0; t < MT; ++t) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int s = t * 16 + 4 * lg + v;
+                        if (s < NSP) ys[ocell * CD + s * NTH + oloc] -= sgw[s] * A(acc[t][v]);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 5: write back the part's DOFs
+    for (int e = tid; e < NOWN; e += UTHREADS) {
+        const int t = own0 + e;
+        bool masked = false;
+        if (restrict_) {
+            const int kk = t / CD, lt = t % CD;
+            masked = cbd[kk] && u_on_bdry(cbd[kk], lt / (N * NTH), (lt / NTH) % N, N);
+        }
+        const A v = masked ? A(0) : ys[e];
+        if (accumulate) {
+            if (!masked) y[cdof + t] = T(A(y[cdof + t]) + v);
+        } else {
+            y[cdof + t] = T(v);
+        }
+    }
+}
+
+template <typename T, typename A, int PREC, int N, int NTH, int NC, int H>
+int launch_one(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t st, int acc) {
+    constexpr int NSP = N * N, NA = NC * NTH, NCD = NSP * NA, CD = NSP * NTH;
+    constexpr int NOWN = (NC / H) * CD, MAXL = 4 * (NC / H), TRL = N * NTH;
+    const size_t lds = size_t(NCD + NOWN + MAXL * TRL) * sizeof(A);
+    RTX_CHECK(p->max_part_links[H] <= MAXL, "uniform apply: %d links in a column part (max %d)",
+              p->max_part_links[H], MAXL);
+    auto kern = k_uniform<T, A, PREC, N, NTH, NC, H>;
+    RTX_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                int(lds)));
+    const int nblocks = ((p->m.ncol + UXCD - 1) / UXCD) * UXCD * H;
+    kern<<<nblocks, UTHREADS, lds, st>>>(p->m, ops, acc, (const T *)dx, (T *)dy);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+template <int N, int NTH, int NC>
+int launch_shape(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
+                 int acc) {
+    // Two parts per column: LDS near 30 KB in FP64 and at most four
+    // output-angle tiles (one per wavefront) for the compiled shapes.
+    constexpr int H = (NC % 2 == 0) ? 2 : 1;
+    if (prec == RTX_PREC_F64)
+        return launch_one<double, double, RTX_PREC_F64, N, NTH, NC, H>(p, ops, dx, dy, st, acc);
+    if (prec == RTX_PREC_F32)
+        return launch_one<float, float, RTX_PREC_F32, N, NTH, NC, H>(p, ops, dx, dy, st, acc);
+    return launch_one<float, float, RTX_PREC_MIXED, N, NTH, NC, H>(p, ops, dx, dy, st, acc);
+}
+
+}  // namespace
+
+// Shapes compiled into the library: (nodes per spatial direction, angular
+// nodes per cell, cells per column).
+int launch_uniform(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
+                   int acc) {
+    const int N = p->uni_n, NTH = p->uni_nth, NC = p->uni_ncells;
+#define RTX_SHAPE(a, b, c_) \
+    if (N == a && NTH == b && NC == c_) return launch_shape<a, b, c_>(p, ops, prec, dx, dy, st, acc);
+    RTX_SHAPE(5, 5, 16)
+    RTX_SHAPE(4, 4, 16)
+    RTX_SHAPE(4, 4, 4)
+    RTX_SHAPE(3, 3, 8)
+    RTX_SHAPE(5, 5, 8)
+    RTX_SHAPE(4, 4, 8)
+    RTX_SHAPE(3, 3, 4)
+#undef RTX_SHAPE
+    return 1;   // not compiled: caller falls back
+}
+
+bool uniform_shape_compiled(int N, int NTH, int NC) {
+    const int shapes[][3] = {{5, 5, 16}, {4, 4, 16}, {4, 4, 4}, {3, 3, 8},
+                             {5, 5, 8},  {4, 4, 8},  {3, 3, 4}};
+    for (auto &s : shapes)
+        if (s[0] == N && s[1] == NTH && s[2] == NC) return true;
+    return false;
+}
+
+}  // namespace rtx

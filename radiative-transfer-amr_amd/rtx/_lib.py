@@ -31,6 +31,9 @@ class MeshDesc(C.Structure):
         ("cell_geom", _f64p),
         ("link_start", _i32p), ("link_src", _i32p), ("link_face", _i32p),
         ("link_Es_off", _i32p), ("link_Eth_off", _i32p), ("link_target", _i32p),
+        ("col_ang_start", _i32p), ("ang_pack", _i32p), ("ang_trig", _f64p),
+        ("link_meta", _i32p), ("col_ntrace", _i32p), ("col_nitems", _i32p),
+        ("nang", C.c_int64),
         ("nEs", C.c_int64), ("nEth", C.c_int64),
         ("Es_pool", _f64p), ("Eth_pool", _f64p),
         ("lgl_x", _f64p), ("lgl_w", _f64p), ("lg_x", _f64p), ("lg_w", _f64p),
@@ -47,6 +50,7 @@ _lib = None
 # (name, restype, argtypes) of every exported entry point.
 SIGNATURES = [
     ("rtx_last_error", C.c_char_p, []),
+    ("rtx_mesh_desc_layout", C.c_int, [C.POINTER(C.c_int64), C.c_int]),
     ("rtx_plan_create", C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     ("rtx_plan_destroy", None, [C.c_void_p]),
     ("rtx_plan_set_coeffs", C.c_int, [C.c_void_p, C.c_int64, _f64p, _f64p, C.c_int64, _f64p]),

@@ -27,6 +27,42 @@ def stream_handle(stream=None):
     return C.c_void_p(s.cuda_stream)
 
 
+def build_desc(t, G_off):
+    """rtx_mesh_desc for MeshTables ``t``; returns (desc, arrays to keep alive)."""
+    keep = []
+    d = _lib.MeshDesc()
+    d.ncol, d.ncell, d.nlink, d.ndof = t.ncol, t.ncell, t.nlink, t.ndof
+
+    def P(arr, ct):
+        a = np.ascontiguousarray(arr)
+        keep.append(a)
+        return ptr(a, ct)
+
+    i32, i64, f64 = C.c_int32, C.c_int64, C.c_double
+    d.col_cell_start = P(t.col_cell_start, i32)
+    d.col_nx, d.col_ny = P(t.col_nx, i32), P(t.col_ny, i32)
+    d.col_sp_off, d.col_nang = P(t.col_sp_off, i32), P(t.col_nang, i32)
+    d.col_G_off = P(np.asarray(G_off, np.int64), i64)
+    d.col_geom = P(t.col_geom.ravel(), f64)
+    d.cell_dof_off, d.cell_col = P(t.cell_dof_off, i32), P(t.cell_col, i32)
+    d.cell_nth, d.cell_quad = P(t.cell_nth, i32), P(t.cell_quad, i32)
+    d.cell_ang_off, d.cell_bdry = P(t.cell_ang_off, i32), P(t.cell_bdry, i32)
+    d.cell_geom = P(t.cell_geom.ravel(), f64)
+    d.link_start, d.link_src = P(t.link_start, i32), P(t.link_src, i32)
+    d.link_face = P(t.link_face, i32)
+    d.link_Es_off, d.link_Eth_off = P(t.link_Es_off, i32), P(t.link_Eth_off, i32)
+    d.link_target = P(t.link_target, i32)
+    d.nEs, d.nEth = t.Es_pool.size, t.Eth_pool.size
+    d.Es_pool, d.Eth_pool = P(t.Es_pool, f64), P(t.Eth_pool, f64)
+    d.lgl_x, d.lgl_w = P(t.lgl_x, f64), P(t.lgl_w, f64)
+    d.lg_x, d.lg_w, d.lgl_D = P(t.lg_x, f64), P(t.lg_w, f64), P(t.lgl_D, f64)
+    d.col_ang_start, d.ang_pack = P(t.col_ang_start, i32), P(t.ang_pack, i32)
+    d.ang_trig, d.link_meta = P(t.ang_trig, f64), P(t.link_meta, i32)
+    d.col_ntrace, d.col_nitems = P(t.col_ntrace, i32), P(t.col_nitems, i32)
+    d.nang = t.nang
+    return d, keep
+
+
 class TransportOperator:
     def __init__(self, mesh, kappa, sigma, Phi, device=None, tables=None):
         if not torch.cuda.is_available():
@@ -36,34 +72,9 @@ class TransportOperator:
         self.t = MeshTables(mesh) if tables is None else tables
         t = self.t
         self.ndof = t.ndof
-        self._keep = []
-        d = _lib.MeshDesc()
-        d.ncol, d.ncell, d.nlink, d.ndof = t.ncol, t.ncell, t.nlink, t.ndof
-
-        def P(arr, ct):
-            a = np.ascontiguousarray(arr)
-            self._keep.append(a)
-            return ptr(a, ct)
-
-        i32, i64, f64 = C.c_int32, C.c_int64, C.c_double
+        f64 = C.c_double
         G_pool, G_off = t.G_pool(Phi)
-        d.col_cell_start = P(t.col_cell_start, i32)
-        d.col_nx, d.col_ny = P(t.col_nx, i32), P(t.col_ny, i32)
-        d.col_sp_off, d.col_nang = P(t.col_sp_off, i32), P(t.col_nang, i32)
-        d.col_G_off = P(G_off.astype(np.int64), i64)
-        d.col_geom = P(t.col_geom.ravel(), f64)
-        d.cell_dof_off, d.cell_col = P(t.cell_dof_off, i32), P(t.cell_col, i32)
-        d.cell_nth, d.cell_quad = P(t.cell_nth, i32), P(t.cell_quad, i32)
-        d.cell_ang_off, d.cell_bdry = P(t.cell_ang_off, i32), P(t.cell_bdry, i32)
-        d.cell_geom = P(t.cell_geom.ravel(), f64)
-        d.link_start, d.link_src = P(t.link_start, i32), P(t.link_src, i32)
-        d.link_face = P(t.link_face, i32)
-        d.link_Es_off, d.link_Eth_off = P(t.link_Es_off, i32), P(t.link_Eth_off, i32)
-        d.link_target = P(t.link_target, i32)
-        d.nEs, d.nEth = t.Es_pool.size, t.Eth_pool.size
-        d.Es_pool, d.Eth_pool = P(t.Es_pool, f64), P(t.Eth_pool, f64)
-        d.lgl_x, d.lgl_w = P(t.lgl_x, f64), P(t.lgl_w, f64)
-        d.lg_x, d.lg_w, d.lgl_D = P(t.lg_x, f64), P(t.lg_w, f64), P(t.lgl_D, f64)
+        d, self._keep = build_desc(t, G_off)
         with torch.cuda.device(self.device):
             h = C.c_void_p()
             check(self.lib.rtx_plan_create(C.byref(d), C.byref(h)), "rtx_plan_create")

@@ -183,6 +183,54 @@ class MeshTables:
         self._links(mesh, col_keys, cell_index)
         self._angular_signatures(mesh, col_keys)
         (self.lgl_x, self.lgl_w, self.lg_x, self.lg_w, self.lgl_D) = quad_tables()
+        self._packed_column_tables()
+
+    # ------------------------------------------ packed tables (fused kernel)
+    def _packed_column_tables(self):
+        """Per-angle and per-link metadata packed for one coalesced load per
+        column (layout documented in rtx_mesh_desc)."""
+        nth = self.cell_nth.astype(np.int64)
+        self.nang = int(nth.sum())
+        self.col_ang_start = np.zeros(self.ncol + 1, np.int32)
+        self.col_ang_start[1:] = np.cumsum(self.col_nang)
+        first_dof = self.cell_dof_off[self.col_cell_start[self.cell_col]]
+        cell_base = (self.cell_dof_off - first_dof).astype(np.int64)
+        cell_of_ang = np.repeat(np.arange(self.ncell), nth)
+        ang_start = np.concatenate([[0], np.cumsum(nth)[:-1]])
+        loc = np.arange(self.nang) - ang_start[cell_of_ang]
+        self.ang_pack = (cell_base[cell_of_ang] | (nth[cell_of_ang] << 20) | (loc << 25)).astype(np.int32)
+        th0 = self.cell_geom[cell_of_ang, 0]
+        th1 = self.cell_geom[cell_of_ang, 1]
+        xi = np.concatenate([_lg(int(n))[0] for n in self.cell_nth]) if self.ncell else np.zeros(0)
+        wa = np.concatenate([_lg(int(n))[1] for n in self.cell_nth]) if self.ncell else np.zeros(0)
+        th = th0 + (th1 - th0) * 0.5 * (xi + 1.0)
+        self.ang_trig = np.ascontiguousarray(np.stack([np.cos(th), np.sin(th), wa], axis=1).ravel())
+        L = self.nlink
+        meta = np.zeros((max(L, 1), 4), np.int64)
+        ntr = np.zeros(self.ncol, np.int64)
+        nit = np.zeros(self.ncol, np.int64)
+        if L:
+            s, F, k = self.link_src.astype(np.int64), self.link_face.astype(np.int64), \
+                self.link_target.astype(np.int64)
+            c0, c1 = self.cell_col[k].astype(np.int64), self.cell_col[s].astype(np.int64)
+            nx1, ny1 = self.col_nx[c1].astype(np.int64), self.col_ny[c1].astype(np.int64)
+            nt1 = self.cell_nth[s].astype(np.int64)
+            kk = k - self.col_cell_start[c0]
+            meta[:L, 0] = self.cell_dof_off[s]
+            meta[:L, 1] = F | (nx1 << 2) | (ny1 << 6) | (nt1 << 10) | \
+                (self.cell_bdry[s].astype(np.int64) << 14) | (kk << 18)
+            tr_sz = np.where(F % 2 == 1, nx1, ny1) * nt1
+            it_sz = np.where(F % 2 == 1, self.col_nx[c0], self.col_ny[c0]) * self.cell_nth[k]
+            # column-local exclusive prefix sums (links are grouped by column)
+            for sz, col in ((tr_sz, 2), (it_sz, 3)):
+                cs = np.cumsum(sz) - sz
+                first = np.searchsorted(c0, c0, side="left")
+                meta[:L, col] = cs - cs[first]
+            ntr = np.bincount(c0, weights=tr_sz, minlength=self.ncol).astype(np.int64)
+            nit = np.bincount(c0, weights=it_sz, minlength=self.ncol).astype(np.int64)
+        self.link_meta = np.ascontiguousarray(meta.astype(np.int32).ravel())
+        self.col_ntrace = ntr.astype(np.int32)
+        self.col_nitems = nit.astype(np.int32)
 
     # --------------------------------------------------------------- links
     def _links(self, mesh, col_keys, cell_index):

@@ -36,11 +36,13 @@ def _dev(x):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("path", [0, 1], ids=["generic", "fused"])
+@pytest.mark.parametrize("path", [0, 1, 2], ids=["generic", "fused", "uniform"])
 @pytest.mark.parametrize("case", ["hp", "c1"])
 def test_apply_parts_match_oracle(cuda, case, path):
     from rtx.operator import (TransportOperator, OP_MASS, OP_SCAT, OP_ICONV, OP_BCONV,
                               OP_SYSTEM, OP_RESTRICT)
+    if case == "hp" and path == 2:
+        pytest.skip("the hp mesh is not a uniform order bin")
     if case == "hp":
         mesh = hp_mesh()
         wl = problems.two_scatterers()
@@ -69,12 +71,15 @@ TOLMIX = 5e-3     # relative 2-norm, BF16 scattering operands with FP32 accumula
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("path", [0, 1], ids=["generic", "fused"])
+@pytest.mark.parametrize("path", [0, 1, 2], ids=["generic", "fused", "uniform"])
 def test_apply_accumulate_and_reduced_precision(cuda, path):
     import torch
     from rtx.operator import (TransportOperator, OP_SYSTEM, OP_ACCUM, OP_SCAT, OP_RESTRICT,
                               PREC_F32, PREC_MIXED)
-    mesh = hp_mesh()
+    if path == 2:
+        mesh = problems.uniform_mesh([1.0, 1.0], [5, 5, 5], 4, 2)   # bin (5, 5, 16)
+    else:
+        mesh = hp_mesh()
     wl = problems.two_scatterers()
     V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi)
     A = (V["BC"] - V["IC"] + V["M"] + V["S"]).tocsr()
@@ -88,7 +93,7 @@ def test_apply_accumulate_and_reduced_precision(cuda, path):
     x32 = _dev(x).to(torch.float32)
     y32 = op.apply(x32, precision=PREC_F32).cpu().double().numpy()
     assert _rel(y32, A @ x) < TOL32
-    if path == 1:
+    if path >= 1:
         ym = op.apply(x32, precision=PREC_MIXED).cpu().double().numpy()
         assert _rel(ym, A @ x) < TOLMIX, _rel(ym, A @ x)
         ys = op.apply(x32, ops=OP_SCAT, precision=PREC_MIXED).cpu().double().numpy()
