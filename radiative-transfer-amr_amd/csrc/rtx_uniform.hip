@@ -36,7 +36,7 @@ __device__ inline uint16_t u_bf16(float f) {
 }
 
 template <typename T, typename A, int PREC, int N, int NTH, int NC, int H>
-__global__ __launch_bounds__(UTHREADS) void k_uniform(DevMesh m, uint32_t ops, int accumulate,
+__global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops, int accumulate,
                                                      const T *__restrict__ x,
                                                      T *__restrict__ y) {
     constexpr int NSP = N * N, NA = NC * NTH, NCD = NSP * NA, CD = NSP * NTH;
@@ -198,45 +198,60 @@ __global__ __launch_bounds__(UTHREADS) void k_uniform(DevMesh m, uint32_t ops, i
 #pragma unroll
         for (int u = 0; u < TU; ++u) {
             const int e = tid + u * UTHREADS;
-            if (e < MAXL * TRL) tr[e] = zero[u] ? A(0) : v[u];
+            if (e < nl * TRL) tr[e] = zero[u] ? A(0) : v[u];
         }
     }
     __syncthreads();
 
-    // ---- phase 2: cell-local terms on the part's DOFs
-    for (int e = tid; e < NOWN; e += UTHREADS) {
-        const int kk = kk_lo + e / CD, lt = e % CD;
-        const int p = lt / (N * NTH), q = (lt / NTH) % N, r = lt % NTH;
-        const int ca = kk * NTH + r;
-        const A dth = dth_[kk], wr = cw[ca];
-        const A *xc = xs + kk * CD;
-        A acc = A(0);
-        if (ops & RTX_OP_MASS) acc += A(dx * dy * 0.125) * dth * wx_[p] * wx_[q] * wr * kap[p * N + q] * xc[lt];
-        if (ops & RTX_OP_QUADW) acc += A(dx * dy * 0.125) * dth * wx_[p] * wx_[q] * wr * xc[lt];
-        if (ops & RTX_OP_ICONV) {
-            A sx = A(0), sy = A(0);
+    // ---- phase 2: cell-local terms on the part's DOFs; a thread's elements
+    // are computed together (all LDS reads before any LDS write) so their
+    // loads overlap
+    {
+        constexpr int EU = (NOWN + UTHREADS - 1) / UTHREADS;
+        A accs[EU];
 #pragma unroll
-            for (int i = 0; i < N; ++i) sx += WD[p * N + i] * xc[(i * N + q) * NTH + r];
+        for (int u = 0; u < EU; ++u) {
+            const int e = tid + u * UTHREADS;
+            A acc = A(0);
+            if (e < NOWN) {
+                const int kk = kk_lo + e / CD, lt = e % CD;
+                const int p = lt / (N * NTH), q = (lt / NTH) % N, r = lt % NTH;
+                const int ca = kk * NTH + r;
+                const A dth = dth_[kk], wr = cw[ca];
+                const A *xc = xs + kk * CD;
+                if (ops & RTX_OP_MASS) acc += A(dx * dy * 0.125) * dth * wx_[p] * wx_[q] * wr * kap[p * N + q] * xc[lt];
+                if (ops & RTX_OP_QUADW) acc += A(dx * dy * 0.125) * dth * wx_[p] * wx_[q] * wr * xc[lt];
+                if (ops & RTX_OP_ICONV) {
+                    A sx = A(0), sy = A(0);
 #pragma unroll
-            for (int j = 0; j < N; ++j) sy += WD[q * N + j] * xc[(p * N + j) * NTH + r];
-            acc -= A(dy * 0.25) * dth * wx_[q] * wr * cth[ca] * sx +
-                   A(dx * 0.25) * dth * wx_[p] * wr * sth[ca] * sy;
-        }
-        const int quad = cqd[kk];
-        if (bconv && quad >= 0) {
+                    for (int i = 0; i < N; ++i) sx += WD[p * N + i] * xc[(i * N + q) * NTH + r];
 #pragma unroll
-            for (int hh = 0; hh < 2; ++hh) {
-                const int F = (quad + hh) & 3;
-                const bool on = (F == 0 && p == N - 1) || (F == 2 && p == 0) ||
-                                (F == 1 && q == N - 1) || (F == 3 && q == 0);
-                if (on) {
-                    const A c_ = cth[ca], s_ = sth[ca];
-                    const A tf = F == 0 ? c_ : (F == 1 ? s_ : (F == 2 ? -c_ : -s_));
-                    acc += A(((F & 1) ? dx : dy) * 0.25) * dth * ((F & 1) ? wx_[p] : wx_[q]) * wr * tf * xc[lt];
+                    for (int j = 0; j < N; ++j) sy += WD[q * N + j] * xc[(p * N + j) * NTH + r];
+                    acc -= A(dy * 0.25) * dth * wx_[q] * wr * cth[ca] * sx +
+                           A(dx * 0.25) * dth * wx_[p] * wr * sth[ca] * sy;
+                }
+                const int quad = cqd[kk];
+                if (bconv && quad >= 0) {
+#pragma unroll
+                    for (int hh = 0; hh < 2; ++hh) {
+                        const int F = (quad + hh) & 3;
+                        const bool on = (F == 0 && p == N - 1) || (F == 2 && p == 0) ||
+                                        (F == 1 && q == N - 1) || (F == 3 && q == 0);
+                        if (on) {
+                            const A c_ = cth[ca], s_ = sth[ca];
+                            const A tf = F == 0 ? c_ : (F == 1 ? s_ : (F == 2 ? -c_ : -s_));
+                            acc += A(((F & 1) ? dx : dy) * 0.25) * dth * ((F & 1) ? wx_[p] : wx_[q]) * wr * tf * xc[lt];
+                        }
+                    }
                 }
             }
+            accs[u] = acc;
         }
-        ys[e] = acc;
+#pragma unroll
+        for (int u = 0; u < EU; ++u) {
+            const int e = tid + u * UTHREADS;
+            if (e < NOWN) ys[e] = accs[u];
+        }
     }
     __syncthreads();
 
@@ -277,9 +292,14 @@ __global__ __launch_bounds__(UTHREADS) void k_uniform(DevMesh m, uint32_t ops, i
     if ((ops & RTX_OP_SCAT) && wave < NTILE) {
         if constexpr (PREC == RTX_PREC_F64 || PREC == RTX_PREC_F32) {
             typedef typename std::conditional<PREC == RTX_PREC_F64, u_dbl4, u_flt4>::type acc4;
-            acc4 acc[MT];
+            // two accumulator chains per tile (even / odd k-steps) halve the
+            // dependent MFMA chain
+            acc4 acc[MT], acc2[MT];
 #pragma unroll
-            for (int t = 0; t < MT; ++t) acc[t] = acc4{0, 0, 0, 0};
+            for (int t = 0; t < MT; ++t) {
+                acc[t] = acc4{0, 0, 0, 0};
+                acc2[t] = acc4{0, 0, 0, 0};
+            }
 #pragma unroll
             for (int ks = 0; ks < NKS; ++ks) {
                 const int Ak = ks * 4 + lg;
@@ -288,12 +308,15 @@ __global__ __launch_bounds__(UTHREADS) void k_uniform(DevMesh m, uint32_t ops, i
                 for (int t = 0; t < MT; ++t) {
                     const int s = t * 16 + l16;
                     const EB a = (Ak < NA && s < NSP) ? EB(xs[kcell * CD + s * NTH + kloc]) : EB(0);
+                    acc4 &dst = (ks & 1) ? acc2[t] : acc[t];
                     if constexpr (PREC == RTX_PREC_F64)
-                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bpre[ks], acc[t], 0, 0, 0);
+                        dst = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bpre[ks], dst, 0, 0, 0);
                     else
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bpre[ks], acc[t], 0, 0, 0);
+                        dst = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bpre[ks], dst, 0, 0, 0);
                 }
             }
+#pragma unroll
+            for (int t = 0; t < MT; ++t) acc[t] += acc2[t];
             if (rok) {
                 const int ocell = Rl / NTH, oloc = Rl % NTH;
 #pragma unroll
@@ -370,9 +393,10 @@ template <typename T, typename A, int PREC, int N, int NTH, int NC, int H>
 int launch_one(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t st, int acc) {
     constexpr int NSP = N * N, NA = NC * NTH, NCD = NSP * NA, CD = NSP * NTH;
     constexpr int NOWN = (NC / H) * CD, MAXL = 4 * (NC / H), TRL = N * NTH;
-    const size_t lds = size_t(NCD + NOWN + MAXL * TRL) * sizeof(A);
     RTX_CHECK(p->max_part_links[H] <= MAXL, "uniform apply: %d links in a column part (max %d)",
               p->max_part_links[H], MAXL);
+    // trace slots for the largest link count of any part
+    const size_t lds = size_t(NCD + NOWN + p->max_part_links[H] * TRL) * sizeof(A);
     auto kern = k_uniform<T, A, PREC, N, NTH, NC, H>;
     RTX_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 int(lds)));
