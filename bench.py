@@ -72,6 +72,19 @@ def algorithmic(op):
     return bytes_, scat + vol + face
 
 
+def measured_traffic():
+    """HBM bytes per apply launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this same workload (counter values scaled by the
+    factors a 1 GiB copy kernel gives), or None when it is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_uniform_c2_f64.md")
+    try:
+        m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(path).read(), re.M)
+    except OSError:
+        return None
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(mesh, wl, target_s=12.0):
     """Oracle assembled CSR SpMV (scipy, one thread) on the rows of a sample
     of columns; DOF/s = rows / time per SpMV."""
@@ -139,7 +152,10 @@ def main():
     t_wall = time.perf_counter() - t_wall0
     ms = ev0.elapsed_time(ev1) / args.steps
     bytes_, flops = algorithmic(op)
-    achieved = bytes_ / (ms * 1e-3) / 1e9
+    gbs = bytes_ / (ms * 1e-3) / 1e9
+    tflops = flops / (ms * 1e-3) / 1e12
+    # the binding resource: the larger of bytes / HBM peak and flops / FP64 peak
+    fp64_bound = flops / (FP64_PEAK_TFLOPS * 1e12) > bytes_ / (HBM_PEAK_GBS * 1e9)
     if args.calib:
         big = torch.empty(1 << 27, dtype=torch.float64, device=dev)   # 1 GiB
         big2 = torch.empty_like(big)
@@ -153,12 +169,16 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "uniform absorber 32x32x16 cells, p=4 (2,048,000 DOF), HG g=0.8, "
                                "sigma/kappa=0.7", "ndof": n, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "operator apply (A_II, FP64)",
+        "roofline": {"bound": "mfma" if fp64_bound else "hbm",
+                     "achieved": tflops if fp64_bound else gbs,
+                     "peak": FP64_PEAK_TFLOPS if fp64_bound else HBM_PEAK_GBS,
+                     "unit": "TFLOP/s" if fp64_bound else "GB/s",
+                     "frac": tflops / FP64_PEAK_TFLOPS if fp64_bound else gbs / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(),
+                     "kernel": "k_uniform (operator apply A_II, FP64, order bin 5x5x16)",
                      "algorithmic_bytes": bytes_, "algorithmic_flops": flops,
-                     "fp64_tflops": flops / (ms * 1e-3) / 1e12,
-                     "fp64_frac": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
+                     "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+                     "fp64_tflops": tflops, "fp64_frac": tflops / FP64_PEAK_TFLOPS},
         "plan_seconds": t_plan, "timed_wall_seconds": t_wall,
     }
     if not args.no_solve:
