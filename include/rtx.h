@@ -143,9 +143,11 @@ int rtx_plan_set_coeffs(rtx_plan *plan, int64_t nsp, const double *kappa,
 int rtx_apply(rtx_plan *plan, uint32_t op_mask, int precision, const void *d_x,
               void *d_y, void *stream);
 
-/* Apply path: -1 automatic (default: the fused per-column kernel whenever
-   every column fits it, else the generic per-cell/per-column kernels), 0
-   generic, 1 fused (rtx_apply then fails if a column does not fit). */
+/* Apply path: -1 automatic (default: the order-binned kernel when the whole
+   mesh is one compiled bin, else the fused per-column kernel whenever every
+   column fits it, else the generic per-cell/per-column kernels), 0 generic,
+   1 fused, 2 order-binned (rtx_apply then fails if the mesh does not fit the
+   forced path). */
 int rtx_set_path(rtx_plan *plan, int mode);
 
 /* In-place mask of a full-length vector: keep = 0 zeroes the inflow-boundary
