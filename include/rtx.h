@@ -208,6 +208,66 @@ int rtx_axpby(int precision, int64_t n, double a, const void *d_x, double b, voi
 /* direction 0: FP64 -> FP32, 1: FP32 -> FP64. */
 int rtx_convert(int direction, int64_t n, const void *d_src, void *d_dst, void *stream);
 
+/* ---- learned coarse scalar-flux correction.  No counterpart in the
+   reference: its preconditioners are PETSc bjacobi / kaczmarz
+   (src/rt/solve.py:86-95).  A coarse grid of n_coarse cells covers the
+   domain; every column maps to the coarse cells it overlaps. */
+
+/* Coarse map (host arrays, copied): entries col_start[c] .. col_start[c+1]-1
+   of column c list coarse cells cell[e] with fcell[e] = area(col & cell) /
+   area(cell) and fcol[e] = area(col & cell) / area(col). */
+int rtx_coarse_setup(rtx_plan *plan, int32_t n_coarse, int32_t nmap, const int32_t *col_start,
+                     const int32_t *cell, const double *fcell, const double *fcol);
+/* mode 0: d_out[q * n_coarse + g] = sum_c fcol * sum_{beta in c} v_beta t_q(theta_beta),
+   t = (1, cos, sin) - angular moments of a weak-form residual (3 n_coarse floats);
+   mode 1: d_out[g] = sum_c fcell * quadrature-weighted column mean of v (n_coarse floats).
+   v is FP64 (precision RTX_PREC_F64) or FP32. */
+int rtx_coarse_restrict(rtx_plan *plan, int mode, int precision, const void *d_v, float *d_out,
+                        void *stream);
+/* z_beta (=, or += when accumulate) = s * sum_e fcol[e] phi[cell[e]] on the interior DOFs of
+   each column and 0 on inflow-boundary DOFs; s = *d_scale (1 when null). */
+int rtx_coarse_prolong(rtx_plan *plan, int precision, const float *d_phi, const float *d_scale,
+                       int accumulate, void *d_z, void *stream);
+/* *d_scale = |m|_2 (1 when m = 0), d_x = m / *d_scale (one workgroup). */
+int rtx_scale_features(int64_t n, const float *d_m, float *d_x, float *d_scale, void *stream);
+
+/* MLP kernels (FP32 training on MFMA, BF16 inference). */
+/* C[m, n] = sum_k A[m, k] B[k, n] with A[m, k] at d_A[m * sam + k * sak] and
+   B[k, n] at d_B[k * sbk + n * sbn]; epilogue 0: C = AB; 1: C = AB + bias[n]
+   and, if d_H, H = relu(C) (same ldc); 2: C = AB * (Z[m, n] > 0), Z at ldz.
+   d_work (rtx_mlp_gemm_workspace floats) enables split-K for tall K. */
+int rtx_mlp_gemm_workspace(int M, int N, int K, int64_t *n_floats);
+int rtx_mlp_gemm(int M, int N, int K, const float *d_A, int64_t sam, int64_t sak,
+                 const float *d_B, int64_t sbk, int64_t sbn, float *d_C, int64_t ldc,
+                 int epilogue, const float *d_bias, float *d_H, const float *d_Z, int64_t ldz,
+                 float *d_work, int64_t work_floats, void *stream);
+/* d_out[n] = sum_m A[m, n] (A is M x N row-major). */
+int rtx_mlp_colsum(int M, int N, const float *d_A, float *d_out, void *stream);
+/* y[n] = act(sum_k W[n, k] x[k] + bias[n]), W BF16 with row stride ldw, act = relu if relu. */
+int rtx_mlp_gemv_bf16(int N, int K, const uint16_t *d_W, int64_t ldw, const float *d_x,
+                      const float *d_bias, int relu, float *d_y, void *stream);
+int rtx_mlp_to_bf16(int64_t n, const float *d_src, uint16_t *d_dst, void *stream);
+/* AdamW step with bias correction (step >= 1). */
+int rtx_adam_step(int64_t n, float *d_p, const float *d_g, float *d_m, float *d_v, float lr,
+                  float beta1, float beta2, float eps, float weight_decay, int step, void *stream);
+/* d_dout = scale (out - y); *d_loss += sum (out - y)^2 (d_loss may be null). */
+int rtx_mse_grad(int64_t n, float scale, const float *d_out, const float *d_y, float *d_dout,
+                 double *d_loss, void *stream);
+/* d_alpha[0] = max(0, r.q / q.q), d_alpha[1] = r.q, d_alpha[2] = q.q, d_alpha[3] = r.r
+   (4 doubles);
+   d_work holds rtx_line_search_workspace doubles. */
+int rtx_line_search_workspace(int64_t *n_doubles);
+int rtx_line_search(int precision, int64_t n, const void *d_r, const void *d_q, double *d_alpha,
+                    double *d_work, void *stream);
+/* Same four outputs for two coarse moment vectors (n floats each): the step
+   minimising |R r - alpha R q|, which is 1 for an exact Galerkin coarse
+   solve.  One workgroup. */
+int rtx_coarse_line_search(int64_t n, const float *d_mr, const float *d_mq, double *d_alpha,
+                           void *stream);
+/* y += sign * d_alpha[0] * x (the scalar stays on the device). */
+int rtx_axpy_dev(int precision, int64_t n, const double *d_alpha, double sign, const void *d_x,
+                 void *d_y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

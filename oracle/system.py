@@ -115,3 +115,94 @@ def mlp_backward(params, cache, dout):
         grads[li] = (g.T @ acts[li], g.sum(axis=0))
         g = g @ W
     return grads
+
+
+# ------------------------------------------------------- coarse-grid oracle
+def coarse_map(mesh, n):
+    """Per leaf column (global order): [(coarse cell gx * n + gy,
+    area(col & cell) / area(cell), area(col & cell) / area(col))] for an
+    n x n grid over the domain (brute force over all coarse cells)."""
+    cols = [ck for ck, _ in leaf_order(mesh)]
+    Lx = max(mesh.cols[ck].pos[2] for ck in cols)
+    Ly = max(mesh.cols[ck].pos[3] for ck in cols)
+    hx, hy = Lx / n, Ly / n
+    out = []
+    for ck in cols:
+        x0, y0, x1, y1 = mesh.cols[ck].pos
+        area = (x1 - x0) * (y1 - y0)
+        ent = []
+        for gx in range(n):
+            for gy in range(n):
+                ox = min(x1, (gx + 1) * hx) - max(x0, gx * hx)
+                oy = min(y1, (gy + 1) * hy) - max(y0, gy * hy)
+                if ox > 1e-12 and oy > 1e-12:
+                    ent.append((gx * n + gy, ox * oy / (hx * hy), ox * oy / area))
+        out.append(ent)
+    return out
+
+
+def _dof_loop(mesh):
+    """Yields (column index, global DOF, cell, i, j, a, theta_a, weight) with
+    weight = (dx dy dth / 8) w_i w_j w_a."""
+    from .quad import lgl, lg, push
+    offs, _ = dof_offsets(mesh)
+    for ci, (ck, kks) in enumerate(leaf_order(mesh)):
+        col = mesh.cols[ck]
+        x0, y0, x1, y1 = col.pos
+        nx, ny = col.ndofs
+        _, wx = lgl(nx)
+        _, wy = lgl(ny)
+        for kk in kks:
+            cell = col.cells[kk]
+            nth = cell.ndofs[0]
+            tb, wt = lg(nth)
+            th = push(cell.pos[0], cell.pos[1], tb)
+            dth = cell.pos[1] - cell.pos[0]
+            o = offs[(ck, kk)]
+            for i in range(nx):
+                for j in range(ny):
+                    for a in range(nth):
+                        w = (x1 - x0) * (y1 - y0) * dth / 8.0 * wx[i] * wy[j] * wt[a]
+                        yield ci, o + nth * ny * i + nth * j + a, cell, i, j, a, th[a], w
+
+
+def coarse_moments(mesh, n, r):
+    """[sum r, sum r cos th, sum r sin th] per coarse cell (3 n^2, channel
+    major), each column's sums split over its coarse cells by area."""
+    cmap = coarse_map(mesh, n)
+    col_m = np.zeros((len(cmap), 3))
+    for ci, b, cell, i, j, a, th, w in _dof_loop(mesh):
+        col_m[ci] += r[b] * np.array([1.0, np.cos(th), np.sin(th)])
+    out = np.zeros(3 * n * n)
+    for ci, ent in enumerate(cmap):
+        for g, fcell, fcol in ent:
+            for q in range(3):
+                out[q * n * n + g] += fcol * col_m[ci, q]
+    return out
+
+
+def coarse_average(mesh, n, v):
+    """Per coarse cell: sum over columns of area(col & cell) / area(cell)
+    times the quadrature-weighted column mean of v."""
+    cmap = coarse_map(mesh, n)
+    num, den = np.zeros(len(cmap)), np.zeros(len(cmap))
+    for ci, b, cell, i, j, a, th, w in _dof_loop(mesh):
+        num[ci] += w * v[b]
+        den[ci] += w
+    out = np.zeros(n * n)
+    for ci, ent in enumerate(cmap):
+        for g, fcell, fcol in ent:
+            out[g] += fcell * num[ci] / den[ci]
+    return out
+
+
+def coarse_prolong(mesh, n, phi, scale, mask):
+    """z_beta = scale * sum_e fcol_e phi[g_e] of the DOF's column, 0 where
+    mask is False."""
+    cmap = coarse_map(mesh, n)
+    uc = [scale * sum(fcol * phi[g] for g, _, fcol in ent) for ent in cmap]
+    offs, N = dof_offsets(mesh)
+    z = np.zeros(N)
+    for ci, b, cell, i, j, a, th, w in _dof_loop(mesh):
+        z[b] = uc[ci] if mask[b] else 0.0
+    return z

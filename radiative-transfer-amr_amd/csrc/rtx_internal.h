@@ -90,4 +90,11 @@ struct rtx_plan {
     int64_t *bj_off = nullptr;   // ncell + 1 offsets of the column-major inverses
     int64_t bj_entries = 0;
     bool bj_ready = false;
+    // coarse grid of the learned correction: per column a list of
+    // (coarse cell, area(col & cell) / area(cell), area(col & cell) / area(col))
+    void *coarse = nullptr;      // one allocation for the map
+    int32_t coarse_n = 0;        // coarse cells
+    int32_t coarse_nmap = 0;     // map entries
+    const int32_t *cmap_start = nullptr, *cmap_cell = nullptr;
+    const double *cmap_fcell = nullptr, *cmap_fcol = nullptr;
 };

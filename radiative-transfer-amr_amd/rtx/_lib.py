@@ -73,6 +73,35 @@ SIGNATURES = [
     ("rtx_axpby", C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_void_p, C.c_double,
                             C.c_void_p, C.c_void_p]),
     ("rtx_convert", C.c_int, [C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    # learned coarse correction
+    ("rtx_coarse_setup", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]),
+    ("rtx_coarse_restrict", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    ("rtx_coarse_prolong", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                     C.c_void_p, C.c_void_p]),
+    ("rtx_scale_features", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtx_mlp_gemm_workspace", C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)]),
+    ("rtx_mlp_gemm", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_int64,
+                               C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int,
+                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                               C.c_int64, C.c_void_p]),
+    ("rtx_mlp_colsum", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtx_mlp_gemv_bf16", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                                    C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    ("rtx_mlp_to_bf16", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtx_adam_step", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                C.c_void_p]),
+    ("rtx_mse_grad", C.c_int, [C.c_int64, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p]),
+    ("rtx_line_search_workspace", C.c_int, [C.POINTER(C.c_int64)]),
+    ("rtx_coarse_line_search", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]),
+    ("rtx_line_search", C.c_int, [C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]),
+    ("rtx_axpy_dev", C.c_int, [C.c_int, C.c_int64, C.c_void_p, C.c_double, C.c_void_p,
+                               C.c_void_p, C.c_void_p]),
 ]
 
 

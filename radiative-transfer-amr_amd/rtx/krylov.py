@@ -125,11 +125,16 @@ def _norm(lib, v, work, out):
 
 
 def gmres(system, b, x=None, rtol=1e-8, atol=0.0, restart=50, max_it=5000,
-          flexible=False, callback=None, true_residual_every_restart=True):
+          flexible=False, callback=None, true_residual_every_restart=True, on_restart=None,
+          on_direction=None):
     """Right-preconditioned restarted GMRES on ``system`` (InteriorSystem).
 
     Converged when |b - A x| <= max(rtol |b|, atol) for the FP64 true
-    residual.  Returns (x, info) with PETSc-like convergence information."""
+    residual.  ``on_restart(x, r)`` (optional) sees the iterate and its true
+    residual at the start of every cycle; ``on_direction(z, w)`` sees every
+    preconditioned direction z and w = A z before orthogonalisation (an exact
+    residual / correction pair: A z = w).  Returns (x, info) with PETSc-like
+    convergence information."""
     lib = system.lib
     n = b.numel()
     dev = b.device
@@ -161,6 +166,8 @@ def gmres(system, b, x=None, rtol=1e-8, atol=0.0, restart=50, max_it=5000,
         system.residual(b, x, r)
         beta = _norm(lib, r, work, scal)
         true_hist.append(beta)
+        if on_restart is not None:
+            on_restart(x, r)
         if its == 0:
             history.append(beta)
         if beta <= target:
@@ -183,6 +190,8 @@ def gmres(system, b, x=None, rtol=1e-8, atol=0.0, restart=50, max_it=5000,
             zj = Z[j] if flexible else z
             system.precondition(V[j], zj)
             system.matvec(zj, w)
+            if on_direction is not None:
+                on_direction(zj, w)
             check(lib.rtx_arnoldi_step(_vp(V), n, j + 1, n, _vp(w), _vp(hcol_d), _vp(work), st),
                   "rtx_arnoldi_step")
             hcol_h[:j + 2].copy_(hcol_d[:j + 2], non_blocking=True)
@@ -243,10 +252,12 @@ def hessenberg_singular_values(H, k):
 
 
 def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=50,
-           max_outer=50, max_inner=200):
+           max_outer=50, max_inner=200, flexible=False, after_outer=None):
     """FP64 iterative refinement: r = b - A x in FP64 (``system_hi``), inner
     correction solve A d = r by GMRES whose operator and preconditioner run in
-    reduced precision (``system_lo``; its Arnoldi is still FP64)."""
+    reduced precision (``system_lo``; its Arnoldi is still FP64).
+    ``flexible`` selects flexible inner GMRES (for a nonlinear
+    preconditioner); ``after_outer(outer, |r|)`` sees every FP64 residual."""
     lib = system_hi.lib
     n = b.numel()
     x = torch.zeros_like(b) if x is None else x
@@ -263,13 +274,16 @@ def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=
         system_hi.residual(b, x, r)
         rn = _norm(lib, r, work, scal)
         hist.append(rn)
+        if after_outer is not None:
+            after_outer(outer, rn)
         if rn <= rtol * bnorm:
             reason = 2
             break
         if outer == max_outer or not np.isfinite(rn):
             reason = -3 if np.isfinite(rn) else -9
             break
-        d, info = gmres(system_lo, r, None, rtol=inner_rtol, restart=restart, max_it=max_inner)
+        d, info = gmres(system_lo, r, None, rtol=inner_rtol, restart=restart, max_it=max_inner,
+                        flexible=flexible)
         inner_its.append(info["iteration_number"])
         check(lib.rtx_axpby(PREC_F64, n, 1.0, _vp(d), 1.0, _vp(x), stream_handle()), "axpby")
     torch.cuda.synchronize()

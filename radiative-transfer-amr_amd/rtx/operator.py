@@ -74,18 +74,32 @@ class TransportOperator:
         self.ndof = t.ndof
         f64 = C.c_double
         G_pool, G_off = t.G_pool(Phi)
+        self._G_off = G_off
         d, self._keep = build_desc(t, G_off)
         with torch.cuda.device(self.device):
             h = C.c_void_p()
             check(self.lib.rtx_plan_create(C.byref(d), C.byref(h)), "rtx_plan_create")
             self.plan = h
-            self.kappa_nodes = np.ascontiguousarray(t.eval_spatial(kappa))
-            self.sigma_nodes = np.ascontiguousarray(t.eval_spatial(sigma))
+        self._keep = []
+        self.G_pool = np.ascontiguousarray(G_pool)
+        self.set_coeffs(kappa, sigma)
+
+    def set_coeffs(self, kappa, sigma, Phi=None):
+        """Re-evaluate kappa(x, y), sigma(x, y) (and Phi, if given) at the
+        nodes and upload them; the mesh tables are kept."""
+        t = self.t
+        f64 = C.c_double
+        if Phi is not None:
+            G_pool, G_off = t.G_pool(Phi)
+            if not np.array_equal(G_off, self._G_off):
+                raise ValueError("a new Phi must keep the angular signatures of the plan")
             self.G_pool = np.ascontiguousarray(G_pool)
+        self.kappa_nodes = np.ascontiguousarray(t.eval_spatial(kappa))
+        self.sigma_nodes = np.ascontiguousarray(t.eval_spatial(sigma))
+        with torch.cuda.device(self.device):
             check(self.lib.rtx_plan_set_coeffs(
                 self.plan, t.nsp, ptr(self.kappa_nodes, f64), ptr(self.sigma_nodes, f64),
                 self.G_pool.size, ptr(self.G_pool, f64)), "rtx_plan_set_coeffs")
-        self._keep = []
 
     def __del__(self):
         try:

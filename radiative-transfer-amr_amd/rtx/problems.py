@@ -140,3 +140,70 @@ def two_scatterers(g=0.9, centres=((0.3, 0.6), (0.7, 0.4)), radius=0.1,
         return np.where(np.asarray(x) == 0.0, 1.0, 0.0) + 0.0 * (y + th)
 
     return Workload("two_scatterers", kappa, sigma, Phi, f, bcs)
+
+
+# ---------------------------------------------------------------- random fields
+def grf(n, length_scale, seed, pad=2):
+    """Zero-mean, unit-variance Gaussian random field on the cell centres of
+    an n x n grid over [0, 1]^2 with squared-exponential covariance
+    exp(-d^2 / (2 l^2)): white noise filtered in Fourier space on a grid
+    padded ``pad`` times (to weaken the periodic wrap), then standardised."""
+    m = n * pad
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal((m, m))
+    k = np.fft.fftfreq(m, d=1.0 / n)            # cycles per unit length
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    spec = np.exp(-2.0 * (np.pi * length_scale) ** 2 * (kx ** 2 + ky ** 2))
+    field = np.real(np.fft.ifft2(np.fft.fft2(noise) * np.sqrt(spec)))[:n, :n]
+    field -= field.mean()
+    sd = field.std()
+    return field / sd if sd > 0 else field
+
+
+class GridField:
+    """Bilinear interpolation of cell-centred values on an n x n grid over
+    [0, Lx] x [0, Ly] (clamped at the walls); callable as f(x, y)."""
+
+    def __init__(self, values, Ls=(1.0, 1.0)):
+        self.v = np.asarray(values, dtype=np.float64)
+        self.Ls = Ls
+
+    def __call__(self, x, y):
+        x, y = np.broadcast_arrays(np.asarray(x, dtype=float), np.asarray(y, dtype=float))
+        nx, ny = self.v.shape
+        u = np.clip(x / self.Ls[0] * nx - 0.5, 0.0, nx - 1.0)
+        w = np.clip(y / self.Ls[1] * ny - 0.5, 0.0, ny - 1.0)
+        i0 = np.minimum(np.floor(u).astype(int), nx - 2) if nx > 1 else np.zeros_like(u, int)
+        j0 = np.minimum(np.floor(w).astype(int), ny - 2) if ny > 1 else np.zeros_like(w, int)
+        fu, fw = u - i0, w - j0
+        v = self.v
+        i1, j1 = np.minimum(i0 + 1, nx - 1), np.minimum(j0 + 1, ny - 1)
+        return ((1 - fu) * (1 - fw) * v[i0, j0] + fu * (1 - fw) * v[i1, j0]
+                + (1 - fu) * fw * v[i0, j1] + fu * fw * v[i1, j1])
+
+
+def grf_scatterer(seed, n=64, length_scale=0.15, ratio_range=(0.9, 0.999), g=0.5,
+                  kappa_scale=1.0):
+    """log kappa ~ GRF(length scale 0.15, mean log(kappa_scale), variance 1)
+    on an n x n grid, sigma / kappa one scalar ~ U(ratio_range) (both from
+    ``seed``), Henyey-Greenstein g, unit isotropic source, vacuum inflow."""
+    logk = grf(n, length_scale, seed) + np.log(kappa_scale)
+    ratio = float(np.random.default_rng(seed + 7919).uniform(*ratio_range))
+    lk = GridField(logk)
+    Phi = henyey_greenstein(g)
+
+    def kappa(x, y):
+        return np.exp(lk(x, y))
+
+    def sigma(x, y):
+        return ratio * kappa(x, y)
+
+    def f(x, y, th):
+        return 1.0 + 0.0 * (x + y + th)
+
+    def bcs(x, y, th):
+        return 0.0 * (x + y + th)
+
+    wl = Workload("grf_scatterer", kappa, sigma, Phi, f, bcs)
+    wl.seed, wl.ratio, wl.logk = seed, ratio, logk
+    return wl
