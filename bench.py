@@ -11,11 +11,15 @@ A "step" is one application of the interior transport operator A_II
 FP64) to a vector already resident in HBM.  ``value`` is DOF/s per apply.
 After the timed steps the script also solves A_II x = rhs to a true FP64
 relative residual of 1e-8 with (a) plain FP64 GMRES + block-Jacobi and
-(b) FP64 iterative refinement around reduced-precision applies, and times
-the oracle's assembled CSR SpMV on a bounded sample on the host CPU.
+(b) FP64 iterative refinement around reduced-precision applies; trains the
+learned coarse preconditioner on solver-generated pairs of the random-field
+workload (64 x 64 x 16 cells, p = 4, 8,192,000 DOF) and times plain FP64,
+mixed and learned+mixed solves of an unseen field there; and times the
+oracle's assembled CSR SpMV on a bounded sample on the host CPU.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve]
-                       [--no-cpu] [--calib]
+                       [--no-cpu] [--no-learned] [--learned-fields N]
+                       [--learned-kappa-scale S] [--calib]
 """
 import argparse
 import json
@@ -38,6 +42,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-fields", type=int, default=20)
+    ap.add_argument("--learned-kappa-scale", type=float, default=1.0)
     ap.add_argument("--calib", action="store_true",
                     help="run one plain 1 GiB device copy after the timed region "
                          "(counter calibration for rocprofv3 --pmc)")
@@ -183,49 +190,115 @@ def main():
     }
     if not args.no_solve:
         out["solve"] = solve_section(op, mesh, wl)
+    if not args.no_learned:
+        del op
+        torch.cuda.empty_cache()
+        out["learned"] = learned_section(fields=args.learned_fields,
+                                         kappa_scale=args.learned_kappa_scale,
+                                         log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, wl)
     print(json.dumps(out), flush=True)
 
 
-def solve_section(op, mesh, wl):
-    """Time-to-1e-8 true FP64 relative residual on the workload's interior
-    system, rhs = f_I - A_IB b_B as in rt.Problem.solve."""
+def _rhs(op, wl):
+    """rhs = f_I - A_IB b_B (full-length, zero on the inflow boundary), as in
+    rt.Problem.solve."""
     import torch
-    from rtx.operator import OP_SYSTEM, PREC_F64, PREC_F32, PREC_MIXED
-    from rtx import krylov
-    dev = op.device
-    res = {}
-    f = torch.from_numpy(op.nodal(wl.f)).to(dev)
-    from rtx.operator import OP_QUADW
-    f = op.apply(f, ops=OP_QUADW)
-    b = torch.from_numpy(op.nodal(wl.bcs)).to(dev)
+    from rtx.operator import OP_SYSTEM, OP_QUADW
+    f = op.apply(torch.from_numpy(op.nodal(wl.f)).to(op.device), ops=OP_QUADW)
+    b = torch.from_numpy(op.nodal(wl.bcs)).to(op.device)
     bb = op.mask(b.clone(), keep_interior=False)
-    rhs = op.mask(f - op.apply(bb, ops=OP_SYSTEM), keep_interior=True)
-    for name, prec in (("fp64", PREC_F64), ("mixed", PREC_MIXED)):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        pc = krylov.BlockJacobi(op, keep_f32=prec != PREC_F64, keep_f64=prec == PREC_F64)
-        torch.cuda.synchronize()
-        t_pc = time.perf_counter() - t0
-        hi = krylov.InteriorSystem(op, PREC_F64, pc if prec == PREC_F64 else None, PREC_F64)
-        if prec == PREC_F64:
-            x, info = krylov.gmres(hi, rhs, None, rtol=1e-8, restart=50, max_it=3000)
-        else:
-            lo = krylov.InteriorSystem(op, prec, pc, PREC_F32)
-            x, info = krylov.refine(hi, lo, rhs, None, rtol=1e-8, inner_rtol=1e-3,
-                                    restart=50, max_inner=3000)
-        torch.cuda.synchronize()
-        total = time.perf_counter() - t0
-        res[name] = {"seconds": total, "pc_setup_seconds": t_pc,
-                     "iterations": int(info["iteration_number"]),
-                     "rel_residual": float(info["rel_residual"]),
-                     "converged": bool(info["converged_reason"] > 0)}
-        if name == "fp64":
-            x64 = x.clone()
-        else:
-            res[name]["rel_l2_vs_fp64"] = float((x - x64).norm() / x64.norm())
-        del pc
+    return op.mask(f - op.apply(bb, ops=OP_SYSTEM), keep_interior=True)
+
+
+def _timed_solve(op, rhs, precision, pc=None, bj=None, inner_rtol=1e-3):
+    import torch
+    from rtx import learned
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, info = learned.solve(op, rhs, None, pc=pc, bj=bj, precision=precision, rtol=1e-8,
+                            restart=50, inner_rtol=inner_rtol)
+    torch.cuda.synchronize()
+    return x, {"seconds": time.perf_counter() - t0,
+               "iterations": int(info["iteration_number"]),
+               "rel_residual": float(info["rel_residual"]),
+               "converged": bool(info["converged_reason"] > 0),
+               "fallback": bool(info.get("fallback", False))}
+
+
+def solve_section(op, mesh, wl):
+    """Time-to-1e-8 true FP64 relative residual on the apply workload's
+    interior system: plain FP64 GMRES(50) + block-Jacobi vs FP64 iterative
+    refinement around FP32 / BF16 applies with BF16 block-Jacobi inverses.
+    Times include the block-Jacobi factorisation."""
+    rhs = _rhs(op, wl)
+    res = {}
+    x64, res["fp64"] = _timed_solve(op, rhs, "fp64")
+    xm, res["mixed"] = _timed_solve(op, rhs, "mixed")
+    res["mixed"]["rel_l2_vs_fp64"] = float((xm - x64).norm() / x64.norm())
+    res["speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
+    return res
+
+
+def learned_section(fields=20, eval_seed=10_000, kappa_scale=1.0, log=None):
+    """The learned coarse preconditioner on the random-field workload: 64 x 64
+    columns x 16 angular cells, p = 4 (8,192,000 DOF), log kappa ~ GRF
+    (length 0.15, mean log kappa_scale, variance 1), sigma / kappa ~
+    U(0.9, 0.999), HG g = 0.5, unit source, vacuum inflow.  Pairs from
+    ``fields`` seeded fields (solver-generated), a 4-layer width-128 MLP
+    trained on the device, then fp64 / mixed / learned+mixed solves of an
+    unseen field (seed ``eval_seed``) to an FP64 residual of 1e-8."""
+    import torch
+    from rtx import problems, learned
+    from rtx.operator import TransportOperator
+    from rtx._lib import PREC_MIXED
+    t0 = time.perf_counter()
+    mesh = problems.uniform_mesh([1.0, 1.0], [5, 5, 5], nref_ang=4, nref_spt=6)
+
+    def field(s):
+        return problems.grf_scatterer(s, n=64, kappa_scale=kappa_scale)
+
+    wl0 = field(0)
+    op = TransportOperator(mesh, wl0.kappa, wl0.sigma, wl0.Phi)
+    grid = learned.CoarseGrid(op, 64)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    X, Y = learned.generate_pairs(op, grid, range(fields), cycles=12, restart=10, skip=1,
+                                  field=field, direction_stride=6, log=log)
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t0
+    nc = grid.nc
+    mlp = learned.MLP([5 * nc, 128, 128, 128, nc], op.device, seed=0)
+    mlp.n_moments = 3 * nc
+    t0 = time.perf_counter()
+    xs, ys, hist = learned.train(mlp, X, Y, epochs=60, batch=64, lr=1e-3, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    npairs = int(X.shape[0])
+    del X, Y
+    wl = field(eval_seed)
+    op.set_coeffs(wl.kappa, wl.sigma)
+    rhs = _rhs(op, wl)
+    res = {"workload": "grf scatterer 64x64x16 cells, p=4 (8,192,000 DOF), kappa_scale {}, "
+                       "seed {}, sigma/kappa {:.4f}".format(kappa_scale, eval_seed, wl.ratio),
+           "pairs": npairs, "fields": fields, "setup_seconds": t_setup,
+           "pair_generation_seconds": t_pairs, "training_seconds": t_train,
+           "validation_rel_error": hist[-1][1]}
+    x64, res["fp64"] = _timed_solve(op, rhs, "fp64")
+    xm, res["mixed"] = _timed_solve(op, rhs, "mixed")
+    model = learned.CoarseModel(mlp, 3 * nc, xs, ys)
+    model.set_coefficients(grid.coefficient_channels(wl.kappa, wl.sigma))
+    from rtx.krylov import BlockJacobi
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bj = BlockJacobi(op, keep_f64=False, keep_bf16=True)
+    pc = learned.TwoLevelPC(op, bj, grid, model, apply_precision=PREC_MIXED)
+    xl, res["learned_mixed"] = _timed_solve(op, rhs, "mixed", pc=pc, bj=bj)
+    res["learned_mixed"]["seconds"] += time.perf_counter() - t0 - res["learned_mixed"]["seconds"]
+    for k, x in (("mixed", xm), ("learned_mixed", xl)):
+        res[k]["rel_l2_vs_fp64"] = float((x - x64).norm() / x64.norm())
+    res["speedup_learned_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["learned_mixed"]["seconds"]
     res["speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
     return res
 

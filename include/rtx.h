@@ -181,8 +181,10 @@ int rtx_bjacobi_size(rtx_plan *plan, int64_t *n_entries);
    not fit LDS.  Synchronises the stream; RTX_ERR_NUMERIC on a singular block. */
 int rtx_bjacobi_factor(rtx_plan *plan, int restrict_interior, double *d_inv, float *d_inv_f32,
                        double *d_scratch, void *stream);
-/* z = blockdiag(inv) r, precision RTX_PREC_F64 (double inverses, vectors) or
-   RTX_PREC_F32 (float inverses, vectors). */
+/* z = blockdiag(inv) r, precision RTX_PREC_F64 (double inverses, vectors),
+   RTX_PREC_F32 (float inverses, vectors) or RTX_PREC_MIXED (BF16 inverses -
+   rtx_mlp_to_bf16 of the FP32 copy - and float vectors; the factorisation
+   itself is always FP64). */
 int rtx_bjacobi_apply(rtx_plan *plan, int precision, const void *d_inv, const void *d_r,
                       void *d_z, void *stream);
 
@@ -191,11 +193,27 @@ int rtx_bjacobi_apply(rtx_plan *plan, int precision, const void *d_inv, const vo
 /* Workspace (doubles) needed by rtx_arnoldi_step / rtx_mdot. */
 int rtx_arnoldi_workspace(int max_vecs, int64_t *n_doubles);
 /* Orthogonalise d_w against the nv columns of d_V (length n, stride ld) by
-   classical Gram-Schmidt with one conditional re-orthogonalisation, store
-   w / |w| as column nv of d_V and the Hessenberg column (nv + 1 values) in
-   d_hcol.  Asynchronous. */
+   classical Gram-Schmidt, store w / |w| as column nv of d_V and the
+   Hessenberg column (nv + 1 values) in d_hcol.  refine 0: one pass (PETSc's
+   default KSPGMRESClassicalGramSchmidtOrthogonalization without refinement,
+   which src/rt/solve.py:81-100 relies on); 1: a second pass when |w| dropped
+   below 1/sqrt(2) of its value (decided on the device); 2: always two
+   passes.  Asynchronous. */
 int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n, double *d_w,
-                     double *d_hcol, double *d_work, void *stream);
+                     double *d_hcol, double *d_work, int refine, void *stream);
+/* Least-squares state of one restart cycle of m steps kept on the device
+   (Givens QR of the Hessenberg matrix; PETSc keeps it on the host, here it
+   stays on the GPU so whole chunks of iterations can run as one HIP graph).
+   Layout (doubles): cs[m], sn[m], g[m+1], R[m*m] (column j at R + j*m),
+   H[(m+1)*m] (raw Hessenberg, column j at H + j*(m+1)), est[m]. */
+int rtx_gmres_state_size(int m, int64_t *n_doubles);
+/* g = (beta, 0, ..., 0). */
+int rtx_gmres_begin(int m, double beta, double *d_state, void *stream);
+/* Fold Hessenberg column j (d_hcol[0..j+1], as written by rtx_arnoldi_step)
+   into the factorisation; est[j] = residual norm after j + 1 steps. */
+int rtx_gmres_givens(int j, int m, const double *d_hcol, double *d_state, void *stream);
+/* d_y[0..k-1] = minimiser over the first k directions (back substitution). */
+int rtx_gmres_solve(int k, int m, const double *d_state, double *d_y, void *stream);
 /* d_out[j] = V_j . w, j < nv. */
 int rtx_mdot(const double *d_V, int64_t ld, int nv, int64_t n, const double *d_w,
              double *d_out, double *d_work, void *stream);

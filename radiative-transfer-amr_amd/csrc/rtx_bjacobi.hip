@@ -241,6 +241,12 @@ __global__ void k_to_f32_bj(const double *__restrict__ s, float *__restrict__ d,
 
 // z_K = inv_K r_K, inv column-major.  One workgroup per cell; the columns
 // are split across the four wavefronts' partial sums when n is small.
+// BF16 storage of an inverse entry, widened on load.
+struct bj_bf16 {
+    uint16_t v;
+    __device__ operator double() const { return double(__uint_as_float(uint32_t(v) << 16)); }
+};
+
 template <typename T, typename M>
 __global__ __launch_bounds__(BJ_THREADS) void k_bj_apply(DevMesh m, const int64_t *inv_off,
                                                         const M *__restrict__ inv,
@@ -371,6 +377,9 @@ extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, 
     else if (precision == RTX_PREC_F32)
         k_bj_apply<float, float><<<p->m.ncell, BJ_THREADS, 0, st>>>(
             p->m, p->bj_off, (const float *)d_inv, (const float *)d_r, (float *)d_z);
+    else if (precision == RTX_PREC_MIXED)
+        k_bj_apply<float, bj_bf16><<<p->m.ncell, BJ_THREADS, 0, st>>>(
+            p->m, p->bj_off, (const bj_bf16 *)d_inv, (const float *)d_r, (float *)d_z);
     else {
         set_error("rtx_bjacobi_apply: unsupported precision %d", precision);
         return RTX_ERR_ARG;

@@ -112,6 +112,10 @@ __global__ void k_reorth_test(const double *state, int *flag) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *flag = (state[1] >= REORTH * state[0]) ? 1 : 0;
 }
 
+__global__ void k_set_flag(int *flag, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v;
+}
+
 // v_out = w / sqrt(nrm2); hcol[nv] = sqrt(nrm2).  nrm2 comes from state.
 __global__ void k_normalize(const double *__restrict__ w, double *__restrict__ v_out, int64_t n,
                             const double *__restrict__ nrm2, double *__restrict__ hcol, int nv) {
@@ -146,6 +150,61 @@ __global__ void k_axpby(int64_t n, double a, const T *__restrict__ x, double b,
         y[i] = T(a * double(x[i]) + (b == 0.0 ? 0.0 : b * double(y[i])));
 }
 
+// ---- least-squares state of one GMRES restart cycle of m steps, on the
+// device (doubles): cs[m], sn[m], g[m + 1], R[m * m] (column j at R + j m),
+// H[(m + 1) * m] (raw Hessenberg, column j at H + j (m + 1)), est[m].
+struct GState {
+    double *cs, *sn, *g, *R, *H, *est;
+    __device__ GState(double *s, int m)
+        : cs(s), sn(s + m), g(s + 2 * m), R(s + 3 * m + 1), H(s + 3 * m + 1 + m * m),
+          est(s + 3 * m + 1 + m * m + (m + 1) * m) {}
+};
+
+__global__ void k_gmres_begin(int m, double beta, double *state) {
+    GState s(state, m);
+    for (int i = threadIdx.x; i <= m; i += blockDim.x) s.g[i] = (i == 0) ? beta : 0.0;
+}
+
+// Fold Hessenberg column j (hcol[0..j+1]) into the QR factorisation: apply
+// the previous rotations, form rotation j, update g; est[j] = |g[j + 1]|
+// is the residual norm of the minimiser over the first j + 1 directions.
+__global__ void k_gmres_givens(int j, int m, const double *__restrict__ hcol, double *state) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    GState s(state, m);
+    double *Rj = s.R + int64_t(j) * m;
+    double *Hj = s.H + int64_t(j) * (m + 1);
+    for (int i = 0; i <= j + 1; ++i) Hj[i] = hcol[i];
+    for (int i = 0; i <= j; ++i) Rj[i] = hcol[i];
+    const double hj1 = hcol[j + 1];
+    for (int i = 0; i < j; ++i) {
+        const double a = Rj[i], b = Rj[i + 1];
+        Rj[i] = s.cs[i] * a + s.sn[i] * b;
+        Rj[i + 1] = -s.sn[i] * a + s.cs[i] * b;
+    }
+    const double den = hypot(Rj[j], hj1);
+    const double c = den != 0.0 ? Rj[j] / den : 1.0;
+    const double sn = den != 0.0 ? hj1 / den : 0.0;
+    s.cs[j] = c;
+    s.sn[j] = sn;
+    Rj[j] = den;
+    const double gj = s.g[j];
+    s.g[j + 1] = -sn * gj;
+    s.g[j] = c * gj;
+    s.est[j] = fabs(s.g[j + 1]);
+}
+
+// y = R_k^{-1} g_k (back substitution over the first k columns).
+__global__ void k_gmres_solve(int k, int m, const double *state, double *__restrict__ y) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    GState s(const_cast<double *>(state), m);
+    for (int i = k - 1; i >= 0; --i) {
+        double acc = s.g[i];
+        for (int l = i + 1; l < k; ++l) acc -= s.R[int64_t(l) * m + i] * y[l];
+        const double d = s.R[int64_t(i) * m + i];
+        y[i] = d != 0.0 ? acc / d : 0.0;
+    }
+}
+
 template <typename S, typename D>
 __global__ void k_convert(int64_t n, const S *__restrict__ s, D *__restrict__ d) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -166,13 +225,15 @@ extern "C" int rtx_arnoldi_workspace(int max_vecs, int64_t *n_doubles) {
 }
 
 /* One Arnoldi step: orthogonalise w against V_0..V_{nv-1} (columns of length n,
-   stride ld), re-orthogonalise once when the norm dropped by more than the
-   "twice is enough" factor, normalise into V_nv and write the Hessenberg
-   column h_0..h_{nv-1}, h_nv = |w| to d_hcol (device, nv + 1 doubles).
-   w is overwritten.  Everything is enqueued on `stream`; no host sync. */
+   stride ld) by classical Gram-Schmidt; refine 0: one pass, 1: a second pass
+   when the norm dropped by more than the "twice is enough" factor, 2: always
+   two passes; normalise into V_nv and write the Hessenberg column
+   h_0..h_{nv-1}, h_nv = |w| to d_hcol (device, nv + 1 doubles).  w is
+   overwritten.  Everything is enqueued on `stream`; no host sync. */
 extern "C" int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n, double *d_w,
-                                double *d_hcol, double *d_work, void *stream) {
+                                double *d_hcol, double *d_work, int refine, void *stream) {
     RTX_CHECK(d_V && d_w && d_hcol && d_work, "rtx_arnoldi_step: null argument");
+    RTX_CHECK(refine >= 0 && refine <= 2, "rtx_arnoldi_step: refine must be 0, 1 or 2");
     RTX_CHECK(nv >= 1 && nv < KMAXV - 1, "rtx_arnoldi_step: nv=%d outside [1, %d)", nv,
               KMAXV - 1);
     RTX_CHECK(n >= 1 && ld >= n, "rtx_arnoldi_step: bad n/ld");
@@ -184,13 +245,22 @@ extern "C" int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n
     const int nchunk = (nv + KCH - 1) / KCH;
     double *V_out = const_cast<double *>(d_V) + int64_t(nv) * ld;
     // pass 1: h = V^T w, |w|^2
-    k_mdot<<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, part, 1, nullptr);
+    k_mdot<<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, part, refine == 1 ? 1 : 0,
+                                             nullptr);
     k_reduce<<<nv, KT, 0, st>>>(part, KNB, d_hcol, 0, nullptr);
-    k_reduce<<<1, KT, 0, st>>>(part + int64_t(nv) * KNB, KNB, state + 0, 0, nullptr);
+    if (refine == 1) k_reduce<<<1, KT, 0, st>>>(part + int64_t(nv) * KNB, KNB, state + 0, 0, nullptr);
     // w -= V h, |w'|^2
     k_update<<<KNB, KT, 0, st>>>(d_V, ld, nv, d_hcol, d_w, n, part2, nullptr);
     k_reduce<<<1, KT, 0, st>>>(part2, KNB, state + 1, 0, nullptr);
-    k_reorth_test<<<1, 1, 0, st>>>(state, flag);
+    if (refine == 0) {     // classical Gram-Schmidt, one pass (PETSc's default)
+        k_normalize<<<2048, KT, 0, st>>>(d_w, V_out, n, state + 1, d_hcol, nv);
+        RTX_HIP(hipGetLastError());
+        return RTX_OK;
+    }
+    if (refine == 1)
+        k_reorth_test<<<1, 1, 0, st>>>(state, flag);
+    else
+        k_set_flag<<<1, 1, 0, st>>>(flag, 0);
     // optional pass 2 (skipped on the device when flag == 1)
     k_mdot<<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, part, 0, flag);
     // accumulate the correction into hcol, keep it separately for the update
@@ -200,6 +270,37 @@ extern "C" int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n
     k_reduce<<<1, KT, 0, st>>>(part2, KNB, state + 1, 0, flag);
     k_reduce<<<nv, KT, 0, st>>>(hcorr, 1, d_hcol, 1, flag);  // hcol += hcorr
     k_normalize<<<2048, KT, 0, st>>>(d_w, V_out, n, state + 1, d_hcol, nv);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+extern "C" int rtx_gmres_state_size(int m, int64_t *n_doubles) {
+    RTX_CHECK(n_doubles && m >= 1 && m < KMAXV - 1, "rtx_gmres_state_size: m outside [1, %d)",
+              KMAXV - 1);
+    *n_doubles = 3 * int64_t(m) + 1 + int64_t(m) * m + int64_t(m + 1) * m + m;
+    return RTX_OK;
+}
+
+extern "C" int rtx_gmres_begin(int m, double beta, double *d_state, void *stream) {
+    RTX_CHECK(d_state && m >= 1 && m < KMAXV - 1, "rtx_gmres_begin: bad argument");
+    k_gmres_begin<<<1, 256, 0, static_cast<hipStream_t>(stream)>>>(m, beta, d_state);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+extern "C" int rtx_gmres_givens(int j, int m, const double *d_hcol, double *d_state,
+                                void *stream) {
+    RTX_CHECK(d_hcol && d_state && m >= 1 && m < KMAXV - 1 && j >= 0 && j < m,
+              "rtx_gmres_givens: bad argument (j=%d, m=%d)", j, m);
+    k_gmres_givens<<<1, 64, 0, static_cast<hipStream_t>(stream)>>>(j, m, d_hcol, d_state);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+extern "C" int rtx_gmres_solve(int k, int m, const double *d_state, double *d_y, void *stream) {
+    RTX_CHECK(d_state && d_y && m >= 1 && m < KMAXV - 1 && k >= 1 && k <= m,
+              "rtx_gmres_solve: bad argument (k=%d, m=%d)", k, m);
+    k_gmres_solve<<<1, 64, 0, static_cast<hipStream_t>(stream)>>>(k, m, d_state, d_y);
     RTX_HIP(hipGetLastError());
     return RTX_OK;
 }

@@ -142,7 +142,8 @@ class Problem:
             "ndof": int(op.ndof), "ndof_interior": int(op.mask(torch.ones(op.ndof, dtype=torch.float64, device=dev)).sum().item()),
             "block_size": 1,
             "extreme_singular_values": info.get("extreme_singular_values", [float("nan")] * 2),
-            "pc_bytes": int(pc.bytes_per_apply) if pc is not None else 0,
+            "pc_bytes": int(pc.bytes_per_apply(PREC_F64 if prec == PREC_F64 else PREC_F32))
+            if pc is not None else 0,
         }
         if kw["verbose"]:
             print("Solve {} - {} finished: reason {}, {} iterations, residual {:.4e}, "

@@ -65,7 +65,7 @@ SIGNATURES = [
                                     C.c_void_p]),
     ("rtx_arnoldi_workspace", C.c_int, [C.c_int, C.POINTER(C.c_int64)]),
     ("rtx_arnoldi_step", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_int64, C.c_void_p,
-                                   C.c_void_p, C.c_void_p, C.c_void_p]),
+                                   C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     ("rtx_mdot", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_int64, C.c_void_p, C.c_void_p,
                            C.c_void_p, C.c_void_p]),
     ("rtx_combine", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64,
@@ -73,6 +73,10 @@ SIGNATURES = [
     ("rtx_axpby", C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_void_p, C.c_double,
                             C.c_void_p, C.c_void_p]),
     ("rtx_convert", C.c_int, [C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtx_gmres_state_size", C.c_int, [C.c_int, C.POINTER(C.c_int64)]),
+    ("rtx_gmres_begin", C.c_int, [C.c_int, C.c_double, C.c_void_p, C.c_void_p]),
+    ("rtx_gmres_givens", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtx_gmres_solve", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     # learned coarse correction
     ("rtx_coarse_setup", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p]),

@@ -36,38 +36,47 @@ def _vp(t):
 class BlockJacobi:
     """Dense inverses of every cell's interior diagonal block, on the device."""
 
-    def __init__(self, op, restrict=True, keep_f32=False, keep_f64=True):
+    def __init__(self, op, restrict=True, keep_f32=False, keep_f64=True, keep_bf16=False):
         self.op, self.lib = op, op.lib
         n = C.c_int64()
         check(self.lib.rtx_bjacobi_size(op.plan, C.byref(n)), "rtx_bjacobi_size")
         self.entries = int(n.value)
         dev = op.device
         self.inv = torch.empty(self.entries, dtype=torch.float64, device=dev)
-        self.inv32 = torch.empty(self.entries, dtype=torch.float32, device=dev) if keep_f32 else None
+        need32 = keep_f32 or keep_bf16       # the BF16 copy is rounded from the FP32 one
+        self.inv32 = torch.empty(self.entries, dtype=torch.float32, device=dev) if need32 else None
         st = stream_handle()
         rc = self.lib.rtx_bjacobi_factor(op.plan, 1 if restrict else 0, _vp(self.inv),
-                                         _vp(self.inv32) if keep_f32 else None, None, st)
+                                         _vp(self.inv32) if need32 else None, None, st)
         if rc == -1 and b"LDS" in self.lib.rtx_last_error():
             scratch = torch.empty_like(self.inv)
             rc = self.lib.rtx_bjacobi_factor(op.plan, 1 if restrict else 0, _vp(self.inv),
-                                             _vp(self.inv32) if keep_f32 else None,
+                                             _vp(self.inv32) if need32 else None,
                                              _vp(scratch), st)
             del scratch
         check(rc, "rtx_bjacobi_factor")
+        self.inv16 = None
+        if keep_bf16:
+            self.inv16 = torch.empty(self.entries, dtype=torch.int16, device=dev)
+            check(self.lib.rtx_mlp_to_bf16(self.entries, _vp(self.inv32), _vp(self.inv16), st),
+                  "rtx_mlp_to_bf16")
+            if not keep_f32:
+                torch.cuda.current_stream().synchronize()
+                self.inv32 = None
         if not keep_f64:
             self.inv = None
 
-    @property
-    def bytes_per_apply(self):
-        return self.entries * (8 if self.inv is not None else 4)
+    def bytes_per_apply(self, precision=PREC_F64):
+        return self.entries * {PREC_F64: 8, PREC_F32: 4, PREC_MIXED: 2}[precision]
 
     def apply(self, r, z, precision=PREC_F64, stream=None):
-        inv = self.inv if precision == PREC_F64 else self.inv32
+        """precision PREC_F64: FP64 inverses and vectors; PREC_F32: FP32;
+        PREC_MIXED: BF16 inverses, FP32 vectors."""
+        inv = {PREC_F64: self.inv, PREC_F32: self.inv32, PREC_MIXED: self.inv16}[precision]
         if inv is None:
             raise ValueError("block-Jacobi inverses for this precision were not kept")
-        check(self.lib.rtx_bjacobi_apply(self.op.plan, PREC_F64 if precision == PREC_F64 else PREC_F32,
-                                         _vp(inv), _vp(r), _vp(z), stream_handle(stream)),
-              "rtx_bjacobi_apply")
+        check(self.lib.rtx_bjacobi_apply(self.op.plan, precision, _vp(inv), _vp(r), _vp(z),
+                                         stream_handle(stream)), "rtx_bjacobi_apply")
         return z
 
 
@@ -102,7 +111,7 @@ class InteriorSystem:
             return self.pc.apply(r, z, PREC_F64)
         st = stream_handle()
         check(self.lib.rtx_convert(0, r.numel(), _vp(r), _vp(self.x32), st), "convert")
-        self.pc.apply(self.x32, self.y32, PREC_F32)
+        self.pc.apply(self.x32, self.y32, self.pc_precision)
         check(self.lib.rtx_convert(1, r.numel(), _vp(self.y32), _vp(z), st), "convert")
         return z
 
@@ -124,17 +133,51 @@ def _norm(lib, v, work, out):
     return float(out[0].item()) ** 0.5
 
 
+class _ChunkGraphs:
+    """HIP graphs of chunks of GMRES iterations, keyed by (j0, j1).  A chunk
+    runs eagerly the first time it is needed (that also loads every kernel
+    it uses) and is captured the next time; replays reuse the same device
+    buffers, so a graph stays valid for the whole solve."""
+
+    def __init__(self):
+        self.seen = set()
+        self.graphs = {}
+
+    def run(self, key, body, enabled):
+        if not enabled:
+            body()
+            return
+        g = self.graphs.get(key)
+        if g is None:
+            if key not in self.seen:
+                self.seen.add(key)
+                body()
+                return
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                body()
+            self.graphs[key] = g
+        g.replay()
+
+
 def gmres(system, b, x=None, rtol=1e-8, atol=0.0, restart=50, max_it=5000,
           flexible=False, callback=None, true_residual_every_restart=True, on_restart=None,
-          on_direction=None):
+          on_direction=None, chunk=10, graphs=True, refine=0):
     """Right-preconditioned restarted GMRES on ``system`` (InteriorSystem).
 
     Converged when |b - A x| <= max(rtol |b|, atol) for the FP64 true
-    residual.  ``on_restart(x, r)`` (optional) sees the iterate and its true
-    residual at the start of every cycle; ``on_direction(z, w)`` sees every
-    preconditioned direction z and w = A z before orthogonalisation (an exact
-    residual / correction pair: A z = w).  Returns (x, info) with PETSc-like
-    convergence information."""
+    residual.  The Arnoldi basis, the orthogonalisation, the Givens QR of the
+    Hessenberg matrix and the small triangular solve all stay on the device;
+    the host reads the residual estimates once per ``chunk`` iterations, and
+    (``graphs``) each chunk of iterations - preconditioner, operator, Arnoldi
+    step, Givens update - is replayed as one HIP graph.  ``on_restart(x, r)``
+    sees the iterate and its true residual at the start of every cycle;
+    ``on_direction(z, w)`` sees every preconditioned direction z and
+    w = A z before orthogonalisation (an exact residual / correction pair);
+    either hook (or ``callback``) runs the iterations eagerly.  ``refine``
+    selects the Gram-Schmidt refinement of ``rtx_arnoldi_step`` (0: none,
+    PETSc's default; 1: when needed; 2: always).  Returns (x, info) with
+    PETSc-like convergence information."""
     lib = system.lib
     n = b.numel()
     dev = b.device
@@ -148,12 +191,28 @@ def gmres(system, b, x=None, rtol=1e-8, atol=0.0, restart=50, max_it=5000,
     ws = C.c_int64()
     check(lib.rtx_arnoldi_workspace(m + 1, C.byref(ws)), "workspace")
     work = torch.empty(int(ws.value), dtype=torch.float64, device=dev)
-    hcol_d = torch.zeros(m + 2, dtype=torch.float64, device=dev)
-    hcol_h = torch.zeros(m + 2, dtype=torch.float64).pin_memory()
+    hcol = torch.zeros(m + 2, dtype=torch.float64, device=dev)
+    check(lib.rtx_gmres_state_size(m, C.byref(ws)), "gmres state size")
+    state = torch.zeros(int(ws.value), dtype=torch.float64, device=dev)
+    est_off = 3 * m + 1 + m * m + (m + 1) * m
+    H_off = 3 * m + 1 + m * m
+    est_h = torch.zeros(m, dtype=torch.float64).pin_memory()
     ydev = torch.empty(m, dtype=torch.float64, device=dev)
     scal = torch.empty(2, dtype=torch.float64, device=dev)
-    st = stream_handle()
-    stream = torch.cuda.current_stream()
+    use_graphs = graphs and callback is None and on_direction is None
+    chunks = _ChunkGraphs()
+
+    def steps(j0, j1):
+        st = stream_handle()
+        for j in range(j0, j1):
+            zj = Z[j] if flexible else z
+            system.precondition(V[j], zj)
+            system.matvec(zj, w)
+            if on_direction is not None:
+                on_direction(zj, w)
+            check(lib.rtx_arnoldi_step(_vp(V), n, j + 1, n, _vp(w), _vp(hcol), _vp(work),
+                                       refine, st), "rtx_arnoldi_step")
+            check(lib.rtx_gmres_givens(j, m, _vp(hcol), _vp(state), st), "rtx_gmres_givens")
 
     bnorm = _norm(lib, b, work, scal)
     target = max(rtol * bnorm, atol)
@@ -179,66 +238,60 @@ def gmres(system, b, x=None, rtol=1e-8, atol=0.0, restart=50, max_it=5000,
         if not np.isfinite(beta):
             reason = -9                                   # KSP_DIVERGED_NANORINF
             break
+        st = stream_handle()
         check(lib.rtx_axpby(PREC_F64, n, 1.0 / beta, _vp(r), 0.0, _vp(V[0]), st), "axpby")
-        H = np.zeros((m + 1, m))          # raw Arnoldi (Hessenberg) matrix
-        Rf = np.zeros((m, m))             # its triangular factor
-        cs, sn = np.zeros(m), np.zeros(m)
-        g = np.zeros(m + 1)
-        g[0] = beta
-        k = 0
-        for j in range(m):
-            zj = Z[j] if flexible else z
-            system.precondition(V[j], zj)
-            system.matvec(zj, w)
-            if on_direction is not None:
-                on_direction(zj, w)
-            check(lib.rtx_arnoldi_step(_vp(V), n, j + 1, n, _vp(w), _vp(hcol_d), _vp(work), st),
-                  "rtx_arnoldi_step")
-            hcol_h[:j + 2].copy_(hcol_d[:j + 2], non_blocking=True)
-            stream.synchronize()
-            h = hcol_h[:j + 2].numpy().copy()
-            H[:j + 2, j] = h
-            for i in range(j):
-                t = cs[i] * h[i] + sn[i] * h[i + 1]
-                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1]
-                h[i] = t
-            den = np.hypot(h[j], h[j + 1])
-            if den == 0.0:
-                cs[j], sn[j] = 1.0, 0.0
-            else:
-                cs[j], sn[j] = h[j] / den, h[j + 1] / den
-            h[j] = den
-            h[j + 1] = 0.0
-            Rf[:j + 1, j] = h[:j + 1]
-            g[j + 1] = -sn[j] * g[j]
-            g[j] = cs[j] * g[j]
-            k = j + 1
-            its += 1
-            est = abs(g[j + 1])
-            history.append(est)
-            if callback is not None:
-                callback(its, est)
-            if est <= target or its >= max_it or hcol_h[j + 1] == 0.0:
+        check(lib.rtx_gmres_begin(m, beta, _vp(state), st), "rtx_gmres_begin")
+        j, k = 0, 0
+        while j < m and its < max_it:
+            j1 = min(j + int(chunk), m, j + (max_it - its))
+            chunks.run((j, j1), lambda a=j, bb=j1: steps(a, bb), use_graphs)
+            est_h[j:j1].copy_(state[est_off + j:est_off + j1], non_blocking=True)
+            torch.cuda.current_stream().synchronize()
+            e = est_h[j:j1].numpy()
+            stop = None
+            for i in range(j1 - j):
+                if not np.isfinite(e[i]):
+                    stop = i               # breakdown: keep the directions before it
+                    break
+                history.append(float(e[i]))
+                if callback is not None:
+                    callback(its + i + 1, float(e[i]))
+                if e[i] <= target:
+                    stop = i + 1
+                    break
+            if stop is not None:
+                its += stop
+                k = j + stop
                 break
-        H_last = (H, k)
-        # solve R y = g and update x
-        y = np.linalg.solve(np.triu(Rf[:k, :k]), g[:k]) if k > 0 else np.zeros(0)
-        ydev[:k].copy_(torch.from_numpy(y))
-        if flexible:
-            check(lib.rtx_combine(_vp(Z), n, k, _vp(ydev), n, _vp(x), 1, st), "combine")
-        else:
-            check(lib.rtx_combine(_vp(V), n, k, _vp(ydev), n, _vp(w), 0, st), "combine")
-            system.precondition(w, z)
-            check(lib.rtx_axpby(PREC_F64, n, 1.0, _vp(z), 1.0, _vp(x), st), "axpby")
+            its += j1 - j
+            j = k = j1
+        H_last = (state, H_off, k)
+        if k > 0:
+            check(lib.rtx_gmres_solve(k, m, _vp(state), _vp(ydev), st), "rtx_gmres_solve")
+            if flexible:
+                check(lib.rtx_combine(_vp(Z), n, k, _vp(ydev), n, _vp(x), 1, st), "combine")
+            else:
+                check(lib.rtx_combine(_vp(V), n, k, _vp(ydev), n, _vp(w), 0, st), "combine")
+                system.precondition(w, z)
+                check(lib.rtx_axpby(PREC_F64, n, 1.0, _vp(z), 1.0, _vp(x), st), "axpby")
+        elif stop is not None:
+            reason = -9
+            break
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    sv = hessenberg_singular_values(*H_last) if H_last is not None else [float("nan")] * 2
+    if H_last is not None and H_last[2] > 0:
+        st_, off, k = H_last
+        Hd = st_[off:off + (m + 1) * m].view(m, m + 1).cpu().numpy().T
+        sv = hessenberg_singular_values(Hd, k)
+    else:
+        sv = [float("nan")] * 2
     info = GMRESResult(converged_reason=reason, convergence_history=history,
                        iteration_number=its, res_f=true_hist[-1], res_best=min(true_hist),
                        true_residual_history=true_hist, rtol=rtol, atol=atol,
                        divtol=float(np.sqrt(np.finfo(np.float64).max)), max_it=max_it,
                        bnorm=bnorm, rel_residual=true_hist[-1] / bnorm if bnorm > 0 else 0.0,
-                       seconds=elapsed, restart=m, extreme_singular_values=sv)
+                       seconds=elapsed, restart=m, extreme_singular_values=sv,
+                       graphs=len(chunks.graphs))
     return x, info
 
 

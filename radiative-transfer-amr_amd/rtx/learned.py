@@ -518,15 +518,19 @@ def validation_error(mlp, X, Y, idx, colscale, y_scale, batch=64):
 
 # ------------------------------------------------------------------- solve
 def solve(op, rhs, x0=None, pc=None, bj=None, precision="mixed", rtol=1e-8, restart=50,
-          max_it=5000, inner_rtol=1e-2, max_outer=60):
+          max_it=5000, inner_rtol=1e-2, max_outer=60, bj_precision=PREC_MIXED):
     """Interior-system solve to an FP64 relative residual ``rtol`` with the
     two-level preconditioner ``pc`` (a TwoLevelPC) or plain block-Jacobi
     (pc None).  precision "mixed": FP64 iterative refinement around flexible
-    GMRES with FP32 / BF16 applies; "fp64": flexible GMRES in FP64.  The FP64
-    true residual judges every outer step: when a step that used the coarse
-    correction fails to reduce it, the correction is switched off."""
+    GMRES with FP32 / BF16 applies and block-Jacobi inverses stored in
+    ``bj_precision`` (PREC_MIXED: BF16, PREC_F32: FP32); "fp64": flexible
+    GMRES in FP64.  The FP64 true residual judges every outer step: when a
+    step that used the coarse correction fails to reduce it, the correction
+    is switched off."""
     if bj is None:
-        bj = BlockJacobi(op, keep_f32=precision != "fp64", keep_f64=precision == "fp64")
+        bj = BlockJacobi(op, keep_f32=precision != "fp64" and bj_precision == PREC_F32,
+                         keep_bf16=precision != "fp64" and bj_precision == PREC_MIXED,
+                         keep_f64=precision == "fp64")
     the_pc = pc if pc is not None else bj
     if precision == "fp64":
         system = InteriorSystem(op, PREC_F64, the_pc, PREC_F64)
@@ -536,7 +540,7 @@ def solve(op, rhs, x0=None, pc=None, bj=None, precision="mixed", rtol=1e-8, rest
         info["fallback"] = judge.fell_back
         return x, info
     hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
-    lo = InteriorSystem(op, PREC_MIXED, the_pc, PREC_F32)
+    lo = InteriorSystem(op, PREC_MIXED, the_pc, bj_precision)
     judge = _Judge(pc)
     x, info = refine(hi, lo, rhs, x0, rtol=rtol, inner_rtol=inner_rtol, restart=restart,
                      max_outer=max_outer, max_inner=max_it, flexible=pc is not None,

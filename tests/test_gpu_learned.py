@@ -276,7 +276,7 @@ def test_learned_preconditioner_gain(cuda, trained):
     wl = _diffusive(10_000, 16)
     op.set_coeffs(wl.kappa, wl.sigma)
     rhs = _rhs(op, wl)
-    bj = BlockJacobi(op, keep_f32=True, keep_f64=True)
+    bj = BlockJacobi(op, keep_f32=True, keep_f64=True, keep_bf16=True)
     x_bj, info_bj = learned.solve(op, rhs, None, pc=None, bj=bj, precision="fp64", rtol=1e-8)
     model = learned.CoarseModel(mlp, 3 * grid.nc, xs, ys)
     model.set_coefficients(grid.coefficient_channels(wl.kappa, wl.sigma))
@@ -307,7 +307,7 @@ def test_untrained_network_falls_back_safely(cuda, trained):
     wl = _diffusive(10_001, 16)
     op.set_coeffs(wl.kappa, wl.sigma)
     rhs = _rhs(op, wl)
-    bj = BlockJacobi(op, keep_f32=True, keep_f64=True)
+    bj = BlockJacobi(op, keep_f32=True, keep_f64=True, keep_bf16=True)
     raw = learned.MLP([5 * grid.nc, 128, 128, 128, grid.nc], "cuda", seed=7)
     model = learned.CoarseModel(raw, 3 * grid.nc, 1.0, -50.0)     # wrong sign and scale
     model.set_coefficients(grid.coefficient_channels(wl.kappa, wl.sigma))

@@ -96,3 +96,48 @@ def test_refinement_mixed_reaches_fp64_residual(cuda):
     assert info["rel_residual"] <= 1e-10
     u = np.where(mask, x.cpu().numpy(), b)
     assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < TOL_SOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", ["plain", "flexible", "mixed"])
+def test_graph_replay_matches_eager(cuda, variant):
+    """Chunks of GMRES iterations replayed as HIP graphs reproduce the eager
+    run exactly (same kernels, same order): iterate, residual history and
+    iteration count, over many restarts (so every chunk graph is replayed
+    several times, with eager residual / update work and host syncs between
+    the replays)."""
+    import torch
+    from rtx.operator import TransportOperator
+    from rtx.krylov import BlockJacobi, InteriorSystem, gmres, refine
+    from rtx._lib import PREC_F64, PREC_F32, PREC_MIXED
+    mesh, wl = _c1_small()
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    pc = BlockJacobi(op, keep_f32=True)
+    rhs = op.mask(_dev(np.random.default_rng(4).uniform(-1, 1, op.ndof)))
+    runs = []
+    for graphs in (False, True, True):
+        if variant == "mixed":
+            hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
+            lo = InteriorSystem(op, PREC_MIXED, pc, PREC_F32)
+            import rtx.krylov as K
+            orig = K.gmres
+            K.gmres = lambda *a, **k: orig(*a, **{**k, "graphs": graphs, "chunk": 3})
+            try:
+                x, info = refine(hi, lo, rhs, None, rtol=1e-10, inner_rtol=1e-2, restart=7,
+                                 max_inner=500)
+            finally:
+                K.gmres = orig
+            runs.append((x.cpu().numpy(), info["residual_history"], info["iteration_number"]))
+        else:
+            system = InteriorSystem(op, PREC_F64, pc, PREC_F64)
+            x, info = gmres(system, rhs, None, rtol=1e-10, restart=7, max_it=500,
+                            flexible=variant == "flexible", chunk=3, graphs=graphs)
+            assert info["converged_reason"] > 0
+            if graphs:
+                assert info["graphs"] >= 2
+            runs.append((x.cpu().numpy(), info["convergence_history"], info["iteration_number"]))
+        torch.cuda.synchronize()
+    for other in runs[1:]:
+        assert other[2] == runs[0][2]
+        assert np.array_equal(other[0], runs[0][0])
+        assert list(other[1]) == list(runs[0][1])
