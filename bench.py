@@ -43,8 +43,10 @@ def parse():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--learned-fields", type=int, default=20)
-    ap.add_argument("--learned-kappa-scale", type=float, default=1.0)
+    ap.add_argument("--learned-fields", type=int, default=17)
+    ap.add_argument("--learned-kappa-scales", default="1,30",
+                    help="comma-separated kappa scales of the random-field workload "
+                         "(1: as specified, median kappa 1; 30: optically thick)")
     ap.add_argument("--calib", action="store_true",
                     help="run one plain 1 GiB device copy after the timed region "
                          "(counter calibration for rocprofv3 --pmc)")
@@ -193,9 +195,9 @@ def main():
     if not args.no_learned:
         del op
         torch.cuda.empty_cache()
-        out["learned"] = learned_section(fields=args.learned_fields,
-                                         kappa_scale=args.learned_kappa_scale,
-                                         log=lambda m: print(m, file=sys.stderr, flush=True))
+        out["learned"] = [learned_section(fields=args.learned_fields, kappa_scale=float(ks),
+                                          log=lambda m: print(m, file=sys.stderr, flush=True))
+                          for ks in args.learned_kappa_scales.split(",") if ks.strip()]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, wl)
     print(json.dumps(out), flush=True)
@@ -212,19 +214,43 @@ def _rhs(op, wl):
     return op.mask(f - op.apply(bb, ops=OP_SYSTEM), keep_interior=True)
 
 
-def _timed_solve(op, rhs, precision, pc=None, bj=None, inner_rtol=1e-3):
+def _timed_solve(op, rhs, precision, make_pc=None, inner_rtol=1e-3):
+    """Set up a solver session (block-Jacobi factorisation, and the coarse
+    preconditioner when ``make_pc(bj)`` is given), then solve twice: the
+    first ("cold") solve also runs each chunk of iterations eagerly once and
+    captures its graph, the second ("warm") replays the graphs, as a repeated
+    solve (an adaptive loop) would.  ``seconds`` = setup + cold solve."""
     import torch
     from rtx import learned
+    from rtx.krylov import BlockJacobi
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    x, info = learned.solve(op, rhs, None, pc=pc, bj=bj, precision=precision, rtol=1e-8,
-                            restart=50, inner_rtol=inner_rtol)
+    bj = BlockJacobi(op, keep_f64=precision == "fp64", keep_bf16=precision != "fp64")
+    pc = make_pc(bj) if make_pc is not None else None
+    sess = learned.Session(op, pc=pc, bj=bj, precision=precision, restart=50)
     torch.cuda.synchronize()
-    return x, {"seconds": time.perf_counter() - t0,
+    t_setup = time.perf_counter() - t0
+    out = {}
+    for tag in ("cold", "warm"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, info = sess.solve(rhs, rtol=1e-8, inner_rtol=inner_rtol)
+        torch.cuda.synchronize()
+        out[tag] = (time.perf_counter() - t0, x, info)
+    t_cold, _, info_c = out["cold"]
+    t_warm, x, info = out["warm"]
+    return x, {"seconds": t_setup + t_cold, "setup_seconds": t_setup,
+               "solve_seconds_cold": t_cold, "solve_seconds_warm": t_warm,
                "iterations": int(info["iteration_number"]),
                "rel_residual": float(info["rel_residual"]),
                "converged": bool(info["converged_reason"] > 0),
                "fallback": bool(info.get("fallback", False))}
+
+
+def _speedups(res, a, b, key):
+    res["speedup_{}_vs_{}".format(a, b)] = res[b]["seconds"] / res[a]["seconds"]
+    res["speedup_{}_vs_{}_warm".format(a, b)] = \
+        res[b]["solve_seconds_warm"] / res[a]["solve_seconds_warm"]
 
 
 def solve_section(op, mesh, wl):
@@ -237,11 +263,11 @@ def solve_section(op, mesh, wl):
     x64, res["fp64"] = _timed_solve(op, rhs, "fp64")
     xm, res["mixed"] = _timed_solve(op, rhs, "mixed")
     res["mixed"]["rel_l2_vs_fp64"] = float((xm - x64).norm() / x64.norm())
-    res["speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
+    _speedups(res, "mixed", "fp64", "seconds")
     return res
 
 
-def learned_section(fields=20, eval_seed=10_000, kappa_scale=1.0, log=None):
+def learned_section(fields=17, eval_seed=10_000, kappa_scale=1.0, log=None):
     """The learned coarse preconditioner on the random-field workload: 64 x 64
     columns x 16 angular cells, p = 4 (8,192,000 DOF), log kappa ~ GRF
     (length 0.15, mean log kappa_scale, variance 1), sigma / kappa ~
@@ -264,15 +290,16 @@ def learned_section(fields=20, eval_seed=10_000, kappa_scale=1.0, log=None):
     grid = learned.CoarseGrid(op, 64)
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
-    X, Y = learned.generate_pairs(op, grid, range(fields), cycles=12, restart=10, skip=1,
-                                  field=field, direction_stride=6, log=log)
+    # 19 restart pairs + 100 direction pairs per field: about 2,000 pairs
+    X, Y = learned.generate_pairs(op, grid, range(fields), cycles=20, restart=10, skip=1,
+                                  field=field, direction_stride=2, log=log)
     torch.cuda.synchronize()
     t_pairs = time.perf_counter() - t0
     nc = grid.nc
     mlp = learned.MLP([5 * nc, 128, 128, 128, nc], op.device, seed=0)
     mlp.n_moments = 3 * nc
     t0 = time.perf_counter()
-    xs, ys, hist = learned.train(mlp, X, Y, epochs=60, batch=64, lr=1e-3, log=log)
+    xs, ys, hist = learned.train(mlp, X, Y, epochs=100, batch=64, lr=1e-3, log=log)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     npairs = int(X.shape[0])
@@ -287,19 +314,17 @@ def learned_section(fields=20, eval_seed=10_000, kappa_scale=1.0, log=None):
            "validation_rel_error": hist[-1][1]}
     x64, res["fp64"] = _timed_solve(op, rhs, "fp64")
     xm, res["mixed"] = _timed_solve(op, rhs, "mixed")
-    model = learned.CoarseModel(mlp, 3 * nc, xs, ys)
-    model.set_coefficients(grid.coefficient_channels(wl.kappa, wl.sigma))
-    from rtx.krylov import BlockJacobi
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    bj = BlockJacobi(op, keep_f64=False, keep_bf16=True)
-    pc = learned.TwoLevelPC(op, bj, grid, model, apply_precision=PREC_MIXED)
-    xl, res["learned_mixed"] = _timed_solve(op, rhs, "mixed", pc=pc, bj=bj)
-    res["learned_mixed"]["seconds"] += time.perf_counter() - t0 - res["learned_mixed"]["seconds"]
+
+    def make_pc(bj):
+        model = learned.CoarseModel(mlp, 3 * nc, xs, ys)
+        model.set_coefficients(grid.coefficient_channels(wl.kappa, wl.sigma))
+        return learned.TwoLevelPC(op, bj, grid, model, apply_precision=PREC_MIXED)
+
+    xl, res["learned_mixed"] = _timed_solve(op, rhs, "mixed", make_pc=make_pc)
     for k, x in (("mixed", xm), ("learned_mixed", xl)):
         res[k]["rel_l2_vs_fp64"] = float((x - x64).norm() / x64.norm())
-    res["speedup_learned_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["learned_mixed"]["seconds"]
-    res["speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
+    _speedups(res, "learned_mixed", "fp64", "seconds")
+    _speedups(res, "mixed", "fp64", "seconds")
     return res
 
 

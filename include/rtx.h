@@ -181,6 +181,11 @@ int rtx_bjacobi_size(rtx_plan *plan, int64_t *n_entries);
    not fit LDS.  Synchronises the stream; RTX_ERR_NUMERIC on a singular block. */
 int rtx_bjacobi_factor(rtx_plan *plan, int restrict_interior, double *d_inv, float *d_inv_f32,
                        double *d_scratch, void *stream);
+/* Build method: -1 automatic (register-resident sweeps for blocks of up to
+   128 DOFs, else the LDS / scratch Gauss-Jordan), 0 LDS / scratch
+   Gauss-Jordan with explicit row interchanges, 1 register-resident sweeps
+   (fails for larger blocks). */
+int rtx_bjacobi_set_method(rtx_plan *plan, int method);
 /* z = blockdiag(inv) r, precision RTX_PREC_F64 (double inverses, vectors),
    RTX_PREC_F32 (float inverses, vectors) or RTX_PREC_MIXED (BF16 inverses -
    rtx_mlp_to_bf16 of the FP32 copy - and float vectors; the factorisation

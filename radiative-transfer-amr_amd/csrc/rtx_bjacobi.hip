@@ -234,6 +234,238 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_build(DevMesh m, int restrict
         for (int row = tid; row < n; row += BJ_THREADS) o[size_t(col) * n + row] = A[size_t(row) * n + col];
 }
 
+// ---- register-resident build for blocks of up to 16 RR DOFs.
+//
+// The block is distributed cyclically over a 16 x TJ thread grid (thread
+// (ti, tj) holds rows ti + 16 u, u < RR, and columns tj + TJ v, v < CC; 16 RR
+// = TJ CC), each entry evaluated directly from the operator's structure
+// (same terms, same summation order as bj_assemble).  Inversion is
+// Gauss-Jordan with partial pivoting done as a sequence of sweeps on the
+// pivot entries (p_k, k) without moving rows: only the pivot column and the
+// pivot row pass through LDS, so a workgroup needs about 3 KB of LDS
+// instead of n^2 doubles and several cells run per CU.  After all sweeps
+// the register matrix M holds the inverse with permuted indices:
+// inv[q_i][p_j] = M[i][j], q = p^-1.
+struct BJCell {
+    int nx, ny, nth, n, nyt, ao, na, quad, bd;
+    double dx, dy, dth;
+    const double *wx, *wy, *wt, *Dx, *Dy, *G, *kap, *sig;
+};
+
+struct BJSelf {                  // links of a cell to itself (periodic faces), in LDS
+    int n;
+    int F[4];
+    int es[4], et[4];
+};
+
+__device__ inline double bj_entry(const DevMesh &m, const BJCell &x, const BJSelf &sl,
+                                  const double *cth, const double *sth, int t, int u) {
+    const int p = t / x.nyt, q = (t / x.nth) % x.ny, r = t % x.nth;
+    const int i = u / x.nyt, j = (u / x.nth) % x.ny, a = u % x.nth;
+    if (x.bd) {
+        const bool rb = bj_on_bdry(x.bd, p, q, x.nx, x.ny);
+        const bool cb = bj_on_bdry(x.bd, i, j, x.nx, x.ny);
+        if (rb || cb) return (t == u) ? 1.0 : 0.0;
+    }
+    double v = 0.0;
+    if (p == i && q == j) {
+        v = -x.dx * x.dy * x.wx[p] * x.wy[q] * x.sig[p * x.ny + q] *
+            x.G[size_t(x.ao + a) * x.na + x.ao + r];
+        if (r == a) {
+            double d = (x.dx * x.dy * x.dth * 0.125) * x.wx[p] * x.wy[q] * x.wt[r] * x.kap[p * x.ny + q];
+            if (x.quad >= 0)
+                for (int h = 0; h < 2; ++h) {
+                    const int F = (x.quad + h) & 3;
+                    const bool on = (F == 0 && p == x.nx - 1) || (F == 2 && p == 0) ||
+                                    (F == 1 && q == x.ny - 1) || (F == 3 && q == 0);
+                    if (on) {
+                        const double tf = F == 0 ? cth[r] : (F == 1 ? sth[r] : (F == 2 ? -cth[r] : -sth[r]));
+                        d += ((F & 1) ? x.dx : x.dy) * x.dth * 0.25 * ((F & 1) ? x.wx[p] : x.wy[q]) *
+                             x.wt[r] * tf;
+                    }
+                }
+            v += d;
+        }
+    }
+    if (q == j && r == a)
+        v -= (x.dy * x.dth * 0.25) * x.wy[q] * x.wt[r] * cth[r] * x.wx[i] * x.Dx[p * NMq + i];
+    if (p == i && r == a)
+        v -= (x.dx * x.dth * 0.25) * x.wx[p] * x.wt[r] * sth[r] * x.wy[j] * x.Dy[q * NMq + j];
+    for (int l = 0; l < sl.n; ++l) {
+        const int F = sl.F[l];
+        int m0, m1;
+        bool hit;
+        if (F == 0) { hit = p == x.nx - 1 && i == 0; m0 = q; m1 = j; }
+        else if (F == 2) { hit = p == 0 && i == x.nx - 1; m0 = q; m1 = j; }
+        else if (F == 1) { hit = q == x.ny - 1 && j == 0; m0 = p; m1 = i; }
+        else { hit = q == 0 && j == x.ny - 1; m0 = p; m1 = i; }
+        if (hit) {
+            const int n0 = (F & 1) ? x.nx : x.ny;
+            v += ((F & 1) ? x.dx : x.dy) * x.dth * 0.25 * m.Es_pool[sl.es[l] + m1 * n0 + m0] *
+                 m.Eth_pool[sl.et[l] + a * x.nth + r];
+        }
+    }
+    return v;
+}
+
+template <int RR, int CC, int TJ>
+__global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restrict_,
+                                                         const int64_t *inv_off,
+                                                         double *__restrict__ out,
+                                                         int *__restrict__ status) {
+    constexpr int NR = 16 * RR;
+    static_assert(NR == TJ * CC, "square register tiles");
+    __shared__ double cth[NMq], sth[NMq];
+    __shared__ double colk[2][NR];
+    __shared__ double prow[NR];
+    __shared__ double red_v[16];
+    __shared__ int red_i[16];
+    __shared__ int perm[NR], qinv[NR];
+    __shared__ int used[NR];
+    __shared__ BJSelf sl;
+    const int k = blockIdx.x, tid = threadIdx.x, ti = tid / TJ, tj = tid % TJ;
+    BJCell x;
+    const int c = m.cell_col[k];
+    x.nx = m.col_nx[c];
+    x.ny = m.col_ny[c];
+    x.nth = m.cell_nth[k];
+    x.n = x.nx * x.ny * x.nth;
+    x.nyt = x.ny * x.nth;
+    x.dx = m.col_geom[4 * c + 2] - m.col_geom[4 * c + 0];
+    x.dy = m.col_geom[4 * c + 3] - m.col_geom[4 * c + 1];
+    const double th0 = m.cell_geom[2 * k];
+    x.dth = m.cell_geom[2 * k + 1] - th0;
+    x.wx = m.lgl_w + x.nx * NMq;
+    x.wy = m.lgl_w + x.ny * NMq;
+    x.wt = m.lg_w + x.nth * NMq;
+    x.Dx = m.lgl_D + size_t(x.nx) * NMq * NMq;
+    x.Dy = m.lgl_D + size_t(x.ny) * NMq * NMq;
+    x.ao = m.cell_ang_off[k];
+    x.na = m.col_nang[c];
+    x.G = m.G + m.col_G_off[c];
+    x.kap = m.kappa + m.col_sp_off[c];
+    x.sig = m.sigma + m.col_sp_off[c];
+    x.quad = m.cell_quad[k];
+    x.bd = restrict_ ? m.cell_bdry[k] : 0;
+    if (tid == 0) {
+        int ns = 0;
+        for (int l = m.link_start[k]; l < m.link_start[k + 1] && ns < 4; ++l)
+            if (m.link_src[l] == k) {
+                sl.F[ns] = m.link_face[l];
+                sl.es[ns] = m.link_Es_off[l];
+                sl.et[ns] = m.link_Eth_off[l];
+                ++ns;
+            }
+        sl.n = ns;
+    }
+    if (tid < x.nth) {
+        double s, cc;
+        sincos(th0 + x.dth * 0.5 * (m.lg_x[x.nth * NMq + tid] + 1.0), &s, &cc);
+        cth[tid] = cc;
+        sth[tid] = s;
+    }
+    if (tid < NR) used[tid] = 0;
+    const int n = x.n;
+    __syncthreads();
+    double A[RR][CC];
+#pragma unroll
+    for (int u = 0; u < RR; ++u)
+#pragma unroll
+        for (int v = 0; v < CC; ++v) {
+            const int row = ti + 16 * u, col = tj + TJ * v;
+            A[u][v] = (row < n && col < n) ? bj_entry(m, x, sl, cth, sth, row, col) : 0.0;
+        }
+    for (int kk = 0; kk < n; ++kk) {
+        const int buf = kk & 1, vk = kk / TJ;
+        if (tj == kk % TJ) {           // the 16 owners of column kk
+            double best = -1.0;
+            int bi = n;
+#pragma unroll
+            for (int u = 0; u < RR; ++u) {
+                double val = 0.0;
+#pragma unroll
+                for (int v = 0; v < CC; ++v)
+                    if (v == vk) val = A[u][v];
+                const int row = ti + 16 * u;
+                if (row < n) {
+                    colk[buf][row] = val;
+                    const double av = fabs(val);
+                    if (!used[row] && (av > best || (av == best && row < bi))) {
+                        best = av;
+                        bi = row;
+                    }
+                }
+            }
+            red_v[ti] = best;
+            red_i[ti] = bi;
+        }
+        __syncthreads();
+        double best = -1.0;
+        int pr = n;
+        for (int s = 0; s < 16; ++s) {
+            const double bv = red_v[s];
+            const int bi = red_i[s];
+            if (bv > best || (bv == best && bi < pr)) {
+                best = bv;
+                pr = bi;
+            }
+        }
+        if (!(best > 0.0) || pr >= n) {
+            if (tid == 0) atomicMax(status, k + 1);
+            return;                    // uniform across the workgroup
+        }
+        if (ti == (pr & 15)) {
+            const int up = pr >> 4;
+#pragma unroll
+            for (int u = 0; u < RR; ++u)
+                if (u == up)
+#pragma unroll
+                    for (int v = 0; v < CC; ++v) {
+                        const int col = tj + TJ * v;
+                        if (col < n) prow[col] = A[u][v];
+                    }
+        }
+        if (tid == 0) {
+            used[pr] = 1;
+            perm[kk] = pr;
+        }
+        __syncthreads();
+        const double invp = 1.0 / colk[buf][pr];
+        double pv[CC];
+#pragma unroll
+        for (int v = 0; v < CC; ++v) {
+            const int col = tj + TJ * v;
+            pv[v] = (col < n) ? prow[col] * invp : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < RR; ++u) {
+            const int row = ti + 16 * u;
+            const double ck = (row < n) ? colk[buf][row] : 0.0;
+#pragma unroll
+            for (int v = 0; v < CC; ++v) {
+                const int col = tj + TJ * v;
+                if (row == pr)
+                    A[u][v] = (col == kk) ? invp : A[u][v] * invp;
+                else if (col == kk)
+                    A[u][v] = -ck * invp;
+                else
+                    A[u][v] = A[u][v] - ck * pv[v];
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < n) qinv[perm[tid]] = tid;
+    __syncthreads();
+    double *o = out + inv_off[k];
+#pragma unroll
+    for (int u = 0; u < RR; ++u)
+#pragma unroll
+        for (int v = 0; v < CC; ++v) {
+            const int row = ti + 16 * u, col = tj + TJ * v;
+            if (row < n && col < n) o[size_t(perm[col]) * n + qinv[row]] = A[u][v];
+        }
+}
+
 __global__ void k_to_f32_bj(const double *__restrict__ s, float *__restrict__ d, int64_t n) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     for (; i < n; i += int64_t(gridDim.x) * blockDim.x) d[i] = float(s[i]);
@@ -331,7 +563,23 @@ extern "C" int rtx_bjacobi_factor(rtx_plan *p, int restrict_interior, double *d_
     int *d_status = nullptr;
     RTX_HIP(hipMallocAsync((void **)&d_status, sizeof(int), st));
     RTX_HIP(hipMemsetAsync(d_status, 0, sizeof(int), st));
-    if (lds + static_lds <= size_t(max_lds)) {
+    const bool reg = p->bj_method != 0 && nmax <= 128;
+    if (reg) {
+        if (nmax <= 32)
+            k_bj_build_reg<2, 2, 16><<<p->m.ncell, 256, 0, st>>>(p->m, restrict_interior,
+                                                                p->bj_off, d_inv, d_status);
+        else if (nmax <= 64)
+            k_bj_build_reg<4, 4, 16><<<p->m.ncell, 256, 0, st>>>(p->m, restrict_interior,
+                                                                p->bj_off, d_inv, d_status);
+        else
+            k_bj_build_reg<8, 4, 32><<<p->m.ncell, 512, 0, st>>>(p->m, restrict_interior,
+                                                                p->bj_off, d_inv, d_status);
+    } else if (p->bj_method == 1) {
+        (void)hipFreeAsync(d_status, st);
+        set_error("rtx_bjacobi_factor: the register build handles blocks of up to 128 DOFs, "
+                  "not %lld", (long long)nmax);
+        return RTX_ERR_ARG;
+    } else if (lds + static_lds <= size_t(max_lds)) {
         RTX_HIP(hipFuncSetAttribute((const void *)k_bj_build<true>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
         k_bj_build<true><<<p->m.ncell, BJ_THREADS, lds, st>>>(p->m, restrict_interior, p->bj_off,
@@ -359,6 +607,12 @@ extern "C" int rtx_bjacobi_factor(rtx_plan *p, int restrict_interior, double *d_
         return RTX_ERR_NUMERIC;
     }
     p->bj_ready = true;
+    return RTX_OK;
+}
+
+extern "C" int rtx_bjacobi_set_method(rtx_plan *p, int method) {
+    RTX_CHECK(p && method >= -1 && method <= 1, "rtx_bjacobi_set_method: method must be -1, 0 or 1");
+    p->bj_method = method;
     return RTX_OK;
 }
 

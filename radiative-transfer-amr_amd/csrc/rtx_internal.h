@@ -90,6 +90,7 @@ struct rtx_plan {
     int64_t *bj_off = nullptr;   // ncell + 1 offsets of the column-major inverses
     int64_t bj_entries = 0;
     bool bj_ready = false;
+    int bj_method = -1;          // -1 automatic, 0 LDS Gauss-Jordan, 1 register sweeps
     // coarse grid of the learned correction: per column a list of
     // (coarse cell, area(col & cell) / area(cell), area(col & cell) / area(col))
     void *coarse = nullptr;      // one allocation for the map

@@ -93,6 +93,11 @@ class InteriorSystem:
             self.x32 = torch.empty(n, dtype=torch.float32, device=dev)
             self.y32 = torch.empty(n, dtype=torch.float32, device=dev)
 
+    def graph_key(self):
+        """Identifies the preconditioner configuration captured in graphs."""
+        return (id(self.pc), getattr(self.pc, "enabled", True), self.precision,
+                self.pc_precision)
+
     def matvec(self, x, y):
         if self.precision == PREC_F64:
             self.op.apply(x, y, ops=OP_SYSTEM | OP_RESTRICT, precision=PREC_F64)
@@ -160,139 +165,166 @@ class _ChunkGraphs:
         g.replay()
 
 
-def gmres(system, b, x=None, rtol=1e-8, atol=0.0, restart=50, max_it=5000,
-          flexible=False, callback=None, true_residual_every_restart=True, on_restart=None,
-          on_direction=None, chunk=10, graphs=True, refine=0):
-    """Right-preconditioned restarted GMRES on ``system`` (InteriorSystem).
+class GMRES:
+    """Right-preconditioned restarted GMRES on ``system`` (InteriorSystem)
+    with persistent device buffers, so repeated solves with the same system
+    (the inner solves of iterative refinement) reuse the captured graphs.
 
-    Converged when |b - A x| <= max(rtol |b|, atol) for the FP64 true
-    residual.  The Arnoldi basis, the orthogonalisation, the Givens QR of the
+    The Arnoldi basis, the orthogonalisation, the Givens QR of the
     Hessenberg matrix and the small triangular solve all stay on the device;
     the host reads the residual estimates once per ``chunk`` iterations, and
     (``graphs``) each chunk of iterations - preconditioner, operator, Arnoldi
-    step, Givens update - is replayed as one HIP graph.  ``on_restart(x, r)``
-    sees the iterate and its true residual at the start of every cycle;
-    ``on_direction(z, w)`` sees every preconditioned direction z and
-    w = A z before orthogonalisation (an exact residual / correction pair);
-    either hook (or ``callback``) runs the iterations eagerly.  ``refine``
-    selects the Gram-Schmidt refinement of ``rtx_arnoldi_step`` (0: none,
-    PETSc's default; 1: when needed; 2: always).  Returns (x, info) with
-    PETSc-like convergence information."""
-    lib = system.lib
-    n = b.numel()
-    dev = b.device
-    m = int(restart)
-    x = torch.zeros_like(b) if x is None else x
-    V = torch.empty((m + 1, n), dtype=torch.float64, device=dev)
-    Z = torch.empty((m, n), dtype=torch.float64, device=dev) if flexible else None
-    w = torch.empty(n, dtype=torch.float64, device=dev)
-    z = torch.empty(n, dtype=torch.float64, device=dev)
-    r = torch.empty(n, dtype=torch.float64, device=dev)
-    ws = C.c_int64()
-    check(lib.rtx_arnoldi_workspace(m + 1, C.byref(ws)), "workspace")
-    work = torch.empty(int(ws.value), dtype=torch.float64, device=dev)
-    hcol = torch.zeros(m + 2, dtype=torch.float64, device=dev)
-    check(lib.rtx_gmres_state_size(m, C.byref(ws)), "gmres state size")
-    state = torch.zeros(int(ws.value), dtype=torch.float64, device=dev)
-    est_off = 3 * m + 1 + m * m + (m + 1) * m
-    H_off = 3 * m + 1 + m * m
-    est_h = torch.zeros(m, dtype=torch.float64).pin_memory()
-    ydev = torch.empty(m, dtype=torch.float64, device=dev)
-    scal = torch.empty(2, dtype=torch.float64, device=dev)
-    use_graphs = graphs and callback is None and on_direction is None
-    chunks = _ChunkGraphs()
+    step, Givens update - is replayed as one HIP graph.  Graphs are keyed by
+    the chunk and by ``system.graph_key()``, so a change of preconditioner
+    configuration never replays a stale graph.  ``refine`` selects the
+    Gram-Schmidt refinement of ``rtx_arnoldi_step`` (0: none, PETSc's
+    default; 1: when needed; 2: always)."""
 
-    def steps(j0, j1):
-        st = stream_handle()
+    def __init__(self, system, n, device, restart=50, flexible=False, chunk=10, graphs=True,
+                 refine=0):
+        self.system, self.lib = system, system.lib
+        self.n, self.m = int(n), int(restart)
+        self.flexible, self.chunk, self.graphs, self.refine = flexible, int(chunk), graphs, refine
+        m, dev = self.m, device
+        self.V = torch.empty((m + 1, n), dtype=torch.float64, device=dev)
+        self.Z = torch.empty((m, n), dtype=torch.float64, device=dev) if flexible else None
+        self.w = torch.empty(n, dtype=torch.float64, device=dev)
+        self.z = torch.empty(n, dtype=torch.float64, device=dev)
+        self.r = torch.empty(n, dtype=torch.float64, device=dev)
+        ws = C.c_int64()
+        check(self.lib.rtx_arnoldi_workspace(m + 1, C.byref(ws)), "workspace")
+        self.work = torch.empty(int(ws.value), dtype=torch.float64, device=dev)
+        self.hcol = torch.zeros(m + 2, dtype=torch.float64, device=dev)
+        check(self.lib.rtx_gmres_state_size(m, C.byref(ws)), "gmres state size")
+        self.state = torch.zeros(int(ws.value), dtype=torch.float64, device=dev)
+        self.est_off = 3 * m + 1 + m * m + (m + 1) * m
+        self.H_off = 3 * m + 1 + m * m
+        self.est_h = torch.zeros(m, dtype=torch.float64).pin_memory()
+        self.ydev = torch.empty(m, dtype=torch.float64, device=dev)
+        self.scal = torch.empty(2, dtype=torch.float64, device=dev)
+        self.chunks = _ChunkGraphs()
+        self._on_direction = None
+
+    def _steps(self, j0, j1):
+        lib, st = self.lib, stream_handle()
+        V, Z, w, z, n, m = self.V, self.Z, self.w, self.z, self.n, self.m
         for j in range(j0, j1):
-            zj = Z[j] if flexible else z
-            system.precondition(V[j], zj)
-            system.matvec(zj, w)
-            if on_direction is not None:
-                on_direction(zj, w)
-            check(lib.rtx_arnoldi_step(_vp(V), n, j + 1, n, _vp(w), _vp(hcol), _vp(work),
-                                       refine, st), "rtx_arnoldi_step")
-            check(lib.rtx_gmres_givens(j, m, _vp(hcol), _vp(state), st), "rtx_gmres_givens")
+            zj = Z[j] if self.flexible else z
+            self.system.precondition(V[j], zj)
+            self.system.matvec(zj, w)
+            if self._on_direction is not None:
+                self._on_direction(zj, w)
+            check(lib.rtx_arnoldi_step(_vp(V), n, j + 1, n, _vp(w), _vp(self.hcol),
+                                       _vp(self.work), self.refine, st), "rtx_arnoldi_step")
+            check(lib.rtx_gmres_givens(j, m, _vp(self.hcol), _vp(self.state), st),
+                  "rtx_gmres_givens")
 
-    bnorm = _norm(lib, b, work, scal)
-    target = max(rtol * bnorm, atol)
-    history, true_hist = [], []
-    its = 0
-    reason = 0
-    H_last = None
-    t0 = time.perf_counter()
-    while True:
-        system.residual(b, x, r)
-        beta = _norm(lib, r, work, scal)
-        true_hist.append(beta)
-        if on_restart is not None:
-            on_restart(x, r)
-        if its == 0:
-            history.append(beta)
-        if beta <= target:
-            reason = 2 if beta <= rtol * bnorm else 3     # KSP_CONVERGED_RTOL / ATOL
-            break
-        if its >= max_it:
-            reason = -3                                   # KSP_DIVERGED_ITS
-            break
-        if not np.isfinite(beta):
-            reason = -9                                   # KSP_DIVERGED_NANORINF
-            break
-        st = stream_handle()
-        check(lib.rtx_axpby(PREC_F64, n, 1.0 / beta, _vp(r), 0.0, _vp(V[0]), st), "axpby")
-        check(lib.rtx_gmres_begin(m, beta, _vp(state), st), "rtx_gmres_begin")
-        j, k = 0, 0
-        while j < m and its < max_it:
-            j1 = min(j + int(chunk), m, j + (max_it - its))
-            chunks.run((j, j1), lambda a=j, bb=j1: steps(a, bb), use_graphs)
-            est_h[j:j1].copy_(state[est_off + j:est_off + j1], non_blocking=True)
-            torch.cuda.current_stream().synchronize()
-            e = est_h[j:j1].numpy()
-            stop = None
-            for i in range(j1 - j):
-                if not np.isfinite(e[i]):
-                    stop = i               # breakdown: keep the directions before it
-                    break
-                history.append(float(e[i]))
-                if callback is not None:
-                    callback(its + i + 1, float(e[i]))
-                if e[i] <= target:
-                    stop = i + 1
-                    break
-            if stop is not None:
-                its += stop
-                k = j + stop
+    def solve(self, b, x=None, rtol=1e-8, atol=0.0, max_it=5000, callback=None,
+              on_restart=None, on_direction=None):
+        """Converged when |b - A x| <= max(rtol |b|, atol) for the FP64 true
+        residual.  ``on_restart(x, r)`` sees the iterate and its true residual
+        at the start of every cycle; ``on_direction(z, w)`` sees every
+        preconditioned direction z and w = A z before orthogonalisation (an
+        exact residual / correction pair); either hook (or ``callback``)
+        runs the iterations eagerly.  Returns (x, info) with PETSc-like
+        convergence information."""
+        system, lib = self.system, self.lib
+        n, m = self.n, self.m
+        V, Z, w, z, r = self.V, self.Z, self.w, self.z, self.r
+        x = torch.zeros_like(b) if x is None else x
+        use_graphs = self.graphs and callback is None and on_direction is None
+        self._on_direction = on_direction
+        bnorm = _norm(lib, b, self.work, self.scal)
+        target = max(rtol * bnorm, atol)
+        history, true_hist = [], []
+        its, reason, k_last, stop = 0, 0, 0, None
+        t0 = time.perf_counter()
+        while True:
+            system.residual(b, x, r)
+            beta = _norm(lib, r, self.work, self.scal)
+            true_hist.append(beta)
+            if on_restart is not None:
+                on_restart(x, r)
+            if its == 0:
+                history.append(beta)
+            if beta <= target:
+                reason = 2 if beta <= rtol * bnorm else 3     # KSP_CONVERGED_RTOL / ATOL
                 break
-            its += j1 - j
-            j = k = j1
-        H_last = (state, H_off, k)
-        if k > 0:
-            check(lib.rtx_gmres_solve(k, m, _vp(state), _vp(ydev), st), "rtx_gmres_solve")
-            if flexible:
-                check(lib.rtx_combine(_vp(Z), n, k, _vp(ydev), n, _vp(x), 1, st), "combine")
-            else:
-                check(lib.rtx_combine(_vp(V), n, k, _vp(ydev), n, _vp(w), 0, st), "combine")
-                system.precondition(w, z)
-                check(lib.rtx_axpby(PREC_F64, n, 1.0, _vp(z), 1.0, _vp(x), st), "axpby")
-        elif stop is not None:
-            reason = -9
-            break
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    if H_last is not None and H_last[2] > 0:
-        st_, off, k = H_last
-        Hd = st_[off:off + (m + 1) * m].view(m, m + 1).cpu().numpy().T
-        sv = hessenberg_singular_values(Hd, k)
-    else:
-        sv = [float("nan")] * 2
-    info = GMRESResult(converged_reason=reason, convergence_history=history,
-                       iteration_number=its, res_f=true_hist[-1], res_best=min(true_hist),
-                       true_residual_history=true_hist, rtol=rtol, atol=atol,
-                       divtol=float(np.sqrt(np.finfo(np.float64).max)), max_it=max_it,
-                       bnorm=bnorm, rel_residual=true_hist[-1] / bnorm if bnorm > 0 else 0.0,
-                       seconds=elapsed, restart=m, extreme_singular_values=sv,
-                       graphs=len(chunks.graphs))
-    return x, info
+            if its >= max_it:
+                reason = -3                                   # KSP_DIVERGED_ITS
+                break
+            if not np.isfinite(beta):
+                reason = -9                                   # KSP_DIVERGED_NANORINF
+                break
+            st = stream_handle()
+            check(lib.rtx_axpby(PREC_F64, n, 1.0 / beta, _vp(r), 0.0, _vp(V[0]), st), "axpby")
+            check(lib.rtx_gmres_begin(m, beta, _vp(self.state), st), "rtx_gmres_begin")
+            gkey = system.graph_key() if hasattr(system, "graph_key") else None
+            j, k, stop = 0, 0, None
+            while j < m and its < max_it:
+                j1 = min(j + self.chunk, m, j + (max_it - its))
+                self.chunks.run((j, j1, gkey), lambda a=j, bb=j1: self._steps(a, bb), use_graphs)
+                self.est_h[j:j1].copy_(self.state[self.est_off + j:self.est_off + j1],
+                                       non_blocking=True)
+                torch.cuda.current_stream().synchronize()
+                e = self.est_h[j:j1].numpy()
+                for i in range(j1 - j):
+                    if not np.isfinite(e[i]):
+                        stop = i               # breakdown: keep the directions before it
+                        break
+                    history.append(float(e[i]))
+                    if callback is not None:
+                        callback(its + i + 1, float(e[i]))
+                    if e[i] <= target:
+                        stop = i + 1
+                        break
+                if stop is not None:
+                    its += stop
+                    k = j + stop
+                    break
+                its += j1 - j
+                j = k = j1
+            k_last = k
+            if k > 0:
+                check(lib.rtx_gmres_solve(k, m, _vp(self.state), _vp(self.ydev), st),
+                      "rtx_gmres_solve")
+                if self.flexible:
+                    check(lib.rtx_combine(_vp(Z), n, k, _vp(self.ydev), n, _vp(x), 1, st),
+                          "combine")
+                else:
+                    check(lib.rtx_combine(_vp(V), n, k, _vp(self.ydev), n, _vp(w), 0, st),
+                          "combine")
+                    system.precondition(w, z)
+                    check(lib.rtx_axpby(PREC_F64, n, 1.0, _vp(z), 1.0, _vp(x), st), "axpby")
+            elif stop is not None:
+                reason = -9
+                break
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        if k_last > 0:
+            Hd = self.state[self.H_off:self.H_off + (m + 1) * m].view(m, m + 1).cpu().numpy().T
+            sv = hessenberg_singular_values(Hd, k_last)
+        else:
+            sv = [float("nan")] * 2
+        info = GMRESResult(converged_reason=reason, convergence_history=history,
+                           iteration_number=its, res_f=true_hist[-1], res_best=min(true_hist),
+                           true_residual_history=true_hist, rtol=rtol, atol=atol,
+                           divtol=float(np.sqrt(np.finfo(np.float64).max)), max_it=max_it,
+                           bnorm=bnorm,
+                           rel_residual=true_hist[-1] / bnorm if bnorm > 0 else 0.0,
+                           seconds=elapsed, restart=m, extreme_singular_values=sv,
+                           graphs=len(self.chunks.graphs))
+        return x, info
+
+
+def gmres(system, b, x=None, rtol=1e-8, atol=0.0, restart=50, max_it=5000,
+          flexible=False, callback=None, true_residual_every_restart=True, on_restart=None,
+          on_direction=None, chunk=10, graphs=True, refine=0):
+    """One solve with a fresh ``GMRES`` object (see there)."""
+    solver = GMRES(system, b.numel(), b.device, restart=restart, flexible=flexible, chunk=chunk,
+                   graphs=graphs, refine=refine)
+    return solver.solve(b, x, rtol=rtol, atol=atol, max_it=max_it, callback=callback,
+                        on_restart=on_restart, on_direction=on_direction)
 
 
 def hessenberg_singular_values(H, k):
@@ -305,12 +337,16 @@ def hessenberg_singular_values(H, k):
 
 
 def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=50,
-           max_outer=50, max_inner=200, flexible=False, after_outer=None):
+           max_outer=50, max_inner=200, flexible=False, after_outer=None, chunk=10,
+           graphs=True, inner=None):
     """FP64 iterative refinement: r = b - A x in FP64 (``system_hi``), inner
     correction solve A d = r by GMRES whose operator and preconditioner run in
-    reduced precision (``system_lo``; its Arnoldi is still FP64).
+    reduced precision (``system_lo``; its Arnoldi is still FP64).  One GMRES
+    object serves every inner solve, so its graphs are captured once.
     ``flexible`` selects flexible inner GMRES (for a nonlinear
-    preconditioner); ``after_outer(outer, |r|)`` sees every FP64 residual."""
+    preconditioner); ``after_outer(outer, |r|)`` sees every FP64 residual;
+    ``inner`` may pass a GMRES object on ``system_lo`` to reuse (with its
+    graphs) across calls."""
     lib = system_hi.lib
     n = b.numel()
     x = torch.zeros_like(b) if x is None else x
@@ -323,6 +359,10 @@ def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=
     hist, inner_its = [], []
     t0 = time.perf_counter()
     reason = -3
+    if inner is None:
+        inner = GMRES(system_lo, n, b.device, restart=restart, flexible=flexible, chunk=chunk,
+                      graphs=graphs)
+    d = torch.empty_like(b)
     for outer in range(max_outer + 1):
         system_hi.residual(b, x, r)
         rn = _norm(lib, r, work, scal)
@@ -335,8 +375,8 @@ def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=
         if outer == max_outer or not np.isfinite(rn):
             reason = -3 if np.isfinite(rn) else -9
             break
-        d, info = gmres(system_lo, r, None, rtol=inner_rtol, restart=restart, max_it=max_inner,
-                        flexible=flexible)
+        d.zero_()
+        d, info = inner.solve(r, d, rtol=inner_rtol, max_it=max_inner)
         inner_its.append(info["iteration_number"])
         check(lib.rtx_axpby(PREC_F64, n, 1.0, _vp(d), 1.0, _vp(x), stream_handle()), "axpby")
     torch.cuda.synchronize()

@@ -37,7 +37,7 @@ import torch
 from . import _lib, problems
 from ._lib import check, PREC_F64, PREC_F32, PREC_MIXED, OP_SYSTEM, OP_RESTRICT
 from .operator import stream_handle
-from .krylov import BlockJacobi, InteriorSystem, gmres, refine, _norm
+from .krylov import BlockJacobi, InteriorSystem, GMRES, gmres, refine, _norm
 
 
 def _vp(t):
@@ -517,50 +517,100 @@ def validation_error(mlp, X, Y, idx, colscale, y_scale, batch=64):
 
 
 # ------------------------------------------------------------------- solve
-def solve(op, rhs, x0=None, pc=None, bj=None, precision="mixed", rtol=1e-8, restart=50,
-          max_it=5000, inner_rtol=1e-2, max_outer=60, bj_precision=PREC_MIXED):
-    """Interior-system solve to an FP64 relative residual ``rtol`` with the
-    two-level preconditioner ``pc`` (a TwoLevelPC) or plain block-Jacobi
-    (pc None).  precision "mixed": FP64 iterative refinement around flexible
-    GMRES with FP32 / BF16 applies and block-Jacobi inverses stored in
-    ``bj_precision`` (PREC_MIXED: BF16, PREC_F32: FP32); "fp64": flexible
-    GMRES in FP64.  The FP64 true residual judges every outer step: when a
-    step that used the coarse correction fails to reduce it, the correction
-    is switched off."""
-    if bj is None:
-        bj = BlockJacobi(op, keep_f32=precision != "fp64" and bj_precision == PREC_F32,
-                         keep_bf16=precision != "fp64" and bj_precision == PREC_MIXED,
-                         keep_f64=precision == "fp64")
-    the_pc = pc if pc is not None else bj
-    if precision == "fp64":
-        system = InteriorSystem(op, PREC_F64, the_pc, PREC_F64)
-        judge = _Judge(pc)
-        x, info = gmres(system, rhs, x0, rtol=rtol, restart=restart, max_it=max_it,
-                        flexible=pc is not None, on_restart=judge.restart_hook(system, rhs))
-        info["fallback"] = judge.fell_back
+class Session:
+    """A reusable interior-system solver for one operator: the systems, the
+    GMRES objects and their captured graphs persist across ``solve`` calls
+    (for example the repeated solves of an adaptive loop).
+
+    precision "mixed": FP64 iterative refinement around flexible GMRES with
+    FP32 / BF16 applies and block-Jacobi inverses stored in ``bj_precision``
+    (PREC_MIXED: BF16, PREC_F32: FP32); "fp64": GMRES in FP64.  ``pc`` is a
+    TwoLevelPC or None (plain block-Jacobi)."""
+
+    def __init__(self, op, pc=None, bj=None, precision="mixed", restart=50,
+                 bj_precision=PREC_MIXED):
+        if precision not in ("mixed", "fp64"):
+            raise ValueError("precision must be 'mixed' or 'fp64'")
+        if bj is None:
+            bj = BlockJacobi(op, keep_f32=precision != "fp64" and bj_precision == PREC_F32,
+                             keep_bf16=precision != "fp64" and bj_precision == PREC_MIXED,
+                             keep_f64=precision == "fp64")
+        self.op, self.pc, self.bj, self.precision = op, pc, bj, precision
+        self.restart = restart
+        the_pc = pc if pc is not None else bj
+        flexible = pc is not None
+        if precision == "fp64":
+            self.system = InteriorSystem(op, PREC_F64, the_pc, PREC_F64)
+            self.gm = GMRES(self.system, op.ndof, op.device, restart=restart, flexible=flexible)
+        else:
+            self.hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
+            self.lo = InteriorSystem(op, PREC_MIXED, the_pc, bj_precision)
+            self.gm = GMRES(self.lo, op.ndof, op.device, restart=restart, flexible=flexible)
+
+    def solve(self, rhs, x0=None, rtol=1e-8, max_it=5000, inner_rtol=1e-2, max_outer=60,
+              judge=True):
+        """Solve to an FP64 relative residual ``rtol``.  With ``judge`` the
+        FP64 true residual decides whether the coarse correction stays on
+        (``_Judge``: one trial step with and one without, then the faster
+        per second; later, a step with the correction that does not cut the
+        residual by 10 % switches it off)."""
+        pc = self.pc
+        if pc is not None:
+            pc.enabled = True
+        jd = _Judge(pc if judge else None)
+        if self.precision == "fp64":
+            x, info = self.gm.solve(rhs, x0, rtol=rtol, max_it=max_it,
+                                    on_restart=jd.restart_hook(self.system, rhs))
+        else:
+            x, info = refine(self.hi, self.lo, rhs, x0, rtol=rtol, inner_rtol=inner_rtol,
+                             restart=self.restart, max_outer=max_outer, max_inner=max_it,
+                             flexible=pc is not None, after_outer=jd.outer_hook, inner=self.gm)
+        info["fallback"] = jd.fell_back
         return x, info
-    hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
-    lo = InteriorSystem(op, PREC_MIXED, the_pc, bj_precision)
-    judge = _Judge(pc)
-    x, info = refine(hi, lo, rhs, x0, rtol=rtol, inner_rtol=inner_rtol, restart=restart,
-                     max_outer=max_outer, max_inner=max_it, flexible=pc is not None,
-                     after_outer=judge.outer_hook)
-    info["fallback"] = judge.fell_back
-    return x, info
+
+
+def solve(op, rhs, x0=None, pc=None, bj=None, precision="mixed", rtol=1e-8, restart=50,
+          max_it=5000, inner_rtol=1e-2, max_outer=60, bj_precision=PREC_MIXED, judge=True):
+    """One solve with a fresh ``Session`` (see there)."""
+    return Session(op, pc=pc, bj=bj, precision=precision, restart=restart,
+                   bj_precision=bj_precision).solve(rhs, x0, rtol=rtol, max_it=max_it,
+                                                    inner_rtol=inner_rtol,
+                                                    max_outer=max_outer, judge=judge)
 
 
 class _Judge:
+    """The FP64 true residual decides whether the coarse correction stays on.
+
+    Over the first two judged steps (outer refinement steps, or restart
+    cycles in FP64) it runs one step with the correction and one without and
+    keeps whichever reduced the FP64 residual faster per second; afterwards a
+    step with the correction that does not cut the residual by 10 % switches
+    it off for good."""
+
     def __init__(self, pc):
         self.pc = pc
         self.prev = None
+        self.t_prev = None
+        self.rates = []
         self.fell_back = False
 
     def _judge(self, rn):
-        if self.pc is not None and self.pc.enabled and self.prev is not None:
-            if not (rn < 0.9 * self.prev):
-                self.pc.enabled = False
+        now = time.perf_counter()
+        pc = self.pc
+        if pc is not None and self.prev is not None and np.isfinite(rn) and rn > 0:
+            dt = max(now - self.t_prev, 1e-9)
+            rate = np.log(self.prev / rn) / dt
+            if len(self.rates) == 0:
+                self.rates.append(("learned", rate))
+                pc.enabled = False                # trial step without the correction
+            elif len(self.rates) == 1:
+                self.rates.append(("plain", rate))
+                pc.enabled = self.rates[0][1] > rate
+                self.fell_back = not pc.enabled
+            elif pc.enabled and not (rn < 0.9 * self.prev):
+                pc.enabled = False
                 self.fell_back = True
-        self.prev = rn
+        self.prev, self.t_prev = rn, now
 
     def outer_hook(self, outer, rn):
         self._judge(rn)
