@@ -71,10 +71,14 @@ def _geom(col):
     return x0, y0, x1, y1, nx, ny, xb, wx, yb, wy
 
 
-def assemble(mesh, kappa, sigma, Phi, cols=None, parts=("M", "S", "IC", "BC", "P")):
+def assemble(mesh, kappa, sigma, Phi, cols=None, parts=("M", "S", "IC", "BC", "P"), drop=True):
     """Dict of CSR matrices (full n x n shape).  ``S`` is the reference's
     returned scattering matrix (i.e. -S); ``P`` = M + S_ret computed as the
-    reference's preconditioner_matrix does."""
+    reference's preconditioner_matrix does.  ``drop=False`` keeps the entries
+    the reference's thresholds drop (|S| <= 1e-14, ``scattering_matrix.py:114``;
+    |BC| <= 1e-15, ``boundary_convection_matrix.py:310,335``): the exact
+    discrete operator, which the matrix-free device path applies."""
+    small = SMALL if drop else -1.0
     offs, n = dof_offsets(mesh)
     sel = None if cols is None else set(cols)
     acc = {p: _Acc() for p in parts}
@@ -119,7 +123,7 @@ def assemble(mesh, kappa, sigma, Phi, cols=None, parts=("M", "S", "IC", "BC", "P
                               o + nth * ny * ii + nth * jj + aa, v)
         if "S" in parts or "P" in parts:
             wws = ww * sig
-            s_ok = np.abs(wws) > SMALL
+            s_ok = np.abs(wws) > small
             sidx = (I * ny + J)
             for k0, c0, n0, w0, th0, dth0, o0 in cinfo:
                 dc = dx * dy * dth0 / 8.0
@@ -131,7 +135,7 @@ def assemble(mesh, kappa, sigma, Phi, cols=None, parts=("M", "S", "IC", "BC", "P
                     colsi = o1 + sidx[:, :, None, None] * n1 + np.arange(n1)[None, None, None, :]
                     rows = np.broadcast_to(rows, vals.shape)
                     colsi = np.broadcast_to(colsi, vals.shape)
-                    keep = np.abs(vals) > SMALL
+                    keep = np.abs(vals) > small
                     if k0 != k1:
                         keep &= s_ok[:, :, None, None]
                     if "S" in parts:
@@ -144,17 +148,17 @@ def assemble(mesh, kappa, sigma, Phi, cols=None, parts=("M", "S", "IC", "BC", "P
                             pv[:, :, r_eq, r_eq] = dc * (w1[None, None, :] * wwk[:, :, None]
                                                          - (dth1 / 2.0) * wws[:, :, None]
                                                          * (w0 * w1 * np.diag(Ph))[None, None, :])
-                        pkeep = np.abs(pv) > SMALL
+                        pkeep = np.abs(pv) > small
                         if k0 != k1:
                             pkeep &= s_ok[:, :, None, None]
                         acc["P"].add(rows[pkeep], colsi[pkeep], pv[pkeep])
         if "BC" in parts:
-            _bc_column(mesh, ck, col, cinfo, offs, acc["BC"])
+            _bc_column(mesh, ck, col, cinfo, offs, acc["BC"], drop)
     return {p: acc[p].csr(n) for p in parts}
 
 
-def _bc_column(mesh, ck, col, cinfo, offs, acc):
-    tol = 1.0e-15
+def _bc_column(mesh, ck, col, cinfo, offs, acc, drop=True):
+    tol = 1.0e-15 if drop else -1.0
     x0, y0, x1, y1, nx, ny, xb, wx, yb, wy = _geom(col)
     dx, dy = x1 - x0, y1 - y0
     for F in range(4):

@@ -207,3 +207,69 @@ def grf_scatterer(seed, n=64, length_scale=0.15, ratio_range=(0.9, 0.999), g=0.5
     wl = Workload("grf_scatterer", kappa, sigma, Phi, f, bcs)
     wl.seed, wl.ratio, wl.logk = seed, ratio, logk
     return wl
+
+
+# ---------------------------------------------------------------- random hp mesh
+def random_hp_mesh(seed=0, base_spt=4, base_ang=3, extra_spt=3, extra_ang=2, p_col=0.3,
+                   p_cell=0.2, orders=(3, 9), Ls=(1.0, 1.0)):
+    """Batched uneven-order hp mesh: a 2^base_spt x 2^base_spt x 2^base_ang
+    base mesh, then (seeded) random h-refinement - every leaf column below
+    base_spt + extra_spt levels is split with probability p_col per round,
+    every leaf cell below base_ang + extra_ang with probability p_cell per
+    round, extra_spt / extra_ang rounds, the reference's 2:1 balance
+    refining neighbours as it goes (``Mesh.ref_col`` / ``ref_cell``) - and
+    finally per-column (nx = ny) and per-cell (nth) node counts drawn
+    uniformly from ``orders`` (inclusive).  Random orders do not keep the
+    reference's |dn| <= 1 p-balance; the face couplings handle any orders.
+    The defaults (16 x 16 x 8 base, p_col 0.3, p_cell 0.2, seed 0) give
+    2,338 columns, 34,279 cells and 8,263,762 DOF."""
+    rng = np.random.default_rng(seed)
+    mesh = uniform_mesh(Ls, [orders[0]] * 3, base_ang, base_spt)
+    for _ in range(extra_spt):
+        for key in sorted(mesh.cols):
+            col = mesh.cols.get(key)
+            if col is None or not col.is_lf or col.lv >= base_spt + extra_spt:
+                continue
+            if rng.random() < p_col:
+                mesh.ref_col(key, kind="spt", form="h")
+    for _ in range(extra_ang):
+        for key in sorted(mesh.cols):
+            col = mesh.cols[key]
+            for ck in sorted(col.cells):
+                cell = col.cells.get(ck)
+                if cell is None or not cell.is_lf or cell.lv >= base_ang + extra_ang:
+                    continue
+                if rng.random() < p_cell:
+                    mesh.ref_cell(key, ck, form="h")
+    for key in sorted(mesh.cols):
+        col = mesh.cols[key]
+        n = int(rng.integers(orders[0], orders[1] + 1))
+        col.ndofs = [n, n]
+        for ck in sorted(col.cells):
+            col.cells[ck].ndofs = [int(rng.integers(orders[0], orders[1] + 1))]
+    return mesh
+
+
+def two_scatterers_c3(g=0.9):
+    """kappa = 1 + 9 max(g1, g2), sigma / kappa = 0.1 + 0.89 max(g1, g2),
+    g_k = exp(-|x - c_k|^2 / 0.1^2), c = (0.3, 0.6), (0.7, 0.4);
+    Henyey-Greenstein g = 0.9; unit isotropic inflow on the left wall."""
+    Phi = henyey_greenstein(g)
+
+    def blob(x, y):
+        return np.maximum(np.exp(-((x - 0.3) ** 2 + (y - 0.6) ** 2) / 0.01),
+                          np.exp(-((x - 0.7) ** 2 + (y - 0.4) ** 2) / 0.01))
+
+    def kappa(x, y):
+        return 1.0 + 9.0 * blob(x, y)
+
+    def sigma(x, y):
+        return (0.1 + 0.89 * blob(x, y)) * kappa(x, y)
+
+    def f(x, y, th):
+        return 0.0 * (x + y + th)
+
+    def bcs(x, y, th):
+        return np.where(np.asarray(x) == 0.0, 1.0, 0.0) + 0.0 * (y + th)
+
+    return Workload("two_scatterers_c3", kappa, sigma, Phi, f, bcs)

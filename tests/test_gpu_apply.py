@@ -129,3 +129,79 @@ def test_apply_c2_rows_and_linearity(cuda):
     lhs = op.apply(2.0 * x1 - 3.0 * x2, ops=OP_SYSTEM)
     rhs = 2.0 * op.apply(x1, ops=OP_SYSTEM) - 3.0 * op.apply(x2, ops=OP_SYSTEM)
     assert float((lhs - rhs).norm() / rhs.norm()) < 1e-13
+
+
+def small_random_hp(seed):
+    """A small mesh of the batched uneven-order kind: 4 x 4 x 4 base, one
+    random spatial and one angular refinement round, node counts 3..9 per
+    column and per cell (about 60,000 DOF)."""
+    return problems.random_hp_mesh(seed, base_spt=2, base_ang=2, extra_spt=1, extra_ang=1,
+                                   p_col=0.5, p_cell=0.5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", [0, 1], ids=["generic", "fused"])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_apply_random_hp_matches_oracle(cuda, seed, path):
+    """Uneven orders on both sides of every face and level jumps in space and
+    angle: every operator part against the oracle's assembled matrices."""
+    from rtx.operator import (TransportOperator, OP_MASS, OP_SCAT, OP_ICONV, OP_BCONV,
+                              OP_SYSTEM, OP_RESTRICT)
+    mesh = small_random_hp(seed)
+    wl = problems.two_scatterers_c3()
+    print("random hp seed", seed, "path", path, "ndof", mesh.get_ndof(), flush=True)
+    V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi)
+    A = (V["BC"] - V["IC"] + V["M"] + V["S"]).tocsr()
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    op.set_path(path)
+    x = np.random.default_rng(seed).uniform(-1.0, 1.0, op.ndof)
+    xd = _dev(x)
+    for name, ops, ref in [("M", OP_MASS, V["M"] @ x), ("S", OP_SCAT, V["S"] @ x),
+                           ("IC", OP_ICONV, -(V["IC"] @ x)), ("BC", OP_BCONV, V["BC"] @ x),
+                           ("A", OP_SYSTEM, A @ x)]:
+        y = op.apply(xd, ops=ops).cpu().numpy()
+        assert _rel(y, ref) < TOL64, (name, _rel(y, ref))
+    mask = asm.intr_mask(mesh)
+    y = op.apply(xd, ops=OP_SYSTEM | OP_RESTRICT).cpu().numpy()
+    assert _rel(y, mask * (A @ (mask * x))) < TOL64
+
+
+@pytest.mark.gpu
+def test_apply_c3_rows_and_linearity(cuda):
+    """Full-size batched uneven-order hp mesh (16 x 16 x 8 base, random
+    refinement, node counts 3..9: 8,263,762 DOF): rows of sampled columns
+    (the largest column among them) against the oracle, linearity over the
+    whole vector."""
+    import torch
+    from rtx.operator import TransportOperator, OP_SYSTEM
+    mesh = problems.random_hp_mesh(0)
+    wl = problems.two_scatterers_c3()
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    assert op.ndof == 8_263_762
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, op.ndof)
+    y = op.apply(_dev(x), ops=OP_SYSTEM).cpu().numpy()
+    keys = sorted(mesh.cols)
+    size = {k: mesh.cols[k].ndofs[0] ** 2 * sum(c.ndofs[0] for c in mesh.cols[k].cells.values())
+            for k in keys}
+    sample = [keys[0], keys[len(keys) // 2], keys[-1], max(keys, key=lambda k: size[k])]
+    # the exact discrete operator (no drop thresholds): the device path applies it
+    V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi, cols=sample, drop=False)
+    A = (V["BC"] - V["IC"] + V["M"] + V["S"]).tocsr()
+    rows = np.unique(A.nonzero()[0])
+    assert rows.size == sum(size[k] for k in set(sample))
+    assert _rel(y[rows], (A @ x)[rows]) < TOL64
+    # the reference's assembled matrix drops |S| <= 1e-14 and |BC| <= 1e-15;
+    # at this mesh's smallest cells genuine entries fall below that, and the
+    # whole difference is those dropped entries
+    Vr = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi, cols=sample)
+    Ar = (Vr["BC"] - Vr["IC"] + Vr["M"] + Vr["S"]).tocsr()
+    Dd = (A - Ar).tocsr()
+    Dd.eliminate_zeros()
+    assert Dd.nnz > 0 and abs(Dd).max() <= 1e-14
+    assert _rel(y[rows] - (Dd @ x)[rows], (Ar @ x)[rows]) < TOL64
+    x2 = torch.from_numpy(rng.uniform(-1, 1, op.ndof)).cuda()
+    x1 = _dev(x)
+    lhs = op.apply(2.0 * x1 - 3.0 * x2, ops=OP_SYSTEM)
+    rhs = 2.0 * op.apply(x1, ops=OP_SYSTEM) - 3.0 * op.apply(x2, ops=OP_SYSTEM)
+    assert float((lhs - rhs).norm() / rhs.norm()) < 1e-13

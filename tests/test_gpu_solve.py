@@ -145,3 +145,26 @@ def test_graph_replay_matches_eager(cuda, variant):
         assert other[2] == runs[0][2]
         assert np.array_equal(other[0], runs[0][0])
         assert list(other[1]) == list(runs[0][1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_random_hp_solve_matches_spsolve(cuda, precision):
+    """Batched uneven-order hp mesh (node counts 3..9, cells of up to 729
+    DOFs): the interior solve against spsolve of the oracle's system."""
+    import torch
+    from rtx.operator import TransportOperator, OP_SYSTEM, OP_QUADW
+    from rtx import learned
+    from test_gpu_apply import small_random_hp
+    mesh = small_random_hp(1)
+    wl = problems.two_scatterers_c3()
+    A, f, b, mask = _oracle_system(mesh, wl)
+    u_ref, A_II, rhs_ref = osys.exact_solve(A, f, b, mask)
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    fd = op.apply(torch.from_numpy(op.nodal(wl.f)).cuda(), ops=OP_QUADW)
+    bd = op.mask(torch.from_numpy(op.nodal(wl.bcs)).cuda(), keep_interior=False)
+    rhs = op.mask(fd - op.apply(bd, ops=OP_SYSTEM), keep_interior=True)
+    x, info = learned.solve(op, rhs, None, precision=precision, rtol=1e-10, inner_rtol=1e-3)
+    assert info["converged_reason"] > 0
+    u = (x + bd).cpu().numpy()
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < TOL_SOL
