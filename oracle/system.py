@@ -53,14 +53,36 @@ def cell_blocks(mesh):
     return out
 
 
-def block_jacobi_inverses(A, mesh, mask):
-    """Dense inverse of each cell's diagonal block of A restricted to the
-    interior DOFs, with identity rows/columns at boundary DOFs."""
+def bj_block_dofs(mesh, split=None):
+    """Global DOF indices of every block-Jacobi block, in cell order.  A cell
+    is one block, or - when ``split`` is given and the cell has more than
+    ``split`` DOFs - several blocks of g = max(1, split // (nx ny))
+    consecutive angles each (all spatial nodes), in angle order."""
+    offs, _ = dof_offsets(mesh)
+    out = []
+    for ck, kks in leaf_order(mesh):
+        col = mesh.cols[ck]
+        nsp = col.ndofs[0] * col.ndofs[1]
+        for kk in kks:
+            nth = col.cells[kk].ndofs[0]
+            o = offs[(ck, kk)]
+            g = max(1, split // nsp) if (split is not None and nsp * nth > split) else nth
+            for a0 in range(0, nth, g):
+                a = np.arange(a0, min(a0 + g, nth))
+                s = np.arange(nsp)
+                out.append((o + s[:, None] * nth + a[None, :]).ravel())
+    return out
+
+
+def block_jacobi_inverses(A, mesh, mask, split=None):
+    """Dense inverse of each block-Jacobi block (``bj_block_dofs``) of A
+    restricted to the interior DOFs, with identity rows/columns at boundary
+    DOFs."""
     A = sp.csr_matrix(A)
     invs = []
-    for o, n in cell_blocks(mesh):
-        blk = A[o:o + n, o:o + n].toarray()
-        m = mask[o:o + n]
+    for idx in bj_block_dofs(mesh, split):
+        blk = A[idx][:, idx].toarray()
+        m = mask[idx]
         blk[~m, :] = 0.0
         blk[:, ~m] = 0.0
         blk[~m, ~m] = 1.0
@@ -68,10 +90,10 @@ def block_jacobi_inverses(A, mesh, mask):
     return invs
 
 
-def block_jacobi_apply(invs, mesh, r):
+def block_jacobi_apply(invs, mesh, r, split=None):
     z = np.zeros_like(r)
-    for (o, n), inv in zip(cell_blocks(mesh), invs):
-        z[o:o + n] = inv @ r[o:o + n]
+    for idx, inv in zip(bj_block_dofs(mesh, split), invs):
+        z[idx] = inv @ r[idx]
     return z
 
 

@@ -13,6 +13,8 @@
 // it by Gauss-Jordan with partial pivoting (column-parallel elimination, one
 // pivot search per wavefront-shuffle reduction), and stores the inverse
 // column-major so the apply is a coalesced batched GEMV.
+#include <algorithm>
+
 #include "rtx_internal.h"
 
 namespace rtx {
@@ -20,6 +22,7 @@ namespace rtx {
 constexpr int NMq = RTX_NMAX;
 constexpr int BJ_THREADS = 256;
 constexpr int BJ_COLS = 128;     // column threads in the elimination
+constexpr int BJ_SPLIT = 128;    // larger cells are split into groups of angles
 
 __device__ inline bool bj_on_bdry(int bd, int i, int j, int nx, int ny) {
     return ((bd & 1) && i == nx - 1) || ((bd & 2) && j == ny - 1) || ((bd & 4) && i == 0) ||
@@ -311,6 +314,7 @@ __device__ inline double bj_entry(const DevMesh &m, const BJCell &x, const BJSel
 template <int RR, int CC, int TJ>
 __global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restrict_,
                                                          const int64_t *inv_off,
+                                                         const int32_t *__restrict__ blk,
                                                          double *__restrict__ out,
                                                          int *__restrict__ status) {
     constexpr int NR = 16 * RR;
@@ -323,7 +327,8 @@ __global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restric
     __shared__ int perm[NR], qinv[NR];
     __shared__ int used[NR];
     __shared__ BJSelf sl;
-    const int k = blockIdx.x, tid = threadIdx.x, ti = tid / TJ, tj = tid % TJ;
+    const int b = blockIdx.x, tid = threadIdx.x, ti = tid / TJ, tj = tid % TJ;
+    const int k = blk[3 * b], a0 = blk[3 * b + 1], g = blk[3 * b + 2];
     BJCell x;
     const int c = m.cell_col[k];
     x.nx = m.col_nx[c];
@@ -365,7 +370,9 @@ __global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restric
         sth[tid] = s;
     }
     if (tid < NR) used[tid] = 0;
-    const int n = x.n;
+    // the block: angles a0 .. a0 + g - 1 of the cell at every spatial node;
+    // block index t <-> cell DOF (t / g) nth + a0 + t % g
+    const int n = (x.n / x.nth) * g;
     __syncthreads();
     double A[RR][CC];
 #pragma unroll
@@ -373,7 +380,10 @@ __global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restric
 #pragma unroll
         for (int v = 0; v < CC; ++v) {
             const int row = ti + 16 * u, col = tj + TJ * v;
-            A[u][v] = (row < n && col < n) ? bj_entry(m, x, sl, cth, sth, row, col) : 0.0;
+            A[u][v] = (row < n && col < n)
+                          ? bj_entry(m, x, sl, cth, sth, (row / g) * x.nth + a0 + row % g,
+                                     (col / g) * x.nth + a0 + col % g)
+                          : 0.0;
         }
     for (int kk = 0; kk < n; ++kk) {
         const int buf = kk & 1, vk = kk / TJ;
@@ -456,7 +466,7 @@ __global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restric
     __syncthreads();
     if (tid < n) qinv[perm[tid]] = tid;
     __syncthreads();
-    double *o = out + inv_off[k];
+    double *o = out + inv_off[b];
 #pragma unroll
     for (int u = 0; u < RR; ++u)
 #pragma unroll
@@ -481,18 +491,22 @@ struct bj_bf16 {
 
 template <typename T, typename M>
 __global__ __launch_bounds__(BJ_THREADS) void k_bj_apply(DevMesh m, const int64_t *inv_off,
+                                                        const int32_t *__restrict__ blk,
                                                         const M *__restrict__ inv,
                                                         const T *__restrict__ r,
                                                         T *__restrict__ z) {
     __shared__ double rs[RTX_NMAX * RTX_NMAX * RTX_NMAX];
     __shared__ double part[BJ_THREADS];
-    const int k = blockIdx.x, tid = threadIdx.x;
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int k = blk[3 * b], a0 = blk[3 * b + 1], g = blk[3 * b + 2];
     const int c = m.cell_col[k];
-    const int n = m.col_nx[c] * m.col_ny[c] * m.cell_nth[k];
-    const int off = m.cell_dof_off[k];
-    for (int i = tid; i < n; i += BJ_THREADS) rs[i] = double(r[off + i]);
+    const int nth = m.cell_nth[k];
+    const int n = m.col_nx[c] * m.col_ny[c] * g;
+    // block index t <-> cell DOF (t / g) nth + a0 + t % g (identity for a whole cell)
+    const int off = m.cell_dof_off[k] + a0;
+    for (int i = tid; i < n; i += BJ_THREADS) rs[i] = double(r[off + (i / g) * nth + i % g]);
     __syncthreads();
-    const M *B = inv + inv_off[k];
+    const M *B = inv + inv_off[b];
     if (n <= BJ_THREADS / 2) {
         // two column halves per output row
         const int half = tid / (BJ_THREADS / 2), i = tid % (BJ_THREADS / 2);
@@ -504,13 +518,13 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_apply(DevMesh m, const int64_
         }
         part[tid] = acc;
         __syncthreads();
-        if (tid < n) z[off + tid] = T(part[tid] + part[tid + BJ_THREADS / 2]);
+        if (tid < n) z[off + (tid / g) * nth + tid % g] = T(part[tid] + part[tid + BJ_THREADS / 2]);
     } else {
         for (int i = tid; i < n; i += BJ_THREADS) {
             double acc = 0.0;
 #pragma unroll 4
             for (int j = 0; j < n; ++j) acc += double(B[size_t(j) * n + i]) * rs[j];
-            z[off + i] = T(acc);
+            z[off + (i / g) * nth + i % g] = T(acc);
         }
     }
 }
@@ -526,16 +540,39 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
     RTX_HIP(hipMemcpy(ny.data(), p->m.col_ny, ny.size() * 4, hipMemcpyDeviceToHost));
     RTX_HIP(hipMemcpy(cc.data(), p->m.cell_col, cc.size() * 4, hipMemcpyDeviceToHost));
     RTX_HIP(hipMemcpy(nt.data(), p->m.cell_nth, nt.size() * 4, hipMemcpyDeviceToHost));
-    std::vector<int64_t> off(p->m.ncell + 1, 0);
+    const int split = (p->bj_method != 0) ? 1 : 0;
+    std::vector<int32_t> blk;
+    std::vector<int64_t> off(1, 0);
+    int32_t maxb = 0;
     for (int k = 0; k < p->m.ncell; ++k) {
-        int64_t n = int64_t(nx[cc[k]]) * ny[cc[k]] * nt[k];
-        off[k + 1] = off[k] + n * n;
+        const int nsp = nx[cc[k]] * ny[cc[k]], nth = nt[k];
+        const int g = (split && nsp * nth > BJ_SPLIT) ? std::max(1, BJ_SPLIT / nsp) : nth;
+        for (int a0 = 0; a0 < nth; a0 += g) {
+            const int gg = std::min(g, nth - a0);
+            const int64_t n = int64_t(nsp) * gg;
+            blk.push_back(k);
+            blk.push_back(a0);
+            blk.push_back(gg);
+            off.push_back(off.back() + n * n);
+            maxb = std::max<int32_t>(maxb, int32_t(n));
+        }
     }
-    if (!p->bj_off) {
+    if (p->bj_split != split || !p->bj_off) {
+        if (p->bj_off) RTX_HIP(hipFree(p->bj_off));
+        if (p->bj_blk) RTX_HIP(hipFree(p->bj_blk));
+        p->bj_off = nullptr;
+        p->bj_blk = nullptr;
         RTX_HIP(hipMalloc(&p->bj_off, off.size() * sizeof(int64_t)));
         RTX_HIP(hipMemcpy(p->bj_off, off.data(), off.size() * sizeof(int64_t),
                           hipMemcpyHostToDevice));
+        RTX_HIP(hipMalloc(&p->bj_blk, blk.size() * sizeof(int32_t)));
+        RTX_HIP(hipMemcpy(p->bj_blk, blk.data(), blk.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice));
+        p->bj_split = split;
+        p->bj_ready = false;
     }
+    p->bj_nblk = int32_t(blk.size() / 3);
+    p->bj_max_block = maxb;
     p->bj_entries = off.back();
     *n_entries = off.back();
     return RTX_OK;
@@ -548,13 +585,13 @@ extern "C" int rtx_bjacobi_factor(rtx_plan *p, int restrict_interior, double *d_
         set_error("rtx_bjacobi_factor: coefficients not set");
         return RTX_ERR_STATE;
     }
-    if (!p->bj_off) {
+    if (!p->bj_off || p->bj_split != ((p->bj_method != 0) ? 1 : 0)) {
         int64_t n;
         int rc = rtx_bjacobi_size(p, &n);
         if (rc) return rc;
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const int64_t nmax = p->max_cell_ndof;
+    const int64_t nmax = p->bj_max_block;
     const size_t lds = size_t(nmax) * nmax * sizeof(double);
     int dev = 0, max_lds = 0;
     RTX_HIP(hipGetDevice(&dev));
@@ -566,14 +603,17 @@ extern "C" int rtx_bjacobi_factor(rtx_plan *p, int restrict_interior, double *d_
     const bool reg = p->bj_method != 0 && nmax <= 128;
     if (reg) {
         if (nmax <= 32)
-            k_bj_build_reg<2, 2, 16><<<p->m.ncell, 256, 0, st>>>(p->m, restrict_interior,
-                                                                p->bj_off, d_inv, d_status);
+            k_bj_build_reg<2, 2, 16><<<p->bj_nblk, 256, 0, st>>>(p->m, restrict_interior,
+                                                                p->bj_off, p->bj_blk, d_inv,
+                                                                d_status);
         else if (nmax <= 64)
-            k_bj_build_reg<4, 4, 16><<<p->m.ncell, 256, 0, st>>>(p->m, restrict_interior,
-                                                                p->bj_off, d_inv, d_status);
+            k_bj_build_reg<4, 4, 16><<<p->bj_nblk, 256, 0, st>>>(p->m, restrict_interior,
+                                                                p->bj_off, p->bj_blk, d_inv,
+                                                                d_status);
         else
-            k_bj_build_reg<8, 4, 32><<<p->m.ncell, 512, 0, st>>>(p->m, restrict_interior,
-                                                                p->bj_off, d_inv, d_status);
+            k_bj_build_reg<8, 4, 32><<<p->bj_nblk, 512, 0, st>>>(p->m, restrict_interior,
+                                                                p->bj_off, p->bj_blk, d_inv,
+                                                                d_status);
     } else if (p->bj_method == 1) {
         (void)hipFreeAsync(d_status, st);
         set_error("rtx_bjacobi_factor: the register build handles blocks of up to 128 DOFs, "
@@ -626,14 +666,14 @@ extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, 
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (precision == RTX_PREC_F64)
-        k_bj_apply<double, double><<<p->m.ncell, BJ_THREADS, 0, st>>>(
-            p->m, p->bj_off, (const double *)d_inv, (const double *)d_r, (double *)d_z);
+        k_bj_apply<double, double><<<p->bj_nblk, BJ_THREADS, 0, st>>>(
+            p->m, p->bj_off, p->bj_blk, (const double *)d_inv, (const double *)d_r, (double *)d_z);
     else if (precision == RTX_PREC_F32)
-        k_bj_apply<float, float><<<p->m.ncell, BJ_THREADS, 0, st>>>(
-            p->m, p->bj_off, (const float *)d_inv, (const float *)d_r, (float *)d_z);
+        k_bj_apply<float, float><<<p->bj_nblk, BJ_THREADS, 0, st>>>(
+            p->m, p->bj_off, p->bj_blk, (const float *)d_inv, (const float *)d_r, (float *)d_z);
     else if (precision == RTX_PREC_MIXED)
-        k_bj_apply<float, bj_bf16><<<p->m.ncell, BJ_THREADS, 0, st>>>(
-            p->m, p->bj_off, (const bj_bf16 *)d_inv, (const float *)d_r, (float *)d_z);
+        k_bj_apply<float, bj_bf16><<<p->bj_nblk, BJ_THREADS, 0, st>>>(
+            p->m, p->bj_off, p->bj_blk, (const bj_bf16 *)d_inv, (const float *)d_r, (float *)d_z);
     else {
         set_error("rtx_bjacobi_apply: unsupported precision %d", precision);
         return RTX_ERR_ARG;

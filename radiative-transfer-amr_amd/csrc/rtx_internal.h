@@ -91,6 +91,11 @@ struct rtx_plan {
     int64_t bj_entries = 0;
     bool bj_ready = false;
     int bj_method = -1;          // -1 automatic, 0 LDS Gauss-Jordan, 1 register sweeps
+    // blocks: (cell, first angle, angle count) triples; a cell of more than
+    // BJ_SPLIT DOFs is split into groups of angles unless bj_method == 0
+    int32_t *bj_blk = nullptr;
+    int32_t bj_nblk = 0, bj_max_block = 0;
+    int bj_split = -1;           // split setting the block table was built with
     // coarse grid of the learned correction: per column a list of
     // (coarse cell, area(col & cell) / area(cell), area(col & cell) / area(col))
     void *coarse = nullptr;      // one allocation for the map

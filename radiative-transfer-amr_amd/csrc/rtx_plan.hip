@@ -326,6 +326,7 @@ extern "C" void rtx_plan_destroy(rtx_plan *p) {
     if (p->arena) (void)hipFree(p->arena);
     if (p->coef) (void)hipFree(p->coef);
     if (p->bj_off) (void)hipFree(p->bj_off);
+    if (p->bj_blk) (void)hipFree(p->bj_blk);
     if (p->coarse) (void)hipFree(p->coarse);
     (void)hipSetDevice(cur);
     delete p;
