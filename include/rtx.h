@@ -150,6 +150,23 @@ int rtx_apply(rtx_plan *plan, uint32_t op_mask, int precision, const void *d_x,
    forced path). */
 int rtx_set_path(rtx_plan *plan, int mode);
 
+/* Autotune the apply's launch shapes on this plan: for the uniform-bin kernel
+   and every fused order bin (columns grouped by the number of parts H their
+   LDS needs), time each valid H in {1, 2, 4} with `reps` launches of the
+   interior operator on d_x -> d_y (device vectors of `precision`) and keep
+   the fastest.  Synchronises the stream.  No counterpart in the reference
+   (its assembly has no launch shapes). */
+int rtx_autotune(rtx_plan *plan, int precision, const void *d_x, void *d_y, int reps,
+                 void *stream);
+/* Force the uniform-bin kernel's parts per column (1, 2 or 4 where valid for
+   the plan's shape; 0 restores the default).  No counterpart in the
+   reference. */
+int rtx_set_uniform_parts(rtx_plan *plan, int parts);
+/* Launch-shape summary (10 values): uniform bin usable, its parts per column;
+   parts and columns of the three fused order bins; columns and cells left to
+   the generic kernels.  Returns the count. */
+int rtx_plan_info(const rtx_plan *plan, int64_t *out, int n);
+
 /* In-place mask of a full-length vector: keep = 0 zeroes the inflow-boundary
    DOFs (keeps the interior, split_vector.py:39-46), keep = 1 zeroes the
    interior DOFs (keeps the boundary part). */

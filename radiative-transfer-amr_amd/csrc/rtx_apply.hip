@@ -35,12 +35,13 @@ __device__ inline double theta_f(int F, double c, double s) {
 
 template <typename T>
 __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int accumulate,
+                                                  const int32_t *__restrict__ cells,
                                                   const T *__restrict__ x, T *__restrict__ y) {
     __shared__ double xs[NM * NM * NM];
     __shared__ double ys[NM * NM * NM];
     __shared__ double tr[NM * NM];
     __shared__ double cth[NM], sth[NM];
-    const int k = blockIdx.x;
+    const int k = cells ? cells[blockIdx.x] : blockIdx.x;
     const int lane = threadIdx.x;
     const int c = m.cell_col[k];
     const int nx = m.col_nx[c], ny = m.col_ny[c], nth = m.cell_nth[k];
@@ -161,6 +162,7 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
 
 template <typename T>
 __global__ __launch_bounds__(SCAT_THREADS) void k_scatter(DevMesh m, uint32_t ops, int accumulate,
+                                                          const int32_t *__restrict__ cols,
                                                           const T *__restrict__ x,
                                                           T *__restrict__ y) {
     __shared__ double us[SCAT_LDS_DOUBLES];
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(SCAT_THREADS) void k_scatter(DevMesh m, uint32_t op
     __shared__ int a_nt[MAX_NANG];     // that cell's nth
     __shared__ int a_loc[MAX_NANG];    // local angular index within the cell
     __shared__ int a_bd[MAX_NANG];     // that cell's masked faces
-    const int c = blockIdx.x;
+    const int c = cols ? cols[blockIdx.x] : blockIdx.x;
     const int tid = threadIdx.x;
     const int nx = m.col_nx[c], ny = m.col_ny[c], nsp = nx * ny;
     const int na = m.col_nang[c];
@@ -241,32 +243,39 @@ __global__ __launch_bounds__(64) void k_mask(DevMesh m, int keep, T *__restrict_
     }
 }
 
+// Generic per-cell transport + per-column scattering on all cells / columns
+// (cells, cols null) or on the given lists (the columns no fused bin takes).
 template <typename T>
-static int launch_apply(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t st) {
+static int launch_apply(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t st,
+                        const int32_t *cells = nullptr, int ncells = -1,
+                        const int32_t *cols = nullptr, int ncols = -1) {
     const T *x = static_cast<const T *>(dx);
     T *y = static_cast<T *>(dy);
     const bool acc = (ops & RTX_OP_ACCUM) != 0;
     const bool tp = (ops & (RTX_OP_MASS | RTX_OP_ICONV | RTX_OP_BCONV | RTX_OP_QUADW)) != 0;
     const bool sc = (ops & RTX_OP_SCAT) != 0;
+    if (ncells < 0) ncells = p->m.ncell;
+    if (ncols < 0) ncols = p->m.ncol;
     if (!tp && !sc) {
         if (!acc) k_zero<T><<<1024, 256, 0, st>>>(y, p->m.ndof);
+        RTX_HIP(hipGetLastError());
+        return RTX_OK;
     }
-    if (tp)
-        k_transport<T><<<p->m.ncell, 64, 0, st>>>(p->m, ops, acc ? 1 : 0, x, y);
-    if (sc)
-        k_scatter<T><<<p->m.ncol, SCAT_THREADS, 0, st>>>(p->m, ops, (acc || tp) ? 1 : 0, x, y);
+    if (tp && ncells > 0)
+        k_transport<T><<<ncells, 64, 0, st>>>(p->m, ops, acc ? 1 : 0, cells, x, y);
+    if (sc && ncols > 0)
+        k_scatter<T><<<ncols, SCAT_THREADS, 0, st>>>(p->m, ops, (acc || tp) ? 1 : 0, cols, x, y);
     RTX_HIP(hipGetLastError());
     return RTX_OK;
 }
 
 // The fused per-column kernel holds a whole column (values and accumulators)
 // in LDS and tiles its spatial nodes by 16 up to 7 tiles.
-constexpr size_t FUSED_LDS_CAP = 96 * 1024;
 
-static bool fused_fits(const rtx_plan *p, int precision) {
-    const int H = fused_parts(p, precision);
-    return p->max_col_cells <= 64 && p->max_nsp <= 112 && p->max_part_links[H] <= 256 &&
-           fused_lds_bytes(p, H, precision) <= FUSED_LDS_CAP;
+static bool any_fused(const rtx_plan *p) {
+    for (const FusedBin &fb : p->fbin)
+        if (fb.n > 0) return true;
+    return false;
 }
 
 }  // namespace rtx
@@ -301,11 +310,30 @@ extern "C" int rtx_apply(rtx_plan *p, uint32_t op_mask, int precision, const voi
     }
     RTX_CHECK(p->fused_mode != 2, "rtx_apply: the uniform-bin path was requested but the mesh "
                                   "is not a compiled uniform bin");
-    if ((p->fused_mode == -1 || p->fused_mode == 1) && fused_fits(p, precision))
-        return launch_fused(p, op_mask, precision, d_x, d_y, st, acc);
-    RTX_CHECK(p->fused_mode != 1, "rtx_apply: the fused per-column path was requested but a "
-                                  "column does not fit it (DOFs %d, cells %d, nodes %d)",
-              p->max_col_dofs, p->max_col_cells, p->max_nsp);
+    RTX_CHECK(p->fused_mode != 1 || p->n_gen_cols == 0,
+              "rtx_apply: the fused per-column path was requested but %d columns do not fit "
+              "it (largest column: DOFs %d, cells %d, nodes %d)",
+              p->n_gen_cols, p->max_col_dofs, p->max_col_cells, p->max_nsp);
+    if ((p->fused_mode == -1 || p->fused_mode == 1) && any_fused(p)) {
+        // order bins: fused bins, then the generic kernels on the rest
+        const bool tp = (op_mask & (RTX_OP_MASS | RTX_OP_ICONV | RTX_OP_BCONV | RTX_OP_QUADW)) != 0;
+        const bool sc = (op_mask & RTX_OP_SCAT) != 0;
+        if (!tp && !sc && !acc) {
+            if (precision == RTX_PREC_F64)
+                k_zero<double><<<1024, 256, 0, st>>>(static_cast<double *>(d_y), p->m.ndof);
+            else
+                k_zero<float><<<1024, 256, 0, st>>>(static_cast<float *>(d_y), p->m.ndof);
+            RTX_HIP(hipGetLastError());
+            return RTX_OK;
+        }
+        int rc = launch_fused(p, op_mask, precision, d_x, d_y, st, acc);
+        if (rc || p->n_gen_cols == 0) return rc;
+        if (precision == RTX_PREC_F64)
+            return launch_apply<double>(p, op_mask, d_x, d_y, st, p->gen_cells, p->n_gen_cells,
+                                        p->gen_cols, p->n_gen_cols);
+        return launch_apply<float>(p, op_mask, d_x, d_y, st, p->gen_cells, p->n_gen_cells,
+                                   p->gen_cols, p->n_gen_cols);
+    }
     switch (precision) {
         case RTX_PREC_F64: return launch_apply<double>(p, op_mask, d_x, d_y, st);
         case RTX_PREC_F32: return launch_apply<float>(p, op_mask, d_x, d_y, st);
@@ -323,4 +351,99 @@ extern "C" int rtx_mask(rtx_plan *p, int keep, int precision, void *d_x, void *s
         k_mask<float><<<p->m.ncell, 64, 0, st>>>(p->m, keep, static_cast<float *>(d_x));
     RTX_HIP(hipGetLastError());
     return RTX_OK;
+}
+
+namespace {
+
+// Mean duration (us) of `reps` launches of fn on the stream.
+template <typename F>
+float time_launches(F fn, int reps, hipStream_t st, int *rc) {
+    hipEvent_t e0, e1;
+    *rc = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        *rc = RTX_ERR_HIP;
+        return 0.f;
+    }
+    *rc = fn();                       // warm-up (also sets the LDS attribute)
+    (void)hipEventRecord(e0, st);
+    for (int i = 0; i < reps && *rc == 0; ++i) *rc = fn();
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 1000.f * ms / float(reps > 0 ? reps : 1);
+}
+
+}  // namespace
+
+/* Autotune the apply's launch shapes on this plan: for the uniform-bin
+   kernel and for every fused order bin, time each valid number of column
+   parts H (1, 2, 4) with `reps` launches of the interior operator on d_x ->
+   d_y (device vectors in `precision`) and keep the fastest.  Synchronises
+   the stream.  The choice holds for every precision. */
+extern "C" int rtx_autotune(rtx_plan *p, int precision, const void *d_x, void *d_y, int reps,
+                            void *stream) {
+    RTX_CHECK(p && d_x && d_y && d_x != d_y && reps >= 1, "rtx_autotune: bad argument");
+    RTX_CHECK(p->has_coeffs, "rtx_autotune: coefficients not set");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const uint32_t ops = RTX_OP_SYSTEM | RTX_OP_RESTRICT;
+    int rc = 0;
+    if (p->uni_ok) {
+        const int mask = uniform_valid_parts(p);
+        float best = 1e30f;
+        int bestH = p->uni_H;
+        for (int H : {1, 2, 4}) {
+            if (!(mask & H)) continue;
+            p->uni_H = H;
+            const float us = time_launches(
+                [&]() { return launch_uniform(p, ops, precision, d_x, d_y, st, 0); }, reps, st, &rc);
+            if (rc) return rc;
+            if (us < best) { best = us; bestH = H; }
+        }
+        p->uni_H = bestH;
+    }
+    for (int bi = 0; bi < 3; ++bi) {
+        FusedBin &fb = p->fbin[bi];
+        if (fb.n == 0) continue;
+        const int mask = fused_bin_valid_parts(p, bi, precision);
+        float best = 1e30f;
+        int bestH = fb.H;
+        for (int H : {1, 2, 4}) {
+            if (!(mask & H)) continue;
+            fb.H = H;
+            const float us = time_launches(
+                [&]() { return launch_fused(p, ops, precision, d_x, d_y, st, 0, bi); }, reps, st,
+                &rc);
+            if (rc) return rc;
+            if (us < best) { best = us; bestH = H; }
+        }
+        fb.H = bestH;
+    }
+    return RTX_OK;
+}
+
+/* Force the uniform kernel's parts per column (1, 2, 4; 0 = default). */
+extern "C" int rtx_set_uniform_parts(rtx_plan *p, int H) {
+    RTX_CHECK(p, "rtx_set_uniform_parts: null plan");
+    RTX_CHECK(H == 0 || (p->uni_ok && (H == 1 || H == 2 || H == 4) &&
+                         (uniform_valid_parts(p) & H)),
+              "rtx_set_uniform_parts: %d parts not valid for this plan", H);
+    p->uni_H = H;
+    return RTX_OK;
+}
+
+/* Launch-shape summary: out[0] uniform bin usable, [1] its parts per column,
+   [2 + 2 b], [3 + 2 b] parts and columns of fused bin b (b < 3), [8], [9]
+   columns and cells on the generic kernels.  Returns the count (10). */
+extern "C" int rtx_plan_info(const rtx_plan *p, int64_t *out, int n) {
+    if (!p) return 0;
+    int64_t v[10] = {p->uni_ok ? 1 : 0, p->uni_ok ? uniform_default_parts(p) : 0, 0, 0, 0, 0, 0, 0, p->n_gen_cols, p->n_gen_cells};
+    for (int b = 0; b < 3; ++b) {
+        v[2 + 2 * b] = p->fbin[b].H;
+        v[3 + 2 * b] = p->fbin[b].n;
+    }
+    for (int i = 0; i < 10 && i < n && out; ++i) out[i] = v[i];
+    return 10;
 }

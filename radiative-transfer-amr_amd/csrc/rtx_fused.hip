@@ -28,6 +28,7 @@
 //              F32   -> v_mfma_f32_16x16x4f32   (x and G in FP32)
 //              MIXED -> v_mfma_f32_16x16x16bf16 (x and G in BF16, FP32 accum.)
 //   phase 5  LDS -> y for the part's DOFs (mask, optional accumulate)
+#include <algorithm>
 #include <type_traits>
 
 #include "rtx_internal.h"
@@ -71,7 +72,8 @@ __device__ inline int f_find(const int *starts, int lo, int hi, int t) {
 // T: storage type of x and y.  PREC: RTX_PREC_*.  H: parts per column.
 template <typename T, typename A, int PREC>
 __global__ __launch_bounds__(FTHREADS) void k_fused(DevMesh m, uint32_t ops, int accumulate,
-                                                   int H, const T *__restrict__ x,
+                                                   int H, const int32_t *__restrict__ cols,
+                                                   int ncols, const T *__restrict__ x,
                                                    T *__restrict__ y) {
     extern __shared__ __align__(16) unsigned char f_lds[];
     __shared__ int cst[FMAX_CELLS + 1];      // column-local cell starts
@@ -85,11 +87,13 @@ __global__ __launch_bounds__(FTHREADS) void k_fused(DevMesh m, uint32_t ops, int
     __shared__ int lm0[FMAX_LINKS], lm1[FMAX_LINKS], l_tr[FMAX_LINKS + 1], l_it[FMAX_LINKS + 1];
     __shared__ int l_es[FMAX_LINKS], l_et[FMAX_LINKS];
 
-    // XCD-aware block -> (column, part): the H parts of a column share b % 8
+    // XCD-aware block -> (column of the bin, part): the H parts of a column
+    // share b % 8
     const int b = blockIdx.x;
     const int grp = b / (FXCD * H), rem = b % (FXCD * H);
-    const int h = rem / FXCD, c = grp * FXCD + rem % FXCD;
-    if (c >= m.ncol) return;
+    const int h = rem / FXCD, ci = grp * FXCD + rem % FXCD;
+    if (ci >= ncols) return;
+    const int c = cols[ci];
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
     const int k0 = m.col_cell_start[c], ncells = m.col_cell_start[c + 1] - k0;
@@ -516,46 +520,81 @@ __global__ __launch_bounds__(FTHREADS) void k_fused(DevMesh m, uint32_t ops, int
 
 // Dynamic LDS of one part: whole-column input, per-angle tables, the part's
 // accumulators and traces.
+size_t fused_bytes(int64_t col_dofs, int64_t nang, int64_t part_dofs, int64_t part_trace,
+                   size_t a) {
+    const size_t ints_as_a = (size_t(nang) + a / sizeof(int) - 1) / (a / sizeof(int));
+    return (size_t(col_dofs) + 3 * size_t(nang) + ints_as_a + size_t(part_dofs) +
+            size_t(part_trace)) * a;
+}
+
+constexpr size_t FUSED_CAP = 96 * 1024;   // dynamic LDS cap of the fused path
+
+int fused_column_ok(int ncells, int nsp, int nang, int part_links, size_t bytes64) {
+    return ncells <= FMAX_CELLS && nsp <= FMAX_NSP && part_links <= FMAX_LINKS &&
+           nang <= 1024 && bytes64 <= FUSED_CAP;
+}
+
+static int h_index(int H) { return H == 1 ? 0 : (H == 2 ? 1 : 2); }
+
 size_t fused_lds_bytes(const rtx_plan *p, int H, int prec) {
-    const size_t a = (prec == RTX_PREC_F64) ? sizeof(double) : sizeof(float);
-    const size_t na = size_t(p->max_nang);
-    const size_t ints_as_a = (na + a / sizeof(int) - 1) / (a / sizeof(int));
-    return (size_t(p->max_col_dofs) + 3 * na + ints_as_a + size_t(p->max_part_dofs[H]) +
-            size_t(p->max_part_trace[H])) * a;
+    // largest part over the bins (for reporting; launches size each bin)
+    size_t mx = 0;
+    for (const FusedBin &fb : p->fbin)
+        if (fb.n > 0) mx = std::max(mx, fb.bytes[prec == RTX_PREC_F64 ? 0 : 1][h_index(H)]);
+    return mx;
 }
 
 int fused_parts(const rtx_plan *p, int prec) {
-    // smallest split that keeps a part within 32 KB of dynamic LDS (several
-    // workgroups per CU), limited by the cells of the narrowest column
-    for (int H : {1, 2, 4})
-        if (H <= p->min_col_cells && fused_lds_bytes(p, H, prec) <= 32 * 1024) return H;
-    return p->min_col_cells >= 4 ? 4 : (p->min_col_cells >= 2 ? 2 : 1);
+    (void)prec;
+    int H = 1;
+    for (const FusedBin &fb : p->fbin)
+        if (fb.n > 0) H = std::max(H, fb.H);
+    return H;
+}
+
+int fused_bin_valid_parts(const rtx_plan *p, int bin, int prec) {
+    const FusedBin &fb = p->fbin[bin];
+    int mask = 0;
+    const int Hs[3] = {1, 2, 4};
+    for (int i = 0; i < 3; ++i)
+        if (fb.n > 0 && Hs[i] <= fb.min_cells && fb.part_links[i] <= FMAX_LINKS &&
+            fb.bytes[prec == RTX_PREC_F64 ? 0 : 1][i] <= FUSED_CAP)
+            mask |= Hs[i];
+    return mask;
+}
+
+template <typename T, typename A, int PREC>
+static int launch_bin(const rtx_plan *p, const FusedBin &fb, uint32_t ops, const void *dx, void *dy,
+                      hipStream_t st, int accumulate) {
+    const int H = fb.H, hi = h_index(H);
+    RTX_CHECK(fb.part_links[hi] <= FMAX_LINKS, "fused apply: %d links in a column part",
+              fb.part_links[hi]);
+    // each column carves its own part from the front of the dynamic LDS
+    const size_t lds = fb.bytes[sizeof(A) == sizeof(double) ? 0 : 1][hi];
+    RTX_CHECK(lds <= FUSED_CAP, "fused apply: %zu bytes of LDS per part", lds);
+    const int nblocks = ((fb.n + FXCD - 1) / FXCD) * FXCD * H;
+    RTX_HIP(hipFuncSetAttribute((const void *)k_fused<T, A, PREC>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    k_fused<T, A, PREC><<<nblocks, FTHREADS, lds, st>>>(p->m, ops, accumulate, H, fb.cols, fb.n,
+                                                        (const T *)dx, (T *)dy);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
 }
 
 int launch_fused(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
-                 int accumulate) {
-    const int H = fused_parts(p, prec);
-    RTX_CHECK(p->max_part_links[H] <= FMAX_LINKS, "fused apply: %d links in a column part",
-              p->max_part_links[H]);
-    const size_t lds = fused_lds_bytes(p, H, prec);
-    const int nblocks = ((p->m.ncol + FXCD - 1) / FXCD) * FXCD * H;
-    if (prec == RTX_PREC_F64) {
-        RTX_HIP(hipFuncSetAttribute((const void *)k_fused<double, double, RTX_PREC_F64>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-        k_fused<double, double, RTX_PREC_F64><<<nblocks, FTHREADS, lds, st>>>(
-            p->m, ops, accumulate, H, (const double *)dx, (double *)dy);
-    } else if (prec == RTX_PREC_F32) {
-        RTX_HIP(hipFuncSetAttribute((const void *)k_fused<float, float, RTX_PREC_F32>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-        k_fused<float, float, RTX_PREC_F32><<<nblocks, FTHREADS, lds, st>>>(
-            p->m, ops, accumulate, H, (const float *)dx, (float *)dy);
-    } else {
-        RTX_HIP(hipFuncSetAttribute((const void *)k_fused<float, float, RTX_PREC_MIXED>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-        k_fused<float, float, RTX_PREC_MIXED><<<nblocks, FTHREADS, lds, st>>>(
-            p->m, ops, accumulate, H, (const float *)dx, (float *)dy);
+                 int accumulate, int only_bin) {
+    for (int bi = 0; bi < 3; ++bi) {
+        const FusedBin &fb = p->fbin[bi];
+        if (fb.n == 0 || (only_bin >= 0 && bi != only_bin)) continue;
+        int rc;
+        if (prec == RTX_PREC_F64)
+            rc = launch_bin<double, double, RTX_PREC_F64>(p, fb, ops, dx, dy, st, accumulate);
+        else if (prec == RTX_PREC_F32)
+            rc = launch_bin<float, float, RTX_PREC_F32>(p, fb, ops, dx, dy, st, accumulate);
+        else
+            rc = launch_bin<float, float, RTX_PREC_MIXED>(p, fb, ops, dx, dy, st, accumulate);
+        if (rc) return rc;
     }
-    RTX_HIP(hipGetLastError());
     return RTX_OK;
 }
 

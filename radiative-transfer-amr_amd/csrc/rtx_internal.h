@@ -61,8 +61,29 @@ int fused_parts(const rtx_plan *p, int prec);
 int launch_uniform(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
                    int accumulate);
 bool uniform_shape_compiled(int N, int NTH, int NC);
+int uniform_valid_parts(const rtx_plan *p);   // bit H set: H parts valid (H = 1, 2, 4)
+int uniform_default_parts(const rtx_plan *p); // parts the uniform kernel launches with
+int fused_bin_valid_parts(const rtx_plan *p, int bin, int prec);
 int launch_fused(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
-                 int accumulate);
+                 int accumulate, int only_bin = -1);
+// Limits of the fused per-column kernel: 0 when a column with these shapes
+// cannot run on it, else 1 (bytes64: its dynamic LDS per part in FP64).
+int fused_column_ok(int ncells, int nsp, int nang, int part_links, size_t bytes64);
+size_t fused_bytes(int64_t col_dofs, int64_t nang, int64_t part_dofs, int64_t part_trace,
+                   size_t a);
+
+// One order bin of the fused kernel: columns that run with the same number
+// of parts H, and the bin's shape maxima for every candidate H (index 0, 1,
+// 2 for H = 1, 2, 4) so the part count can be re-chosen (autotuned).
+struct FusedBin {
+    int H = 0;                      // parts in use
+    int n = 0;                      // columns
+    const int32_t *cols = nullptr;  // device list of column indices
+    // largest per-column dynamic LDS of a part, [0]: FP64, [1]: FP32 storage
+    size_t bytes[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int part_links[3] = {0, 0, 0};
+    int min_cells = 0;
+};
 }  // namespace rtx
 
 struct rtx_plan {
@@ -80,11 +101,18 @@ struct rtx_plan {
     // uni_ncells cells per column, one shared angular mesh (G block)
     bool uni_ok = false;
     int32_t uni_n = 0, uni_nth = 0, uni_ncells = 0;
+    int uni_H = 0;                // parts per column of the uniform kernel (0: default)
     // largest own-DOF count and own-trace count of a column part when the
     // fused kernel splits every column into H = 1, 2, 4 cell ranges
     int32_t max_part_dofs[5] = {0, 0, 0, 0, 0}, max_part_trace[5] = {0, 0, 0, 0, 0};
     int32_t max_part_links[5] = {0, 0, 0, 0, 0};
     int fused_mode = -1;   // -1 auto, 0 generic kernels, 1 fused per-column kernel
+    // order bins of the apply (non-uniform meshes): fused-kernel bins by part
+    // count, and the columns (and their cells) left to the generic kernels
+    rtx::FusedBin fbin[3];
+    const int32_t *gen_cols = nullptr, *gen_cells = nullptr;
+    int n_gen_cols = 0, n_gen_cells = 0;
+    void *bins = nullptr;
     bool has_coeffs = false;
     // block-Jacobi
     int64_t *bj_off = nullptr;   // ncell + 1 offsets of the column-major inverses

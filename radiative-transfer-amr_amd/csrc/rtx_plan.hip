@@ -1,4 +1,5 @@
 // Plan creation: copy the flattened hp mesh and coefficient tables to HBM.
+#include <algorithm>
 #include <cstdarg>
 #include <cstddef>
 #include <cstring>
@@ -313,6 +314,83 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
             }
         }
     }
+    // Order bins: every column goes to the fused kernel with the smallest
+    // part count H whose part fits 32 KB of LDS (else the smallest within the
+    // kernel's cap), or to the generic kernels when no H fits.
+    {
+        std::vector<int32_t> lists[3], gcols, gcells;
+        const int Hs[3] = {1, 2, 4};
+        for (int c = 0; c < nc; ++c) {
+            const int k0 = d->col_cell_start[c], k1 = d->col_cell_start[c + 1], ncl = k1 - k0;
+            const int nsp = d->col_nx[c] * d->col_ny[c];
+            const int na = d->col_nang[c];
+            int64_t pd[3] = {0, 0, 0}, pt[3] = {0, 0, 0};
+            int pl[3] = {0, 0, 0};
+            for (int hi_ = 0; hi_ < 3; ++hi_) {
+                const int H = Hs[hi_];
+                for (int h = 0; h < H; ++h) {
+                    const int lo = k0 + (h * ncl) / H, hi = k0 + ((h + 1) * ncl) / H;
+                    int64_t dofs = 0, tr = 0;
+                    for (int k = lo; k < hi; ++k) dofs += int64_t(nsp) * d->cell_nth[k];
+                    const int l0 = d->link_start[lo], l1 = d->link_start[hi];
+                    for (int l = l0; l < l1; ++l) {
+                        const int s = d->link_src[l], F = d->link_face[l], c1 = d->cell_col[s];
+                        tr += ((F & 1) ? d->col_nx[c1] : d->col_ny[c1]) * d->cell_nth[s];
+                    }
+                    pd[hi_] = std::max(pd[hi_], dofs);
+                    pt[hi_] = std::max(pt[hi_], tr);
+                    pl[hi_] = std::max(pl[hi_], l1 - l0);
+                }
+            }
+            int pick = -1;
+            for (int pass = 0; pass < 2 && pick < 0; ++pass)
+                for (int hi_ = 0; hi_ < 3 && pick < 0; ++hi_) {
+                    if (Hs[hi_] > ncl) continue;
+                    const size_t b64 = fused_bytes(int64_t(nsp) * na, na, pd[hi_], pt[hi_], 8);
+                    if (!fused_column_ok(ncl, nsp, na, pl[hi_], b64)) continue;
+                    if (pass == 0 && b64 > 32 * 1024) continue;
+                    pick = hi_;
+                }
+            if (pick < 0) {
+                gcols.push_back(c);
+                for (int k = k0; k < k1; ++k) gcells.push_back(k);
+                continue;
+            }
+            FusedBin &fb = p->fbin[pick];
+            lists[pick].push_back(c);
+            for (int hi_ = 0; hi_ < 3; ++hi_) {
+                const int64_t cd = int64_t(nsp) * na;
+                fb.bytes[0][hi_] = std::max(fb.bytes[0][hi_], fused_bytes(cd, na, pd[hi_], pt[hi_], 8));
+                fb.bytes[1][hi_] = std::max(fb.bytes[1][hi_], fused_bytes(cd, na, pd[hi_], pt[hi_], 4));
+                fb.part_links[hi_] = std::max(fb.part_links[hi_], pl[hi_]);
+            }
+            fb.min_cells = fb.n == 0 ? ncl : std::min(fb.min_cells, ncl);
+            fb.n += 1;
+            fb.H = Hs[pick];
+        }
+        size_t total = 0;
+        for (int b = 0; b < 3; ++b) total += lists[b].size();
+        total += gcols.size() + gcells.size();
+        if (total > 0) {
+            RTX_HIP(hipMalloc(&p->bins, total * sizeof(int32_t)));
+            int32_t *base = static_cast<int32_t *>(p->bins);
+            size_t o = 0;
+            auto put = [&](const std::vector<int32_t> &v) -> const int32_t * {
+                if (v.empty()) return nullptr;
+                (void)hipMemcpy(base + o, v.data(), v.size() * sizeof(int32_t),
+                                hipMemcpyHostToDevice);
+                const int32_t *ptr = base + o;
+                o += v.size();
+                return ptr;
+            };
+            for (int b = 0; b < 3; ++b) p->fbin[b].cols = put(lists[b]);
+            p->gen_cols = put(gcols);
+            p->gen_cells = put(gcells);
+            p->n_gen_cols = int(gcols.size());
+            p->n_gen_cells = int(gcells.size());
+            RTX_HIP(hipGetLastError());
+        }
+    }
     // sizes of the column G blocks are checked in set_coeffs
     *out = p;
     return RTX_OK;
@@ -328,6 +406,7 @@ extern "C" void rtx_plan_destroy(rtx_plan *p) {
     if (p->bj_off) (void)hipFree(p->bj_off);
     if (p->bj_blk) (void)hipFree(p->bj_blk);
     if (p->coarse) (void)hipFree(p->coarse);
+    if (p->bins) (void)hipFree(p->bins);
     (void)hipSetDevice(cur);
     delete p;
 }

@@ -406,17 +406,44 @@ int launch_one(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t 
     return RTX_OK;
 }
 
+// A part count H is valid for a shape when it divides the cells and the
+// part's output angles fit one 16-wide tile per wavefront.
+template <int NTH, int NC, int H>
+constexpr bool u_valid() {
+    return NC % H == 0 && ((NC / H) * NTH + 15) / 16 <= UTHREADS / 64;
+}
+
+template <int N, int NTH, int NC, int H>
+int launch_h(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
+             int acc) {
+    if constexpr (u_valid<NTH, NC, H>()) {
+        if (prec == RTX_PREC_F64)
+            return launch_one<double, double, RTX_PREC_F64, N, NTH, NC, H>(p, ops, dx, dy, st, acc);
+        if (prec == RTX_PREC_F32)
+            return launch_one<float, float, RTX_PREC_F32, N, NTH, NC, H>(p, ops, dx, dy, st, acc);
+        return launch_one<float, float, RTX_PREC_MIXED, N, NTH, NC, H>(p, ops, dx, dy, st, acc);
+    } else {
+        set_error("uniform apply: %d parts are not valid for this shape", H);
+        return RTX_ERR_ARG;
+    }
+}
+
 template <int N, int NTH, int NC>
 int launch_shape(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
                  int acc) {
-    // Two parts per column: LDS near 30 KB in FP64 and at most four
-    // output-angle tiles (one per wavefront) for the compiled shapes.
-    constexpr int H = (NC % 2 == 0) ? 2 : 1;
-    if (prec == RTX_PREC_F64)
-        return launch_one<double, double, RTX_PREC_F64, N, NTH, NC, H>(p, ops, dx, dy, st, acc);
-    if (prec == RTX_PREC_F32)
-        return launch_one<float, float, RTX_PREC_F32, N, NTH, NC, H>(p, ops, dx, dy, st, acc);
-    return launch_one<float, float, RTX_PREC_MIXED, N, NTH, NC, H>(p, ops, dx, dy, st, acc);
+    // default: two parts per column (LDS near 30 KB in FP64) when valid;
+    // rtx_autotune may pick another valid part count
+    int H = p->uni_H;
+    if (H == 0) H = u_valid<NTH, NC, 2>() ? 2 : (u_valid<NTH, NC, 4>() ? 4 : 1);
+    if (H == 1) return launch_h<N, NTH, NC, 1>(p, ops, prec, dx, dy, st, acc);
+    if (H == 2) return launch_h<N, NTH, NC, 2>(p, ops, prec, dx, dy, st, acc);
+    return launch_h<N, NTH, NC, 4>(p, ops, prec, dx, dy, st, acc);
+}
+
+template <int N, int NTH, int NC>
+int valid_parts_mask() {
+    return (u_valid<NTH, NC, 1>() ? 1 : 0) | (u_valid<NTH, NC, 2>() ? 2 : 0) |
+           (u_valid<NTH, NC, 4>() ? 4 : 0);
 }
 
 }  // namespace
@@ -437,6 +464,27 @@ int launch_uniform(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy
     RTX_SHAPE(3, 3, 4)
 #undef RTX_SHAPE
     return 1;   // not compiled: caller falls back
+}
+
+int uniform_default_parts(const rtx_plan *p) {
+    if (p->uni_H) return p->uni_H;
+    const int mask = uniform_valid_parts(p);
+    return (mask & 2) ? 2 : ((mask & 4) ? 4 : 1);
+}
+
+int uniform_valid_parts(const rtx_plan *p) {
+    const int N = p->uni_n, NTH = p->uni_nth, NC = p->uni_ncells;
+#define RTX_SHAPE(a, b, c_) \
+    if (N == a && NTH == b && NC == c_) return valid_parts_mask<a, b, c_>();
+    RTX_SHAPE(5, 5, 16)
+    RTX_SHAPE(4, 4, 16)
+    RTX_SHAPE(4, 4, 4)
+    RTX_SHAPE(3, 3, 8)
+    RTX_SHAPE(5, 5, 8)
+    RTX_SHAPE(4, 4, 8)
+    RTX_SHAPE(3, 3, 4)
+#undef RTX_SHAPE
+    return 0;
 }
 
 bool uniform_shape_compiled(int N, int NTH, int NC) {

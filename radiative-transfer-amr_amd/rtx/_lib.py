@@ -57,6 +57,10 @@ SIGNATURES = [
     ("rtx_apply", C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rtx_mask", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     ("rtx_set_path", C.c_int, [C.c_void_p, C.c_int]),
+    ("rtx_autotune", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                               C.c_void_p]),
+    ("rtx_plan_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int]),
+    ("rtx_set_uniform_parts", C.c_int, [C.c_void_p, C.c_int]),
     ("rtx_assemble_entries", C.c_int, [C.c_void_p, C.c_int, C.c_int64] + [C.c_void_p] * 7),
     ("rtx_bjacobi_size", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("rtx_bjacobi_factor", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -130,6 +130,33 @@ class TransportOperator:
         """-1 automatic, 0 generic per-cell/per-column kernels, 1 fused per-column kernel."""
         check(self.lib.rtx_set_path(self.plan, int(mode)), "rtx_set_path")
 
+    def autotune(self, precision=PREC_F64, reps=10, stream=None):
+        """Time every valid number of column parts (1, 2, 4) of the uniform
+        kernel and of each fused order bin on a scratch vector pair and keep
+        the fastest; returns plan_info() afterwards."""
+        dtype = torch.float64 if precision == PREC_F64 else torch.float32
+        x = torch.rand(self.ndof, dtype=dtype, device=self.device)
+        y = torch.empty_like(x)
+        check(self.lib.rtx_autotune(self.plan, int(precision), C.c_void_p(x.data_ptr()),
+                                    C.c_void_p(y.data_ptr()), int(reps), stream_handle(stream)),
+              "rtx_autotune")
+        return self.plan_info()
+
+    def plan_info(self):
+        """Launch shapes of the apply: uniform bin (usable, parts per column),
+        the three fused order bins (parts, columns) and the column and cell
+        counts left to the generic kernels."""
+        out = (C.c_int64 * 10)()
+        self.lib.rtx_plan_info(self.plan, out, 10)
+        v = list(out)
+        return {"uniform": bool(v[0]), "uniform_parts": v[1],
+                "fused_bins": [(v[2 + 2 * b], v[3 + 2 * b]) for b in range(3) if v[3 + 2 * b]],
+                "generic_cols": v[8], "generic_cells": v[9]}
+
+    def set_uniform_parts(self, H):
+        """Force the uniform kernel's parts per column (0: default)."""
+        check(self.lib.rtx_set_uniform_parts(self.plan, int(H)), "rtx_set_uniform_parts")
+
     def mask(self, x, keep_interior=True, stream=None):
         """Zero the inflow-boundary DOFs (keep_interior) or the interior ones."""
         prec = PREC_F64 if x.dtype == torch.float64 else PREC_F32

@@ -205,3 +205,51 @@ def test_apply_c3_rows_and_linearity(cuda):
     lhs = op.apply(2.0 * x1 - 3.0 * x2, ops=OP_SYSTEM)
     rhs = 2.0 * op.apply(x1, ops=OP_SYSTEM) - 3.0 * op.apply(x2, ops=OP_SYSTEM)
     assert float((lhs - rhs).norm() / rhs.norm()) < 1e-13
+
+
+@pytest.mark.gpu
+def test_uniform_parts_and_autotune(cuda):
+    """Every valid part count of the uniform kernel, and the shapes the
+    autotuner picks for the uniform kernel and the fused order bins, give the
+    oracle's operator."""
+    from rtx.operator import TransportOperator, OP_SYSTEM, OP_RESTRICT
+    from rtx import RtxError
+    mesh = problems.uniform_mesh([1.0, 1.0], [4, 4, 4], 2, 2)
+    wl = problems.manufactured()
+    V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi)
+    A = (V["BC"] - V["IC"] + V["M"] + V["S"]).tocsr()
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    assert op.plan_info()["uniform"]
+    x = np.random.default_rng(3).uniform(-1.0, 1.0, op.ndof)
+    xd = _dev(x)
+    ran = []
+    for H in (1, 2, 4):
+        try:
+            op.set_uniform_parts(H)
+        except RtxError:
+            continue
+        print("uniform parts", H, flush=True)
+        assert op.plan_info()["uniform_parts"] == H
+        y = op.apply(xd, ops=OP_SYSTEM).cpu().numpy()
+        assert _rel(y, A @ x) < TOL64, (H, _rel(y, A @ x))
+        ran.append(H)
+    assert len(ran) >= 2
+    op.set_uniform_parts(0)
+    info = op.autotune(reps=3)
+    assert info["uniform_parts"] in ran
+    y = op.apply(xd, ops=OP_SYSTEM).cpu().numpy()
+    assert _rel(y, A @ x) < TOL64
+
+    mesh = small_random_hp(1)
+    wl = problems.two_scatterers_c3()
+    V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi)
+    A = (V["BC"] - V["IC"] + V["M"] + V["S"]).tocsr()
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    before = op.plan_info()
+    info = op.autotune(reps=3)
+    print("hp bins", before, "->", info, flush=True)
+    assert [n for _, n in info["fused_bins"]] == [n for _, n in before["fused_bins"]]
+    x = np.random.default_rng(4).uniform(-1.0, 1.0, op.ndof)
+    mask = asm.intr_mask(mesh)
+    y = op.apply(_dev(x), ops=OP_SYSTEM | OP_RESTRICT).cpu().numpy()
+    assert _rel(y, mask * (A @ (mask * x))) < TOL64
