@@ -162,9 +162,9 @@ int rtx_autotune(rtx_plan *plan, int precision, const void *d_x, void *d_y, int 
    the plan's shape; 0 restores the default).  No counterpart in the
    reference. */
 int rtx_set_uniform_parts(rtx_plan *plan, int parts);
-/* Launch-shape summary (10 values): uniform bin usable, its parts per column;
-   parts and columns of the three fused order bins; columns and cells left to
-   the generic kernels.  Returns the count. */
+/* Launch-shape summary (16 values): uniform bin usable, its parts per column;
+   parts and columns of the six fused order bins (part count by LDS class);
+   columns and cells left to the generic kernels.  Returns the count. */
 int rtx_plan_info(const rtx_plan *plan, int64_t *out, int n);
 
 /* In-place mask of a full-length vector: keep = 0 zeroes the inflow-boundary

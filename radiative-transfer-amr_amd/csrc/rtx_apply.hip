@@ -4,10 +4,10 @@
 //                 convection (-IC), outflow face terms and upwind inflow
 //                 couplings through the link table (+BC).  The cell's nodal
 //                 values are staged in LDS.
-//   k_scatter   : one workgroup per spatial column.  Adds the angular
-//                 scattering contraction (-S) per spatial node with the
-//                 column's G^T table, the angular values of a tile of
-//                 spatial nodes staged in LDS.
+//   k_scatter_mma : one workgroup per (column, 16 spatial nodes, 128
+//                 output angles) work item.  Adds the angular scattering
+//                 contraction (-S) as a matrix-core GEMM against the
+//                 column's G^T table.
 //
 // Formulas (matrix entries of the reference restated as actions):
 //   M      : src/rt/mass_matrix.py:140-173
@@ -20,7 +20,7 @@ namespace rtx {
 
 constexpr int NM = RTX_NMAX;
 constexpr int SCAT_THREADS = 256;
-constexpr int SCAT_LDS_DOUBLES = 5120;   // 40 KiB of staged angular values
+constexpr int SC_KC = 32;                // angles per K step of the tiled scattering
 constexpr int MAX_NANG = 1024;           // angular nodes per column
 
 __device__ inline bool on_bdry(int bd, int i, int j, int nx, int ny) {
@@ -160,18 +160,51 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
     }
 }
 
+template <typename T> struct ScatMfma;
+template <> struct ScatMfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    __device__ static v4 step(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // row (spatial node of the tile) of accumulator v of this lane
+    __device__ static int row(int lg, int v) { return lg + 4 * v; }
+    __device__ static const double *G(const DevMesh &m) { return m.G; }
+};
+template <> struct ScatMfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    __device__ static v4 step(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static int row(int lg, int v) { return 4 * lg + v; }
+    __device__ static const float *G(const DevMesh &m) { return m.G_f; }
+};
+
+// Scattering of one work item: 16 spatial nodes (s) x SC_RG output angles
+// (R) of a column, Y[s][R] = sum_A X[s][A] G^T[A][R] on the matrix cores
+// (FP64 MFMA for double, FP32 MFMA for float), then scaled by
+// -dx dy w_i w_j sigma_s (the angular weights and constant factors of the
+// reference's entry are folded into G^T).  The X tile (16 x SC_KC angles, inflow-masked) is gathered into
+// LDS per K step; every wave owns SC_RG / 64 output tiles of 16 angles and
+// reads its G^T operands straight from L2.  Each (s, R) has one owner, so
+// accumulation into y is a plain read-modify-write.
 template <typename T>
-__global__ __launch_bounds__(SCAT_THREADS) void k_scatter(DevMesh m, uint32_t ops, int accumulate,
-                                                          const int32_t *__restrict__ cols,
-                                                          const T *__restrict__ x,
-                                                          T *__restrict__ y) {
-    __shared__ double us[SCAT_LDS_DOUBLES];
-    __shared__ int a_off[MAX_NANG];    // first DOF of the cell owning angle A
-    __shared__ int a_nt[MAX_NANG];     // that cell's nth
-    __shared__ int a_loc[MAX_NANG];    // local angular index within the cell
-    __shared__ int a_bd[MAX_NANG];     // that cell's masked faces
-    const int c = cols ? cols[blockIdx.x] : blockIdx.x;
-    const int tid = threadIdx.x;
+__global__ __launch_bounds__(SCAT_THREADS) void k_scatter_mma(DevMesh m, uint32_t ops,
+                                                              int accumulate,
+                                                              const int32_t *__restrict__ items,
+                                                              const T *__restrict__ x,
+                                                              T *__restrict__ y) {
+    typedef ScatMfma<T> F;
+    typedef typename F::v4 v4;
+    constexpr int RT = SC_RG / 64;             // output tiles per wave
+    __shared__ int a_dof[MAX_NANG];            // first DOF of the cell owning angle A
+    __shared__ int a_meta[MAX_NANG];           // nth | local index << 8 | masked faces << 16
+    __shared__ T xs[16][SC_KC + 1];
+    const int c = items[2 * blockIdx.x];
+    if (c < 0) return;                         // padding of the XCD interleave
+    const int word = items[2 * blockIdx.x + 1];
+    const int s0 = (word & 0xffff) * 16, rbase = (word >> 16) * SC_RG;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lg = lane >> 4;
     const int nx = m.col_nx[c], ny = m.col_ny[c], nsp = nx * ny;
     const int na = m.col_nang[c];
     const int k0 = m.col_cell_start[c], k1 = m.col_cell_start[c + 1];
@@ -180,43 +213,62 @@ __global__ __launch_bounds__(SCAT_THREADS) void k_scatter(DevMesh m, uint32_t op
         const int nt = m.cell_nth[k], ao = m.cell_ang_off[k], dof = m.cell_dof_off[k];
         const int bd = restrict_ ? m.cell_bdry[k] : 0;
         for (int a = 0; a < nt; ++a) {
-            a_off[ao + a] = dof;
-            a_nt[ao + a] = nt;
-            a_loc[ao + a] = a;
-            a_bd[ao + a] = bd;
+            a_dof[ao + a] = dof;
+            a_meta[ao + a] = nt | (a << 8) | (bd << 16);
         }
+    }
+    const T *G = F::G(m) + m.col_G_off[c];
+    v4 acc[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) acc[t] = v4{0, 0, 0, 0};
+    __syncthreads();
+    for (int A0 = 0; A0 < na; A0 += SC_KC) {
+        for (int t = tid; t < 16 * SC_KC; t += SCAT_THREADS) {
+            const int sl = t / SC_KC, al = t % SC_KC, s = s0 + sl, A = A0 + al;
+            T v = T(0);
+            if (s < nsp && A < na) {
+                const int meta = a_meta[A], nt = meta & 255, loc = (meta >> 8) & 255;
+                const int bd = meta >> 16;
+                v = x[a_dof[A] + s * nt + loc];
+                if (bd && on_bdry(bd, s / ny, s % ny, nx, ny)) v = T(0);
+            }
+            xs[sl][al] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            const int R = rbase + (wave * RT + t) * 16 + l16;
+            if (rbase + (wave * RT + t) * 16 >= na) continue;   // uniform per wave
+#pragma unroll
+            for (int kk = 0; kk < SC_KC; kk += 4) {
+                const int Ak = A0 + kk + lg;
+                const T b = (Ak < na && R < na) ? G[size_t(Ak) * na + R] : T(0);
+                acc[t] = F::step(xs[l16][kk + lg], b, acc[t]);
+            }
+        }
+        __syncthreads();
     }
     const double x0 = m.col_geom[4 * c + 0], y0 = m.col_geom[4 * c + 1];
     const double dxdy = (m.col_geom[4 * c + 2] - x0) * (m.col_geom[4 * c + 3] - y0);
     const double *wx = m.lgl_w + nx * NM, *wy = m.lgl_w + ny * NM;
-    const double *G = m.G + m.col_G_off[c];
     const double *sig = m.sigma + m.col_sp_off[c];
-    const int ts = SCAT_LDS_DOUBLES / na < nsp ? SCAT_LDS_DOUBLES / na : nsp;
-    __syncthreads();
-    for (int s0 = 0; s0 < nsp; s0 += ts) {
-        const int sn = (nsp - s0) < ts ? (nsp - s0) : ts;
-        __syncthreads();
-        for (int t = tid; t < sn * na; t += SCAT_THREADS) {
-            const int sl = t / na, A = t % na, s = s0 + sl;
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        const int R = rbase + (wave * RT + t) * 16 + l16;
+        if (R >= na) continue;
+        const int meta = a_meta[R], nt = meta & 255, loc = (meta >> 8) & 255, bd = meta >> 16;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int s = s0 + F::row(lg, v);
+            if (s >= nsp) continue;
             const int i = s / ny, j = s % ny;
-            double v = double(x[a_off[A] + s * a_nt[A] + a_loc[A]]);
-            if (a_bd[A] && on_bdry(a_bd[A], i, j, nx, ny)) v = 0.0;
-            us[t] = v;
-        }
-        __syncthreads();
-        for (int t = tid; t < sn * na; t += SCAT_THREADS) {
-            const int sl = t / na, R = t % na, s = s0 + sl;
-            const int i = s / ny, j = s % ny;
-            const double *u = us + sl * na;
-            double acc = 0.0;
-            for (int A = 0; A < na; ++A) acc += G[size_t(A) * na + R] * u[A];
-            const double v = -dxdy * wx[i] * wy[j] * sig[s] * acc;
-            const int addr = a_off[R] + s * a_nt[R] + a_loc[R];
-            const bool masked = restrict_ && a_bd[R] && on_bdry(a_bd[R], i, j, nx, ny);
+            const double val = -dxdy * wx[i] * wy[j] * sig[s] * double(acc[t][v]);
+            const int addr = a_dof[R] + s * nt + loc;
+            const bool masked = bd && on_bdry(bd, i, j, nx, ny);
             if (accumulate) {
-                if (!masked) y[addr] = T(double(y[addr]) + v);
+                if (!masked) y[addr] = T(double(y[addr]) + val);
             } else {
-                y[addr] = T(masked ? 0.0 : v);
+                y[addr] = T(masked ? 0.0 : val);
             }
         }
     }
@@ -248,14 +300,17 @@ __global__ __launch_bounds__(64) void k_mask(DevMesh m, int keep, T *__restrict_
 template <typename T>
 static int launch_apply(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t st,
                         const int32_t *cells = nullptr, int ncells = -1,
-                        const int32_t *cols = nullptr, int ncols = -1) {
+                        const int32_t *items = nullptr, int nitems = -1) {
     const T *x = static_cast<const T *>(dx);
     T *y = static_cast<T *>(dy);
     const bool acc = (ops & RTX_OP_ACCUM) != 0;
     const bool tp = (ops & (RTX_OP_MASS | RTX_OP_ICONV | RTX_OP_BCONV | RTX_OP_QUADW)) != 0;
     const bool sc = (ops & RTX_OP_SCAT) != 0;
     if (ncells < 0) ncells = p->m.ncell;
-    if (ncols < 0) ncols = p->m.ncol;
+    if (!items) {
+        items = p->sc_all;
+        nitems = p->n_sc_all;
+    }
     if (!tp && !sc) {
         if (!acc) k_zero<T><<<1024, 256, 0, st>>>(y, p->m.ndof);
         RTX_HIP(hipGetLastError());
@@ -263,8 +318,9 @@ static int launch_apply(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hip
     }
     if (tp && ncells > 0)
         k_transport<T><<<ncells, 64, 0, st>>>(p->m, ops, acc ? 1 : 0, cells, x, y);
-    if (sc && ncols > 0)
-        k_scatter<T><<<ncols, SCAT_THREADS, 0, st>>>(p->m, ops, (acc || tp) ? 1 : 0, cols, x, y);
+    if (sc && nitems > 0)
+        k_scatter_mma<T><<<nitems, SCAT_THREADS, 0, st>>>(p->m, ops, (acc || tp) ? 1 : 0, items,
+                                                          x, y);
     RTX_HIP(hipGetLastError());
     return RTX_OK;
 }
@@ -330,9 +386,9 @@ extern "C" int rtx_apply(rtx_plan *p, uint32_t op_mask, int precision, const voi
         if (rc || p->n_gen_cols == 0) return rc;
         if (precision == RTX_PREC_F64)
             return launch_apply<double>(p, op_mask, d_x, d_y, st, p->gen_cells, p->n_gen_cells,
-                                        p->gen_cols, p->n_gen_cols);
+                                        p->sc_gen, p->n_sc_gen);
         return launch_apply<float>(p, op_mask, d_x, d_y, st, p->gen_cells, p->n_gen_cells,
-                                   p->gen_cols, p->n_gen_cols);
+                                   p->sc_gen, p->n_sc_gen);
     }
     switch (precision) {
         case RTX_PREC_F64: return launch_apply<double>(p, op_mask, d_x, d_y, st);
@@ -404,7 +460,7 @@ extern "C" int rtx_autotune(rtx_plan *p, int precision, const void *d_x, void *d
         }
         p->uni_H = bestH;
     }
-    for (int bi = 0; bi < 3; ++bi) {
+    for (int bi = 0; bi < RTX_NBINS; ++bi) {
         FusedBin &fb = p->fbin[bi];
         if (fb.n == 0) continue;
         const int mask = fused_bin_valid_parts(p, bi, precision);
@@ -435,15 +491,20 @@ extern "C" int rtx_set_uniform_parts(rtx_plan *p, int H) {
 }
 
 /* Launch-shape summary: out[0] uniform bin usable, [1] its parts per column,
-   [2 + 2 b], [3 + 2 b] parts and columns of fused bin b (b < 3), [8], [9]
-   columns and cells on the generic kernels.  Returns the count (10). */
+   [2 + 2 b], [3 + 2 b] parts and columns of fused bin b (b < 6), [14], [15]
+   columns and cells on the generic kernels.  Returns the count (16). */
 extern "C" int rtx_plan_info(const rtx_plan *p, int64_t *out, int n) {
+    constexpr int NV = 4 + 2 * RTX_NBINS;
     if (!p) return 0;
-    int64_t v[10] = {p->uni_ok ? 1 : 0, p->uni_ok ? uniform_default_parts(p) : 0, 0, 0, 0, 0, 0, 0, p->n_gen_cols, p->n_gen_cells};
-    for (int b = 0; b < 3; ++b) {
+    int64_t v[NV] = {};
+    v[0] = p->uni_ok ? 1 : 0;
+    v[1] = p->uni_ok ? uniform_default_parts(p) : 0;
+    for (int b = 0; b < RTX_NBINS; ++b) {
         v[2 + 2 * b] = p->fbin[b].H;
         v[3 + 2 * b] = p->fbin[b].n;
     }
-    for (int i = 0; i < 10 && i < n && out; ++i) out[i] = v[i];
-    return 10;
+    v[NV - 2] = p->n_gen_cols;
+    v[NV - 1] = p->n_gen_cells;
+    for (int i = 0; i < NV && i < n && out; ++i) out[i] = v[i];
+    return NV;
 }

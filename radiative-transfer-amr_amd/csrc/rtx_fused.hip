@@ -583,7 +583,7 @@ static int launch_bin(const rtx_plan *p, const FusedBin &fb, uint32_t ops, const
 
 int launch_fused(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
                  int accumulate, int only_bin) {
-    for (int bi = 0; bi < 3; ++bi) {
+    for (int bi = 0; bi < RTX_NBINS; ++bi) {
         const FusedBin &fb = p->fbin[bi];
         if (fb.n == 0 || (only_bin >= 0 && bi != only_bin)) continue;
         int rc;

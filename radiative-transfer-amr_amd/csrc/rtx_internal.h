@@ -72,6 +72,11 @@ int fused_column_ok(int ncells, int nsp, int nang, int part_links, size_t bytes6
 size_t fused_bytes(int64_t col_dofs, int64_t nang, int64_t part_dofs, int64_t part_trace,
                    size_t a);
 
+constexpr int RTX_NBINS = 6;
+// Tiled scattering work items: 16 spatial nodes x SC_RG output angles
+constexpr int SC_RG = 128;
+constexpr int SC_XCD = 8;
+
 // One order bin of the fused kernel: columns that run with the same number
 // of parts H, and the bin's shape maxima for every candidate H (index 0, 1,
 // 2 for H = 1, 2, 4) so the part count can be re-chosen (autotuned).
@@ -109,9 +114,15 @@ struct rtx_plan {
     int fused_mode = -1;   // -1 auto, 0 generic kernels, 1 fused per-column kernel
     // order bins of the apply (non-uniform meshes): fused-kernel bins by part
     // count, and the columns (and their cells) left to the generic kernels
-    rtx::FusedBin fbin[3];
+    // bins b = 3 * cls + h: cls 0 holds columns whose part fits 32 KB of LDS
+    // at some H, cls 1 the larger ones; h the part count index (H = 1, 2, 4)
+    rtx::FusedBin fbin[rtx::RTX_NBINS];
     const int32_t *gen_cols = nullptr, *gen_cells = nullptr;
     int n_gen_cols = 0, n_gen_cells = 0;
+    // work items of the tiled scattering kernel (pairs: column, tile word),
+    // over every column and over the generic columns, XCD-interleaved
+    const int32_t *sc_all = nullptr, *sc_gen = nullptr;
+    int n_sc_all = 0, n_sc_gen = 0;
     void *bins = nullptr;
     bool has_coeffs = false;
     // block-Jacobi

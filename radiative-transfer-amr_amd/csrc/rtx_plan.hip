@@ -318,7 +318,7 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
     // part count H whose part fits 32 KB of LDS (else the smallest within the
     // kernel's cap), or to the generic kernels when no H fits.
     {
-        std::vector<int32_t> lists[3], gcols, gcells;
+        std::vector<int32_t> lists[RTX_NBINS], gcols, gcells;
         const int Hs[3] = {1, 2, 4};
         for (int c = 0; c < nc; ++c) {
             const int k0 = d->col_cell_start[c], k1 = d->col_cell_start[c + 1], ncl = k1 - k0;
@@ -342,7 +342,7 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
                     pl[hi_] = std::max(pl[hi_], l1 - l0);
                 }
             }
-            int pick = -1;
+            int pick = -1, cls = 0;
             for (int pass = 0; pass < 2 && pick < 0; ++pass)
                 for (int hi_ = 0; hi_ < 3 && pick < 0; ++hi_) {
                     if (Hs[hi_] > ncl) continue;
@@ -350,14 +350,15 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
                     if (!fused_column_ok(ncl, nsp, na, pl[hi_], b64)) continue;
                     if (pass == 0 && b64 > 32 * 1024) continue;
                     pick = hi_;
+                    cls = pass;
                 }
             if (pick < 0) {
                 gcols.push_back(c);
                 for (int k = k0; k < k1; ++k) gcells.push_back(k);
                 continue;
             }
-            FusedBin &fb = p->fbin[pick];
-            lists[pick].push_back(c);
+            FusedBin &fb = p->fbin[3 * cls + pick];
+            lists[3 * cls + pick].push_back(c);
             for (int hi_ = 0; hi_ < 3; ++hi_) {
                 const int64_t cd = int64_t(nsp) * na;
                 fb.bytes[0][hi_] = std::max(fb.bytes[0][hi_], fused_bytes(cd, na, pd[hi_], pt[hi_], 8));
@@ -368,9 +369,43 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
             fb.n += 1;
             fb.H = Hs[pick];
         }
+        // tiled scattering items: columns go to the XCD with the least work
+        // so far (work: MFMA tiles x angles), each XCD's queue in column
+        // order, then interleaved so block b runs on XCD b % 8
+        auto scatter_items = [&](const std::vector<int32_t> &cols) {
+            std::vector<std::vector<int32_t>> q(SC_XCD);
+            int64_t load[SC_XCD] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int c : cols) {
+                const int nsp = d->col_nx[c] * d->col_ny[c], na = d->col_nang[c];
+                const int nst = (nsp + 15) / 16, nrg = (na + SC_RG - 1) / SC_RG;
+                int x = 0;
+                for (int i = 1; i < SC_XCD; ++i)
+                    if (load[i] < load[x]) x = i;
+                for (int st = 0; st < nst; ++st)
+                    for (int rg = 0; rg < nrg; ++rg) {
+                        q[x].push_back(c);
+                        q[x].push_back(st | (rg << 16));
+                    }
+                load[x] += int64_t(nst) * nrg * na;
+            }
+            size_t len = 0;
+            for (auto &v : q) len = std::max(len, v.size() / 2);
+            std::vector<int32_t> out;
+            out.reserve(len * SC_XCD * 2);
+            for (size_t i = 0; i < len; ++i)
+                for (int x = 0; x < SC_XCD; ++x) {
+                    const bool has = 2 * i < q[x].size();
+                    out.push_back(has ? q[x][2 * i] : -1);
+                    out.push_back(has ? q[x][2 * i + 1] : 0);
+                }
+            return out;
+        };
+        std::vector<int32_t> all_cols(nc);
+        for (int c = 0; c < nc; ++c) all_cols[c] = c;
+        const std::vector<int32_t> sc_all = scatter_items(all_cols), sc_gen = scatter_items(gcols);
         size_t total = 0;
-        for (int b = 0; b < 3; ++b) total += lists[b].size();
-        total += gcols.size() + gcells.size();
+        for (int b = 0; b < RTX_NBINS; ++b) total += lists[b].size();
+        total += gcols.size() + gcells.size() + sc_all.size() + sc_gen.size();
         if (total > 0) {
             RTX_HIP(hipMalloc(&p->bins, total * sizeof(int32_t)));
             int32_t *base = static_cast<int32_t *>(p->bins);
@@ -383,7 +418,11 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
                 o += v.size();
                 return ptr;
             };
-            for (int b = 0; b < 3; ++b) p->fbin[b].cols = put(lists[b]);
+            for (int b = 0; b < RTX_NBINS; ++b) p->fbin[b].cols = put(lists[b]);
+            p->sc_all = put(sc_all);
+            p->sc_gen = put(sc_gen);
+            p->n_sc_all = int(sc_all.size() / 2);
+            p->n_sc_gen = int(sc_gen.size() / 2);
             p->gen_cols = put(gcols);
             p->gen_cells = put(gcells);
             p->n_gen_cols = int(gcols.size());

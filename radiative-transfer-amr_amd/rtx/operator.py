@@ -144,14 +144,15 @@ class TransportOperator:
 
     def plan_info(self):
         """Launch shapes of the apply: uniform bin (usable, parts per column),
-        the three fused order bins (parts, columns) and the column and cell
+        the fused order bins (parts, columns; by LDS class) and the column and cell
         counts left to the generic kernels."""
-        out = (C.c_int64 * 10)()
-        self.lib.rtx_plan_info(self.plan, out, 10)
-        v = list(out)
+        out = (C.c_int64 * 16)()
+        n = self.lib.rtx_plan_info(self.plan, out, 16)
+        v = list(out)[:n]
+        nb = (n - 4) // 2
         return {"uniform": bool(v[0]), "uniform_parts": v[1],
-                "fused_bins": [(v[2 + 2 * b], v[3 + 2 * b]) for b in range(3) if v[3 + 2 * b]],
-                "generic_cols": v[8], "generic_cells": v[9]}
+                "fused_bins": [(v[2 + 2 * b], v[3 + 2 * b]) for b in range(nb) if v[3 + 2 * b]],
+                "generic_cols": v[n - 2], "generic_cells": v[n - 1]}
 
     def set_uniform_parts(self, H):
         """Force the uniform kernel's parts per column (0: default)."""
