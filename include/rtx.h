@@ -154,17 +154,22 @@ int rtx_set_path(rtx_plan *plan, int mode);
    and every fused order bin (columns grouped by the number of parts H their
    LDS needs), time each valid H in {1, 2, 4} with `reps` launches of the
    interior operator on d_x -> d_y (device vectors of `precision`) and keep
-   the fastest.  Synchronises the stream.  No counterpart in the reference
-   (its assembly has no launch shapes). */
+   the fastest; a bin whose columns run faster on the generic kernels moves
+   there.  Synchronises the stream.  No counterpart in the reference (its
+   assembly has no launch shapes). */
 int rtx_autotune(rtx_plan *plan, int precision, const void *d_x, void *d_y, int reps,
                  void *stream);
 /* Force the uniform-bin kernel's parts per column (1, 2 or 4 where valid for
    the plan's shape; 0 restores the default).  No counterpart in the
    reference. */
 int rtx_set_uniform_parts(rtx_plan *plan, int parts);
-/* Launch-shape summary (16 values): uniform bin usable, its parts per column;
-   parts and columns of the six fused order bins (part count by LDS class);
-   columns and cells left to the generic kernels.  Returns the count. */
+/* Run fused order bin `bin` (0..5) on the generic kernels (generic = 1) or
+   on the fused kernel (0); rebuilds the generic lists (synchronises the
+   device).  No counterpart in the reference. */
+int rtx_set_bin_generic(rtx_plan *plan, int bin, int generic);
+/* Launch-shape summary (22 values): uniform bin usable, its parts per column;
+   parts, columns and generic flag of the six fused order bins (part count by
+   LDS class); columns and cells on the generic kernels.  Returns the count. */
 int rtx_plan_info(const rtx_plan *plan, int64_t *out, int n);
 
 /* In-place mask of a full-length vector: keep = 0 zeroes the inflow-boundary

@@ -37,23 +37,27 @@ template <typename T>
 __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int accumulate,
                                                   const int32_t *__restrict__ cells,
                                                   const T *__restrict__ x, T *__restrict__ y) {
-    __shared__ double xs[NM * NM * NM];
-    __shared__ double ys[NM * NM * NM];
-    __shared__ double tr[NM * NM];
-    __shared__ double cth[NM], sth[NM];
+    // x and y of the cell (dynamic: twice the plan's largest cell), the
+    // small operands of the cell and of one upwind link
+    extern __shared__ double t_lds[];
+    __shared__ double Dxw[NM * NM], Dyw[NM * NM];   // w_i D[p][i]
+    __shared__ double wxs[NM], wys[NM], wts[NM], cth[NM], sth[NM];
+    __shared__ double kap[NM * NM];
+    __shared__ double tr[NM * NM], inn[NM * NM], Ets[NM * NM], Ess[NM * NM];
     const int k = cells ? cells[blockIdx.x] : blockIdx.x;
     const int lane = threadIdx.x;
     const int c = m.cell_col[k];
     const int nx = m.col_nx[c], ny = m.col_ny[c], nth = m.cell_nth[k];
     const int n = nx * ny * nth;
+    double *xs = t_lds, *ys = t_lds + n;
     const int off = m.cell_dof_off[k];
     const bool restrict_ = (ops & RTX_OP_RESTRICT) != 0;
     const int bd = restrict_ ? m.cell_bdry[k] : 0;
     const double x0 = m.col_geom[4 * c + 0], y0 = m.col_geom[4 * c + 1];
     const double dx = m.col_geom[4 * c + 2] - x0, dy = m.col_geom[4 * c + 3] - y0;
     const double th0 = m.cell_geom[2 * k + 0], dth = m.cell_geom[2 * k + 1] - th0;
-    const double *wx = m.lgl_w + nx * NM, *wy = m.lgl_w + ny * NM, *wt = m.lg_w + nth * NM;
     const int nyt = ny * nth;
+    const int sp0 = m.col_sp_off[c];
 
     for (int t = lane; t < n; t += 64) {
         const int i = t / nyt, j = (t / nth) % ny;
@@ -61,32 +65,42 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
         if (bd && on_bdry(bd, i, j, nx, ny)) v = 0.0;
         xs[t] = v;
     }
+    for (int t = lane; t < nx * nx; t += 64) {
+        const int pp = t / nx, i = t % nx;
+        Dxw[t] = m.lgl_w[nx * NM + i] * m.lgl_D[(size_t(nx) * NM + pp) * NM + i];
+    }
+    for (int t = lane; t < ny * ny; t += 64) {
+        const int q = t / ny, j = t % ny;
+        Dyw[t] = m.lgl_w[ny * NM + j] * m.lgl_D[(size_t(ny) * NM + q) * NM + j];
+    }
+    for (int t = lane; t < nx * ny; t += 64) kap[t] = m.kappa[sp0 + t];
+    if (lane < nx) wxs[lane] = m.lgl_w[nx * NM + lane];
+    if (lane < ny) wys[lane] = m.lgl_w[ny * NM + lane];
     if (lane < nth) {
+        wts[lane] = m.lg_w[nth * NM + lane];
         const double th = th0 + dth * 0.5 * (m.lg_x[nth * NM + lane] + 1.0);
-        double s, cc;
-        sincos(th, &s, &cc);
-        cth[lane] = cc;
-        sth[lane] = s;
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        cth[lane] = cs;
+        sth[lane] = sn;
     }
     __syncthreads();
 
-    const double *Dx = m.lgl_D + size_t(nx) * NM * NM;
-    const double *Dy = m.lgl_D + size_t(ny) * NM * NM;
     const int quad = m.cell_quad[k];
-    const int sp0 = m.col_sp_off[c];
+    const double vol = dx * dy * dth * 0.125;
     for (int t = lane; t < n; t += 64) {
         const int p = t / nyt, q = (t / nth) % ny, r = t % nth;
         double acc = 0.0;
-        if (ops & RTX_OP_MASS)
-            acc += (dx * dy * dth * 0.125) * wx[p] * wy[q] * wt[r] * m.kappa[sp0 + p * ny + q] *
-                   xs[t];
-        if (ops & RTX_OP_QUADW) acc += (dx * dy * dth * 0.125) * wx[p] * wy[q] * wt[r] * xs[t];
+        const double w3 = vol * wxs[p] * wys[q] * wts[r];
+        if (ops & RTX_OP_MASS) acc += w3 * kap[p * ny + q] * xs[t];
+        if (ops & RTX_OP_QUADW) acc += w3 * xs[t];
         if (ops & RTX_OP_ICONV) {
             double sx = 0.0, sy = 0.0;
-            for (int i = 0; i < nx; ++i) sx += wx[i] * Dx[p * NM + i] * xs[i * nyt + q * nth + r];
-            for (int j = 0; j < ny; ++j) sy += wy[j] * Dy[q * NM + j] * xs[p * nyt + j * nth + r];
-            acc -= (dy * dth * 0.25) * wy[q] * wt[r] * cth[r] * sx +
-                   (dx * dth * 0.25) * wx[p] * wt[r] * sth[r] * sy;
+            const double *xi = xs + q * nth + r, *xj = xs + p * nyt + r;
+            for (int i = 0; i < nx; ++i) sx += Dxw[p * nx + i] * xi[i * nyt];
+            for (int j = 0; j < ny; ++j) sy += Dyw[q * ny + j] * xj[j * nth];
+            acc -= (dy * dth * 0.25) * wys[q] * wts[r] * cth[r] * sx +
+                   (dx * dth * 0.25) * wxs[p] * wts[r] * sth[r] * sy;
         }
         if ((ops & RTX_OP_BCONV) && quad >= 0) {
 #pragma unroll
@@ -96,8 +110,8 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
                           (F == 1 && q == ny - 1) || (F == 3 && q == 0);
                 if (on) {
                     const double cf = ((F & 1) ? dx : dy) * dth * 0.25;
-                    const double wm = (F & 1) ? wx[p] : wy[q];
-                    acc += cf * wm * wt[r] * theta_f(F, cth[r], sth[r]) * xs[t];
+                    const double wm = (F & 1) ? wxs[p] : wys[q];
+                    acc += cf * wm * wts[r] * theta_f(F, cth[r], sth[r]) * xs[t];
                 }
             }
         }
@@ -114,7 +128,9 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
             const int bd1 = restrict_ ? m.cell_bdry[s] : 0;
             const int n1 = (F & 1) ? nx1 : ny1;   // neighbour trace length
             const int n0 = (F & 1) ? nx : ny;     // own trace length
-            __syncthreads();                      // tr reuse
+            const double *Es = m.Es_pool + m.link_Es_off[l];
+            const double *Et = m.Eth_pool + m.link_Eth_off[l];
+            __syncthreads();                      // tr, inn, Ets, Ess reuse
             for (int t = lane; t < n1 * nt1; t += 64) {
                 const int m1 = t / nt1, a = t % nt1;
                 int i1, j1;
@@ -126,18 +142,22 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
                 if (bd1 && on_bdry(bd1, i1, j1, nx1, ny1)) v = 0.0;
                 tr[t] = v;
             }
+            for (int t = lane; t < nt1 * nth; t += 64) Ets[t] = Et[t];
+            for (int t = lane; t < n1 * n0; t += 64) Ess[t] = Es[t];
             __syncthreads();
-            const double *Es = m.Es_pool + m.link_Es_off[l];
-            const double *Et = m.Eth_pool + m.link_Eth_off[l];
+            // sum-factorised: angle first (n1 x nth), then space (n0 x nth)
+            for (int t = lane; t < n1 * nth; t += 64) {
+                const int m1 = t / nth, r = t % nth;
+                double v = 0.0;
+                for (int a = 0; a < nt1; ++a) v += Ets[a * nth + r] * tr[m1 * nt1 + a];
+                inn[t] = v;
+            }
+            __syncthreads();
             const double cf = ((F & 1) ? dx : dy) * dth * 0.25;
             for (int t = lane; t < n0 * nth; t += 64) {
                 const int m0 = t / nth, r = t % nth;
                 double acc = 0.0;
-                for (int m1 = 0; m1 < n1; ++m1) {
-                    double inner = 0.0;
-                    for (int a = 0; a < nt1; ++a) inner += Et[a * nth + r] * tr[m1 * nt1 + a];
-                    acc += Es[m1 * n0 + m0] * inner;
-                }
+                for (int m1 = 0; m1 < n1; ++m1) acc += Ess[m1 * n0 + m0] * inn[m1 * nth + r];
                 int node;
                 if (F == 0) node = (nx - 1) * nyt + m0 * nth + r;
                 else if (F == 2) node = m0 * nth + r;
@@ -317,7 +337,8 @@ static int launch_apply(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hip
         return RTX_OK;
     }
     if (tp && ncells > 0)
-        k_transport<T><<<ncells, 64, 0, st>>>(p->m, ops, acc ? 1 : 0, cells, x, y);
+        k_transport<T><<<ncells, 64, size_t(2) * p->max_cell_ndof * sizeof(double), st>>>(
+            p->m, ops, acc ? 1 : 0, cells, x, y);
     if (sc && nitems > 0)
         k_scatter_mma<T><<<nitems, SCAT_THREADS, 0, st>>>(p->m, ops, (acc || tp) ? 1 : 0, items,
                                                           x, y);
@@ -328,9 +349,9 @@ static int launch_apply(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hip
 // The fused per-column kernel holds a whole column (values and accumulators)
 // in LDS and tiles its spatial nodes by 16 up to 7 tiles.
 
-static bool any_fused(const rtx_plan *p) {
+static bool any_fused(const rtx_plan *p, bool all_bins) {
     for (const FusedBin &fb : p->fbin)
-        if (fb.n > 0) return true;
+        if (fb.n > 0 && (all_bins || !fb.generic)) return true;
     return false;
 }
 
@@ -366,11 +387,12 @@ extern "C" int rtx_apply(rtx_plan *p, uint32_t op_mask, int precision, const voi
     }
     RTX_CHECK(p->fused_mode != 2, "rtx_apply: the uniform-bin path was requested but the mesh "
                                   "is not a compiled uniform bin");
-    RTX_CHECK(p->fused_mode != 1 || p->n_gen_cols == 0,
+    const bool forced = p->fused_mode == 1;
+    RTX_CHECK(!forced || p->h_over_cols.empty(),
               "rtx_apply: the fused per-column path was requested but %d columns do not fit "
               "it (largest column: DOFs %d, cells %d, nodes %d)",
-              p->n_gen_cols, p->max_col_dofs, p->max_col_cells, p->max_nsp);
-    if ((p->fused_mode == -1 || p->fused_mode == 1) && any_fused(p)) {
+              int(p->h_over_cols.size()), p->max_col_dofs, p->max_col_cells, p->max_nsp);
+    if ((p->fused_mode == -1 || forced) && any_fused(p, forced)) {
         // order bins: fused bins, then the generic kernels on the rest
         const bool tp = (op_mask & (RTX_OP_MASS | RTX_OP_ICONV | RTX_OP_BCONV | RTX_OP_QUADW)) != 0;
         const bool sc = (op_mask & RTX_OP_SCAT) != 0;
@@ -382,8 +404,8 @@ extern "C" int rtx_apply(rtx_plan *p, uint32_t op_mask, int precision, const voi
             RTX_HIP(hipGetLastError());
             return RTX_OK;
         }
-        int rc = launch_fused(p, op_mask, precision, d_x, d_y, st, acc);
-        if (rc || p->n_gen_cols == 0) return rc;
+        int rc = launch_fused(p, op_mask, precision, d_x, d_y, st, acc, -1, forced);
+        if (rc || forced || p->n_gen_cols == 0) return rc;
         if (precision == RTX_PREC_F64)
             return launch_apply<double>(p, op_mask, d_x, d_y, st, p->gen_cells, p->n_gen_cells,
                                         p->sc_gen, p->n_sc_gen);
@@ -437,8 +459,9 @@ float time_launches(F fn, int reps, hipStream_t st, int *rc) {
 /* Autotune the apply's launch shapes on this plan: for the uniform-bin
    kernel and for every fused order bin, time each valid number of column
    parts H (1, 2, 4) with `reps` launches of the interior operator on d_x ->
-   d_y (device vectors in `precision`) and keep the fastest.  Synchronises
-   the stream.  The choice holds for every precision. */
+   d_y (device vectors in `precision`) and keep the fastest; then time the
+   bin's columns on the generic kernels and move the bin there if they are
+   faster.  Synchronises the stream.  The choice holds for every precision. */
 extern "C" int rtx_autotune(rtx_plan *p, int precision, const void *d_x, void *d_y, int reps,
                             void *stream) {
     RTX_CHECK(p && d_x && d_y && d_x != d_y && reps >= 1, "rtx_autotune: bad argument");
@@ -460,7 +483,8 @@ extern "C" int rtx_autotune(rtx_plan *p, int precision, const void *d_x, void *d
         }
         p->uni_H = bestH;
     }
-    for (int bi = 0; bi < RTX_NBINS; ++bi) {
+    bool changed = false;
+    for (int bi = 0; bi < RTX_NBINS && !p->uni_ok; ++bi) {   // a uniform mesh uses no bin
         FusedBin &fb = p->fbin[bi];
         if (fb.n == 0) continue;
         const int mask = fused_bin_valid_parts(p, bi, precision);
@@ -476,8 +500,50 @@ extern "C" int rtx_autotune(rtx_plan *p, int precision, const void *d_x, void *d
             if (us < best) { best = us; bestH = H; }
         }
         fb.H = bestH;
+        // the same columns on the generic kernels
+        auto gen = [&]() {
+            if (precision == RTX_PREC_F64)
+                return launch_apply<double>(p, ops, d_x, d_y, st, fb.cells, fb.ncells, fb.items,
+                                            fb.nitems);
+            return launch_apply<float>(p, ops, d_x, d_y, st, fb.cells, fb.ncells, fb.items,
+                                       fb.nitems);
+        };
+        const float ug = time_launches(gen, reps, st, &rc);
+        if (rc) return rc;
+        const bool g = ug < best;
+        changed |= g != fb.generic;
+        fb.generic = g;
+    }
+    if (changed && (rc = rebuild_generic(p))) return rc;
+    // the mix against every bin on the generic kernels (one launch pair)
+    if (!p->uni_ok && any_fused(p, false)) {
+        bool saved[RTX_NBINS];
+        for (int bi = 0; bi < RTX_NBINS; ++bi) saved[bi] = p->fbin[bi].generic;
+        auto whole = [&]() { return rtx_apply(p, ops, precision, d_x, d_y, stream); };
+        const int mode = p->fused_mode;
+        p->fused_mode = -1;
+        const float t_mix = time_launches(whole, reps, st, &rc);
+        if (rc) return rc;
+        for (int bi = 0; bi < RTX_NBINS; ++bi) p->fbin[bi].generic = true;
+        if ((rc = rebuild_generic(p))) return rc;
+        const float t_gen = time_launches(whole, reps, st, &rc);
+        if (rc) return rc;
+        if (t_mix <= t_gen) {
+            for (int bi = 0; bi < RTX_NBINS; ++bi) p->fbin[bi].generic = saved[bi];
+            if ((rc = rebuild_generic(p))) return rc;
+        }
+        p->fused_mode = mode;
     }
     return RTX_OK;
+}
+
+/* Run fused bin `bin` on the generic kernels (generic = 1) or fused (0). */
+extern "C" int rtx_set_bin_generic(rtx_plan *p, int bin, int generic) {
+    RTX_CHECK(p && bin >= 0 && bin < RTX_NBINS && (generic == 0 || generic == 1),
+              "rtx_set_bin_generic: bad argument");
+    if (p->fbin[bin].generic == (generic != 0)) return RTX_OK;
+    p->fbin[bin].generic = generic != 0;
+    return rebuild_generic(p);
 }
 
 /* Force the uniform kernel's parts per column (1, 2, 4; 0 = default). */
@@ -491,17 +557,19 @@ extern "C" int rtx_set_uniform_parts(rtx_plan *p, int H) {
 }
 
 /* Launch-shape summary: out[0] uniform bin usable, [1] its parts per column,
-   [2 + 2 b], [3 + 2 b] parts and columns of fused bin b (b < 6), [14], [15]
-   columns and cells on the generic kernels.  Returns the count (16). */
+   [2 + 3 b .. 4 + 3 b] parts, columns and generic flag of fused bin b
+   (b < 6), [20], [21] columns and cells on the generic kernels.  Returns the
+   count (22). */
 extern "C" int rtx_plan_info(const rtx_plan *p, int64_t *out, int n) {
-    constexpr int NV = 4 + 2 * RTX_NBINS;
+    constexpr int NV = 4 + 3 * RTX_NBINS;
     if (!p) return 0;
     int64_t v[NV] = {};
     v[0] = p->uni_ok ? 1 : 0;
     v[1] = p->uni_ok ? uniform_default_parts(p) : 0;
     for (int b = 0; b < RTX_NBINS; ++b) {
-        v[2 + 2 * b] = p->fbin[b].H;
-        v[3 + 2 * b] = p->fbin[b].n;
+        v[2 + 3 * b] = p->fbin[b].H;
+        v[3 + 3 * b] = p->fbin[b].n;
+        v[4 + 3 * b] = p->fbin[b].generic ? 1 : 0;
     }
     v[NV - 2] = p->n_gen_cols;
     v[NV - 1] = p->n_gen_cells;

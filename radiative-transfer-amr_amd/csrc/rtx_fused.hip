@@ -582,10 +582,11 @@ static int launch_bin(const rtx_plan *p, const FusedBin &fb, uint32_t ops, const
 }
 
 int launch_fused(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
-                 int accumulate, int only_bin) {
+                 int accumulate, int only_bin, bool all_bins) {
     for (int bi = 0; bi < RTX_NBINS; ++bi) {
         const FusedBin &fb = p->fbin[bi];
         if (fb.n == 0 || (only_bin >= 0 && bi != only_bin)) continue;
+        if (fb.generic && !all_bins && only_bin < 0) continue;
         int rc;
         if (prec == RTX_PREC_F64)
             rc = launch_bin<double, double, RTX_PREC_F64>(p, fb, ops, dx, dy, st, accumulate);

@@ -64,8 +64,15 @@ bool uniform_shape_compiled(int N, int NTH, int NC);
 int uniform_valid_parts(const rtx_plan *p);   // bit H set: H parts valid (H = 1, 2, 4)
 int uniform_default_parts(const rtx_plan *p); // parts the uniform kernel launches with
 int fused_bin_valid_parts(const rtx_plan *p, int bin, int prec);
+// Launch the fused bins (all of them with all_bins, else those not moved to
+// the generic kernels; only_bin >= 0 restricts to one bin).
 int launch_fused(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
-                 int accumulate, int only_bin = -1);
+                 int accumulate, int only_bin = -1, bool all_bins = false);
+// Scattering work items of a set of columns (pairs: column, tile word).
+std::vector<int32_t> scatter_items(const rtx_plan *p, const std::vector<int32_t> &cols);
+// Rebuild the generic kernels' column, cell and item lists: the columns no
+// fused bin takes, plus the bins marked generic.
+int rebuild_generic(rtx_plan *p);
 // Limits of the fused per-column kernel: 0 when a column with these shapes
 // cannot run on it, else 1 (bytes64: its dynamic LDS per part in FP64).
 int fused_column_ok(int ncells, int nsp, int nang, int part_links, size_t bytes64);
@@ -88,6 +95,11 @@ struct FusedBin {
     size_t bytes[2][3] = {{0, 0, 0}, {0, 0, 0}};
     int part_links[3] = {0, 0, 0};
     int min_cells = 0;
+    // the same columns on the generic kernels (cells, scattering items), and
+    // whether the apply runs them there (set by the autotuner)
+    const int32_t *cells = nullptr, *items = nullptr;
+    int ncells = 0, nitems = 0;
+    bool generic = false;
 };
 }  // namespace rtx
 
@@ -123,6 +135,11 @@ struct rtx_plan {
     // over every column and over the generic columns, XCD-interleaved
     const int32_t *sc_all = nullptr, *sc_gen = nullptr;
     int n_sc_all = 0, n_sc_gen = 0;
+    void *gen_alloc = nullptr;   // device lists of the generic set
+    // host copies: per-column cell start (ncol + 1), spatial nodes, angles;
+    // fused-bin columns and the columns no bin takes
+    std::vector<int32_t> h_col_cell_start, h_col_nsp, h_col_nang;
+    std::vector<int32_t> h_bin_cols[rtx::RTX_NBINS], h_over_cols;
     void *bins = nullptr;
     bool has_coeffs = false;
     // block-Jacobi

@@ -369,43 +369,25 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
             fb.n += 1;
             fb.H = Hs[pick];
         }
-        // tiled scattering items: columns go to the XCD with the least work
-        // so far (work: MFMA tiles x angles), each XCD's queue in column
-        // order, then interleaved so block b runs on XCD b % 8
-        auto scatter_items = [&](const std::vector<int32_t> &cols) {
-            std::vector<std::vector<int32_t>> q(SC_XCD);
-            int64_t load[SC_XCD] = {0, 0, 0, 0, 0, 0, 0, 0};
-            for (int c : cols) {
-                const int nsp = d->col_nx[c] * d->col_ny[c], na = d->col_nang[c];
-                const int nst = (nsp + 15) / 16, nrg = (na + SC_RG - 1) / SC_RG;
-                int x = 0;
-                for (int i = 1; i < SC_XCD; ++i)
-                    if (load[i] < load[x]) x = i;
-                for (int st = 0; st < nst; ++st)
-                    for (int rg = 0; rg < nrg; ++rg) {
-                        q[x].push_back(c);
-                        q[x].push_back(st | (rg << 16));
-                    }
-                load[x] += int64_t(nst) * nrg * na;
-            }
-            size_t len = 0;
-            for (auto &v : q) len = std::max(len, v.size() / 2);
-            std::vector<int32_t> out;
-            out.reserve(len * SC_XCD * 2);
-            for (size_t i = 0; i < len; ++i)
-                for (int x = 0; x < SC_XCD; ++x) {
-                    const bool has = 2 * i < q[x].size();
-                    out.push_back(has ? q[x][2 * i] : -1);
-                    out.push_back(has ? q[x][2 * i + 1] : 0);
-                }
-            return out;
-        };
+        p->h_col_cell_start.assign(d->col_cell_start, d->col_cell_start + nc + 1);
+        p->h_col_nsp.resize(nc);
+        p->h_col_nang.assign(d->col_nang, d->col_nang + nc);
+        for (int c = 0; c < nc; ++c) p->h_col_nsp[c] = d->col_nx[c] * d->col_ny[c];
+        for (int b = 0; b < RTX_NBINS; ++b) p->h_bin_cols[b] = lists[b];
+        p->h_over_cols = gcols;
         std::vector<int32_t> all_cols(nc);
         for (int c = 0; c < nc; ++c) all_cols[c] = c;
-        const std::vector<int32_t> sc_all = scatter_items(all_cols), sc_gen = scatter_items(gcols);
-        size_t total = 0;
-        for (int b = 0; b < RTX_NBINS; ++b) total += lists[b].size();
-        total += gcols.size() + gcells.size() + sc_all.size() + sc_gen.size();
+        const std::vector<int32_t> sc_all = scatter_items(p, all_cols);
+        std::vector<int32_t> bcells[RTX_NBINS], bitems[RTX_NBINS];
+        for (int b = 0; b < RTX_NBINS; ++b) {
+            for (int c : lists[b])
+                for (int k = d->col_cell_start[c]; k < d->col_cell_start[c + 1]; ++k)
+                    bcells[b].push_back(k);
+            bitems[b] = scatter_items(p, lists[b]);
+        }
+        size_t total = sc_all.size();
+        for (int b = 0; b < RTX_NBINS; ++b)
+            total += lists[b].size() + bcells[b].size() + bitems[b].size();
         if (total > 0) {
             RTX_HIP(hipMalloc(&p->bins, total * sizeof(int32_t)));
             int32_t *base = static_cast<int32_t *>(p->bins);
@@ -418,22 +400,96 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
                 o += v.size();
                 return ptr;
             };
-            for (int b = 0; b < RTX_NBINS; ++b) p->fbin[b].cols = put(lists[b]);
+            for (int b = 0; b < RTX_NBINS; ++b) {
+                FusedBin &fb = p->fbin[b];
+                fb.cols = put(lists[b]);
+                fb.cells = put(bcells[b]);
+                fb.ncells = int(bcells[b].size());
+                fb.items = put(bitems[b]);
+                fb.nitems = int(bitems[b].size() / 2);
+            }
             p->sc_all = put(sc_all);
-            p->sc_gen = put(sc_gen);
             p->n_sc_all = int(sc_all.size() / 2);
-            p->n_sc_gen = int(sc_gen.size() / 2);
-            p->gen_cols = put(gcols);
-            p->gen_cells = put(gcells);
-            p->n_gen_cols = int(gcols.size());
-            p->n_gen_cells = int(gcells.size());
             RTX_HIP(hipGetLastError());
         }
+        const int rc = rebuild_generic(p);
+        if (rc) return rc;
     }
     // sizes of the column G blocks are checked in set_coeffs
     *out = p;
     return RTX_OK;
 }
+
+namespace rtx {
+
+// Columns go to the XCD with the least work so far (work: tiles x angles),
+// each XCD's queue in column order, then the queues are interleaved so that
+// block b runs on XCD b % 8 and a column's items share that XCD's L2.
+std::vector<int32_t> scatter_items(const rtx_plan *p, const std::vector<int32_t> &cols) {
+    std::vector<std::vector<int32_t>> q(SC_XCD);
+    int64_t load[SC_XCD] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c : cols) {
+        const int nsp = p->h_col_nsp[c], na = p->h_col_nang[c];
+        const int nst = (nsp + 15) / 16, nrg = (na + SC_RG - 1) / SC_RG;
+        int x = 0;
+        for (int i = 1; i < SC_XCD; ++i)
+            if (load[i] < load[x]) x = i;
+        for (int st = 0; st < nst; ++st)
+            for (int rg = 0; rg < nrg; ++rg) {
+                q[x].push_back(c);
+                q[x].push_back(st | (rg << 16));
+            }
+        load[x] += int64_t(nst) * nrg * na;
+    }
+    size_t len = 0;
+    for (auto &v : q) len = std::max(len, v.size() / 2);
+    std::vector<int32_t> out;
+    out.reserve(len * SC_XCD * 2);
+    for (size_t i = 0; i < len; ++i)
+        for (int x = 0; x < SC_XCD; ++x) {
+            const bool has = 2 * i < q[x].size();
+            out.push_back(has ? q[x][2 * i] : -1);
+            out.push_back(has ? q[x][2 * i + 1] : 0);
+        }
+    return out;
+}
+
+int rebuild_generic(rtx_plan *p) {
+    std::vector<int32_t> cols = p->h_over_cols;
+    for (int b = 0; b < RTX_NBINS; ++b)
+        if (p->fbin[b].generic)
+            cols.insert(cols.end(), p->h_bin_cols[b].begin(), p->h_bin_cols[b].end());
+    std::sort(cols.begin(), cols.end());
+    std::vector<int32_t> cells;
+    for (int c : cols)
+        for (int k = p->h_col_cell_start[c]; k < p->h_col_cell_start[c + 1]; ++k)
+            cells.push_back(k);
+    const std::vector<int32_t> items = scatter_items(p, cols);
+    if (p->gen_alloc) {
+        RTX_HIP(hipDeviceSynchronize());   // no launch may still read the old lists
+        RTX_HIP(hipFree(p->gen_alloc));
+        p->gen_alloc = nullptr;
+    }
+    p->gen_cols = p->gen_cells = p->sc_gen = nullptr;
+    p->n_gen_cols = int(cols.size());
+    p->n_gen_cells = int(cells.size());
+    p->n_sc_gen = int(items.size() / 2);
+    const size_t total = cols.size() + cells.size() + items.size();
+    if (total == 0) return RTX_OK;
+    RTX_HIP(hipMalloc(&p->gen_alloc, total * sizeof(int32_t)));
+    int32_t *base = static_cast<int32_t *>(p->gen_alloc);
+    RTX_HIP(hipMemcpy(base, cols.data(), cols.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    RTX_HIP(hipMemcpy(base + cols.size(), cells.data(), cells.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+    RTX_HIP(hipMemcpy(base + cols.size() + cells.size(), items.data(),
+                      items.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    p->gen_cols = base;
+    p->gen_cells = base + cols.size();
+    p->sc_gen = base + cols.size() + cells.size();
+    return RTX_OK;
+}
+
+}  // namespace rtx
 
 extern "C" void rtx_plan_destroy(rtx_plan *p) {
     if (!p) return;
@@ -446,6 +502,7 @@ extern "C" void rtx_plan_destroy(rtx_plan *p) {
     if (p->bj_blk) (void)hipFree(p->bj_blk);
     if (p->coarse) (void)hipFree(p->coarse);
     if (p->bins) (void)hipFree(p->bins);
+    if (p->gen_alloc) (void)hipFree(p->gen_alloc);
     (void)hipSetDevice(cur);
     delete p;
 }

@@ -61,6 +61,7 @@ SIGNATURES = [
                                C.c_void_p]),
     ("rtx_plan_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int]),
     ("rtx_set_uniform_parts", C.c_int, [C.c_void_p, C.c_int]),
+    ("rtx_set_bin_generic", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("rtx_assemble_entries", C.c_int, [C.c_void_p, C.c_int, C.c_int64] + [C.c_void_p] * 7),
     ("rtx_bjacobi_size", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("rtx_bjacobi_factor", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,

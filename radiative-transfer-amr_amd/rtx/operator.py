@@ -146,13 +146,19 @@ class TransportOperator:
         """Launch shapes of the apply: uniform bin (usable, parts per column),
         the fused order bins (parts, columns; by LDS class) and the column and cell
         counts left to the generic kernels."""
-        out = (C.c_int64 * 16)()
-        n = self.lib.rtx_plan_info(self.plan, out, 16)
+        out = (C.c_int64 * 22)()
+        n = self.lib.rtx_plan_info(self.plan, out, 22)
         v = list(out)[:n]
-        nb = (n - 4) // 2
+        bins = [tuple(v[2 + 3 * b: 5 + 3 * b]) for b in range((n - 4) // 3)]
         return {"uniform": bool(v[0]), "uniform_parts": v[1],
-                "fused_bins": [(v[2 + 2 * b], v[3 + 2 * b]) for b in range(nb) if v[3 + 2 * b]],
+                "fused_bins": [(H, nc) for H, nc, g in bins if nc and not g],
+                "bins_on_generic": [nc for H, nc, g in bins if nc and g],
                 "generic_cols": v[n - 2], "generic_cells": v[n - 1]}
+
+    def set_bin_generic(self, b, generic=True):
+        """Run fused order bin ``b`` on the generic kernels (or back)."""
+        check(self.lib.rtx_set_bin_generic(self.plan, int(b), int(bool(generic))),
+              "rtx_set_bin_generic")
 
     def set_uniform_parts(self, H):
         """Force the uniform kernel's parts per column (0: default)."""

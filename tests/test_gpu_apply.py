@@ -248,8 +248,20 @@ def test_uniform_parts_and_autotune(cuda):
     before = op.plan_info()
     info = op.autotune(reps=3)
     print("hp bins", before, "->", info, flush=True)
-    assert [n for _, n in info["fused_bins"]] == [n for _, n in before["fused_bins"]]
+    assert sorted([n for _, n in info["fused_bins"]] + info["bins_on_generic"]) == \
+        sorted(n for _, n in before["fused_bins"])
     x = np.random.default_rng(4).uniform(-1.0, 1.0, op.ndof)
     mask = asm.intr_mask(mesh)
-    y = op.apply(_dev(x), ops=OP_SYSTEM | OP_RESTRICT).cpu().numpy()
-    assert _rel(y, mask * (A @ (mask * x))) < TOL64
+    ref = mask * (A @ (mask * x))
+    for path in (-1, 0, 1):   # tuned, all generic, all fused
+        op.set_path(path)
+        y = op.apply(_dev(x), ops=OP_SYSTEM | OP_RESTRICT).cpu().numpy()
+        assert _rel(y, ref) < TOL64, (path, _rel(y, ref))
+    # bins moved to the generic kernels by hand: the generic set is rebuilt
+    op.set_path(-1)
+    for flags in ([1] * 6, [0, 1] * 3, [0] * 6):
+        for b, g in enumerate(flags):
+            op.set_bin_generic(b, g)
+        y = op.apply(_dev(x), ops=OP_SYSTEM | OP_RESTRICT).cpu().numpy()
+        assert _rel(y, ref) < TOL64, (flags, _rel(y, ref))
+    assert op.plan_info()["bins_on_generic"] == []
