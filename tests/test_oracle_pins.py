@@ -587,3 +587,176 @@ def test_mlp_oracle_forward_and_gradients():
                 fd = (fp - fm) / (2 * h)
                 an = grads[li][which].reshape(-1)[idx]
                 assert abs(fd - an) < 1e-6 * max(1.0, abs(fd)), (li, which, fd, an)
+
+
+# ------------------------------------------------------------ split blocks, coarse grid
+
+
+def _hp_cols_mesh():
+    """4 x 4 columns of 3 x 3 nodes, 2 angular cells of 3 angles; one column
+    p-refined and one cell h-refined so DOF counts differ by position."""
+    m = problems.uniform_mesh([1.0, 1.0], [3, 3, 3], 1, 2)
+    keys = sorted(m.cols)
+    m.ref_col(keys[5], "spt", "p")
+    last = sorted(m.cols)[-1]
+    m.ref_cell(last, sorted(m.cols[last].cells)[0], "h")
+    return m
+
+
+@pytest.mark.parametrize("split", [None, 9, 20, 40])
+def test_bj_split_blocks_partition_cells(split):
+    """Blocks cover every DOF once, stay inside one cell, hold every spatial
+    node of a contiguous angle range, and respect the size bound."""
+    mesh = _hp_cols_mesh()
+    blocks = system.bj_block_dofs(mesh, split)
+    cells = system.cell_blocks(mesh)
+    N = sum(n for _, n in cells)
+    seen = np.concatenate(blocks)
+    assert np.array_equal(np.sort(seen), np.arange(N))
+    starts = np.array([o for o, _ in cells])
+    for idx in blocks:
+        ci = np.searchsorted(starts, idx[0], side="right") - 1
+        o, n = cells[ci]
+        assert idx.min() >= o and idx.max() < o + n
+    # structure per cell from the geometry of the mesh (not the oracle's loop)
+    offs, _ = meshq.dof_offsets(mesh)
+    for ck, kks in meshq.leaf_order(mesh):
+        col = mesh.cols[ck]
+        nsp = col.ndofs[0] * col.ndofs[1]
+        for kk in kks:
+            nth = col.cells[kk].ndofs[0]
+            o = offs[(ck, kk)]
+            mine = [b for b in blocks if o <= b[0] < o + nsp * nth]
+            if split is None or nsp * nth <= split:
+                assert len(mine) == 1 and mine[0].size == nsp * nth
+                continue
+            angs = []
+            for b in mine:
+                loc = b - o
+                s, a = loc // nth, loc % nth
+                assert np.array_equal(np.unique(s), np.arange(nsp))        # all nodes
+                ua = np.unique(a)
+                assert np.array_equal(ua, np.arange(ua[0], ua[-1] + 1))    # contiguous
+                assert b.size <= max(split, nsp)
+                angs.append(ua[0])
+            assert angs == sorted(angs)
+
+
+def test_bj_split_inverses_solve_their_blocks():
+    mesh = _hp_cols_mesh()
+    wl = problems.manufactured()
+    V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi)
+    A = asm.system_matrix(V).tocsr()
+    mask = asm.intr_mask(mesh)
+    r = np.random.default_rng(0).normal(size=A.shape[0]) * mask
+    for split in (None, 20):
+        invs = system.block_jacobi_inverses(A, mesh, mask, split)
+        z = system.block_jacobi_apply(invs, mesh, r, split)
+        for idx in system.bj_block_dofs(mesh, split):
+            blk = A[idx][:, idx].toarray()
+            m = mask[idx]
+            # interior rows of the block: blk_II z_I = r_I (backward error,
+            # some blocks of this coarse mesh are badly conditioned), boundary z = 0
+            B = blk[np.ix_(m, m)]
+            lhs = B @ z[idx][m]
+            scale = np.abs(B).sum(1) * np.abs(z[idx][m]).max() + np.abs(r[idx][m])
+            assert np.all(np.abs(lhs - r[idx][m]) <= 1e-12 * scale)
+            assert np.all(z[idx][~m] == 0.0)
+
+
+def _col_geom(mesh):
+    return [mesh.cols[ck].pos for ck, _ in meshq.leaf_order(mesh)]
+
+
+@pytest.mark.parametrize("n", [4, 2, 3])
+def test_coarse_map_partitions_and_positions(n):
+    """fcol sums to 1 per column, fcell to 1 per coarse cell; when the grid
+    matches or coarsens the columns every column lies in the coarse cell
+    holding its centre (g = gx n + gy)."""
+    mesh = _hp_cols_mesh()
+    cmap = system.coarse_map(mesh, n)
+    geo = _col_geom(mesh)
+    tot_cell = np.zeros(n * n)
+    for ent, (x0, y0, x1, y1) in zip(cmap, geo):
+        assert abs(sum(e[2] for e in ent) - 1.0) < 1e-12
+        for g, fcell, fcol in ent:
+            tot_cell[g] += fcell
+            # fcell / fcol = area(col) / area(coarse cell)
+            assert abs(fcell / fcol - (x1 - x0) * (y1 - y0) * n * n) < 1e-12
+        if n in (2, 4):
+            cx, cy = 0.5 * (x0 + x1), 0.5 * (y0 + y1)
+            assert len(ent) == 1 and ent[0][0] == int(cx * n) * n + int(cy * n)
+    assert np.allclose(tot_cell, 1.0, atol=1e-12)
+
+
+def test_coarse_moments_closed_forms():
+    """On a grid that coarsens the columns: channel 0 of r = 1 counts the
+    DOFs of the columns inside each coarse cell; the three channels of a
+    general r sum to the global sums of r, r cos th, r sin th."""
+    mesh = _hp_cols_mesh()
+    n = 2
+    offs, N = meshq.dof_offsets(mesh)
+    geo = _col_geom(mesh)
+    mom = system.coarse_moments(mesh, n, np.ones(N))
+    want = np.zeros(n * n)
+    for (ck, kks), (x0, y0, x1, y1) in zip(meshq.leaf_order(mesh), geo):
+        col = mesh.cols[ck]
+        nd = col.ndofs[0] * col.ndofs[1] * sum(col.cells[k].ndofs[0] for k in kks)
+        want[int(0.5 * (x0 + x1) * n) * n + int(0.5 * (y0 + y1) * n)] += nd
+    assert np.allclose(mom[:n * n], want, rtol=0, atol=1e-9)
+    # global sums against the quadrature nodes of every cell
+    r = np.random.default_rng(1).normal(size=N)
+    th = np.zeros(N)
+    for ck, kks in meshq.leaf_order(mesh):
+        col = mesh.cols[ck]
+        nx, ny = col.ndofs
+        for kk in kks:
+            c = col.cells[kk]
+            nth = c.ndofs[0]
+            t = c.pos[0] + (c.pos[1] - c.pos[0]) * 0.5 * (np.polynomial.legendre.leggauss(nth)[0] + 1)
+            th[offs[(ck, kk)]:offs[(ck, kk)] + nx * ny * nth] = np.tile(t, nx * ny)
+    for m3 in (2, 3):
+        mom = system.coarse_moments(mesh, m3, r)
+        k = m3 * m3
+        assert abs(mom[:k].sum() - r.sum()) < 1e-10 * np.abs(r).sum()
+        assert abs(mom[k:2 * k].sum() - (r * np.cos(th)).sum()) < 1e-10 * np.abs(r).sum()
+        assert abs(mom[2 * k:].sum() - (r * np.sin(th)).sum()) < 1e-10 * np.abs(r).sum()
+
+
+def test_coarse_average_and_prolong_closed_forms():
+    """Average of a constant is that constant; of x (resp. y) it is the
+    coarse cell's centre when the cell is a union of columns.  Prolongation
+    of a coarse indicator is `scale` on the columns inside it, 0 elsewhere
+    and on masked DOFs."""
+    mesh = _hp_cols_mesh()
+    n = 2
+    offs, N = meshq.dof_offsets(mesh)
+    xs, ys = np.zeros(N), np.zeros(N)
+    colid = np.zeros(N, int)
+    geo = _col_geom(mesh)
+    for ci, ((ck, kks), (x0, y0, x1, y1)) in enumerate(zip(meshq.leaf_order(mesh), geo)):
+        col = mesh.cols[ck]
+        nx, ny = col.ndofs
+        xn = x0 + (x1 - x0) * 0.5 * (quad.lgl(nx)[0] + 1)
+        yn = y0 + (y1 - y0) * 0.5 * (quad.lgl(ny)[0] + 1)
+        for kk in kks:
+            nth = col.cells[kk].ndofs[0]
+            o = offs[(ck, kk)]
+            sl = slice(o, o + nx * ny * nth)
+            xs[sl] = np.repeat(xn, ny * nth)
+            ys[sl] = np.tile(np.repeat(yn, nth), nx)
+            colid[sl] = ci
+    assert np.allclose(system.coarse_average(mesh, n, np.full(N, 2.5)), 2.5, atol=1e-12)
+    cen = (np.arange(n) + 0.5) / n
+    ax = system.coarse_average(mesh, n, xs).reshape(n, n)
+    ay = system.coarse_average(mesh, n, ys).reshape(n, n)
+    assert np.allclose(ax, np.repeat(cen[:, None], n, 1), atol=1e-12)
+    assert np.allclose(ay, np.repeat(cen[None, :], n, 0), atol=1e-12)
+    mask = asm.intr_mask(mesh)
+    for g in range(n * n):
+        phi = np.zeros(n * n)
+        phi[g] = 1.0
+        z = system.coarse_prolong(mesh, n, phi, 0.7, mask)
+        inside = np.array([int(0.5 * (x0 + x1) * n) * n + int(0.5 * (y0 + y1) * n) == g
+                           for (x0, y0, x1, y1) in geo])[colid]
+        assert np.allclose(z, np.where(inside & mask, 0.7, 0.0), atol=1e-14)
