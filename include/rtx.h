@@ -228,6 +228,11 @@ int rtx_arnoldi_workspace(int max_vecs, int64_t *n_doubles);
    passes.  Asynchronous. */
 int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n, double *d_w,
                      double *d_hcol, double *d_work, int refine, void *stream);
+/* The same step on an FP32 basis d_V / d_w (the inner solves of mixed
+   iterative refinement); dot products, norms and d_hcol in FP64.  No
+   counterpart in the reference (PETSc's basis is FP64). */
+int rtx_arnoldi_step_f32(const float *d_V, int64_t ld, int nv, int64_t n, float *d_w,
+                         double *d_hcol, double *d_work, int refine, void *stream);
 /* Least-squares state of one restart cycle of m steps kept on the device
    (Givens QR of the Hessenberg matrix; PETSc keeps it on the host, here it
    stays on the GPU so whole chunks of iterations can run as one HIP graph).
@@ -247,9 +252,16 @@ int rtx_mdot(const double *d_V, int64_t ld, int nv, int64_t n, const double *d_w
 /* d_x = (accumulate ? d_x : 0) + sum_j y_j V_j, y on the device. */
 int rtx_combine(const double *d_V, int64_t ld, int nv, const double *d_y, int64_t n,
                 double *d_x, int accumulate, void *stream);
+/* d_x (+)= sum_j y_j V_j for an FP32 basis; d_x FP64 (x_precision 0) or
+   FP32 (1); accumulated in FP64.  No counterpart in the reference. */
+int rtx_combine_f32(const float *d_V, int64_t ld, int nv, const double *d_y, int64_t n,
+                    void *d_x, int x_precision, int accumulate, void *stream);
 /* y = a x + b y (precision RTX_PREC_F64 or RTX_PREC_F32). */
 int rtx_axpby(int precision, int64_t n, double a, const void *d_x, double b, void *d_y,
               void *stream);
+/* y = a x + b y, x FP32 and y FP64 (direction 0) or x FP64 and y FP32 (1). */
+int rtx_axpby_mixed(int direction, int64_t n, double a, const void *d_x, double b, void *d_y,
+                    void *stream);
 /* direction 0: FP64 -> FP32, 1: FP32 -> FP64. */
 int rtx_convert(int direction, int64_t n, const void *d_src, void *d_dst, void *stream);
 

@@ -35,8 +35,9 @@ __device__ inline void block_sum(double (&v)[N], double *sh) {
 
 // partial[j * nb + b] = sum over block b's rows of V_j[i] w[i], j < nv;
 // if with_ww, partial[nv * nb + b] = sum w[i]^2 (computed by chunk 0).
-__global__ __launch_bounds__(KT) void k_mdot(const double *__restrict__ V, int64_t ld, int nv,
-                                             const double *__restrict__ w, int64_t n,
+template <typename T>
+__global__ __launch_bounds__(KT) void k_mdot(const T *__restrict__ V, int64_t ld, int nv,
+                                             const T *__restrict__ w, int64_t n,
                                              double *__restrict__ partial, int with_ww,
                                              const int *__restrict__ skip) {
     if (skip && *skip) return;
@@ -49,10 +50,10 @@ __global__ __launch_bounds__(KT) void k_mdot(const double *__restrict__ V, int64
 #pragma unroll
     for (int j = 0; j <= KCH; ++j) acc[j] = 0.0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += KT) {
-        const double wi = w[i];
+        const double wi = double(w[i]);
 #pragma unroll
         for (int j = 0; j < KCH; ++j)
-            if (j0 + j < nv) acc[j] += V[int64_t(j0 + j) * ld + i] * wi;
+            if (j0 + j < nv) acc[j] += double(V[int64_t(j0 + j) * ld + i]) * wi;
         acc[KCH] += wi * wi;
     }
     block_sum<KCH + 1>(acc, sh);
@@ -76,10 +77,12 @@ __global__ __launch_bounds__(KT) void k_reduce(const double *__restrict__ partia
     if (threadIdx.x == 0) out[j] = accumulate ? out[j] + sh[0] : sh[0];
 }
 
-// w -= V h (nv vectors); partial[b] = sum over block b of the new w^2.
-__global__ __launch_bounds__(KT) void k_update(const double *__restrict__ V, int64_t ld, int nv,
+// w -= V h (nv vectors); partial[b] = sum over block b of the new w^2
+// (accumulated in FP64 for either storage type).
+template <typename T>
+__global__ __launch_bounds__(KT) void k_update(const T *__restrict__ V, int64_t ld, int nv,
                                                const double *__restrict__ h,
-                                               double *__restrict__ w, int64_t n,
+                                               T *__restrict__ w, int64_t n,
                                                double *__restrict__ partial,
                                                const int *__restrict__ skip) {
     if (skip && *skip) return;
@@ -92,15 +95,18 @@ __global__ __launch_bounds__(KT) void k_update(const double *__restrict__ V, int
     const int64_t i0 = int64_t(b) * per, i1 = (i0 + per < n) ? i0 + per : n;
     double ss[1] = {0.0};
     for (int64_t i = i0 + threadIdx.x; i < i1; i += KT) {
-        double acc = w[i];
+        double acc = double(w[i]);
         int j = 0;
         for (; j + 4 <= nv; j += 4) {
-            acc -= V[int64_t(j) * ld + i] * hs[j] + V[int64_t(j + 1) * ld + i] * hs[j + 1] +
-                   V[int64_t(j + 2) * ld + i] * hs[j + 2] + V[int64_t(j + 3) * ld + i] * hs[j + 3];
+            acc -= double(V[int64_t(j) * ld + i]) * hs[j] +
+                   double(V[int64_t(j + 1) * ld + i]) * hs[j + 1] +
+                   double(V[int64_t(j + 2) * ld + i]) * hs[j + 2] +
+                   double(V[int64_t(j + 3) * ld + i]) * hs[j + 3];
         }
-        for (; j < nv; ++j) acc -= V[int64_t(j) * ld + i] * hs[j];
-        w[i] = acc;
-        ss[0] += acc * acc;
+        for (; j < nv; ++j) acc -= double(V[int64_t(j) * ld + i]) * hs[j];
+        const T wn = T(acc);
+        w[i] = wn;
+        ss[0] += double(wn) * double(wn);
     }
     block_sum<1>(ss, sh);
     if (threadIdx.x == 0) partial[b] = sh[0];
@@ -117,28 +123,39 @@ __global__ void k_set_flag(int *flag, int v) {
 }
 
 // v_out = w / sqrt(nrm2); hcol[nv] = sqrt(nrm2).  nrm2 comes from state.
-__global__ void k_normalize(const double *__restrict__ w, double *__restrict__ v_out, int64_t n,
+template <typename T>
+__global__ void k_normalize(const T *__restrict__ w, T *__restrict__ v_out, int64_t n,
                             const double *__restrict__ nrm2, double *__restrict__ hcol, int nv) {
     const double beta = sqrt(*nrm2);
     const double s = beta > 0.0 ? 1.0 / beta : 0.0;
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    for (; i < n; i += int64_t(gridDim.x) * blockDim.x) v_out[i] = w[i] * s;
+    for (; i < n; i += int64_t(gridDim.x) * blockDim.x) v_out[i] = T(double(w[i]) * s);
     if (blockIdx.x == 0 && threadIdx.x == 0) hcol[nv] = beta;
 }
 
 // x += sum_j y_j V_j  (y on the device)
-__global__ __launch_bounds__(KT) void k_combine(const double *__restrict__ V, int64_t ld, int nv,
+template <typename TV, typename TX>
+__global__ __launch_bounds__(KT) void k_combine(const TV *__restrict__ V, int64_t ld, int nv,
                                                 const double *__restrict__ y,
-                                                double *__restrict__ x, int64_t n, int accumulate) {
+                                                TX *__restrict__ x, int64_t n, int accumulate) {
     __shared__ double ys[KMAXV];
     for (int j = threadIdx.x; j < nv; j += KT) ys[j] = y[j];
     __syncthreads();
     int64_t i = int64_t(blockIdx.x) * KT + threadIdx.x;
     for (; i < n; i += int64_t(gridDim.x) * KT) {
-        double acc = accumulate ? x[i] : 0.0;
-        for (int j = 0; j < nv; ++j) acc += V[int64_t(j) * ld + i] * ys[j];
-        x[i] = acc;
+        double acc = accumulate ? double(x[i]) : 0.0;
+        for (int j = 0; j < nv; ++j) acc += double(V[int64_t(j) * ld + i]) * ys[j];
+        x[i] = TX(acc);
     }
+}
+
+// y = a x + b y with x and y of different storage types
+template <typename TX, typename TY>
+__global__ void k_axpby_mixed(int64_t n, double a, const TX *__restrict__ x, double b,
+                              TY *__restrict__ y) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        y[i] = TY(a * double(x[i]) + (b == 0.0 ? 0.0 : b * double(y[i])));
 }
 
 // y = a x + b y (device scalars optional)
@@ -230,8 +247,9 @@ extern "C" int rtx_arnoldi_workspace(int max_vecs, int64_t *n_doubles) {
    two passes; normalise into V_nv and write the Hessenberg column
    h_0..h_{nv-1}, h_nv = |w| to d_hcol (device, nv + 1 doubles).  w is
    overwritten.  Everything is enqueued on `stream`; no host sync. */
-extern "C" int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n, double *d_w,
-                                double *d_hcol, double *d_work, int refine, void *stream) {
+template <typename T>
+static int arnoldi_step(const T *d_V, int64_t ld, int nv, int64_t n, T *d_w, double *d_hcol,
+                        double *d_work, int refine, void *stream) {
     RTX_CHECK(d_V && d_w && d_hcol && d_work, "rtx_arnoldi_step: null argument");
     RTX_CHECK(refine >= 0 && refine <= 2, "rtx_arnoldi_step: refine must be 0, 1 or 2");
     RTX_CHECK(nv >= 1 && nv < KMAXV - 1, "rtx_arnoldi_step: nv=%d outside [1, %d)", nv,
@@ -243,17 +261,17 @@ extern "C" int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n
     double *state = part2 + KNB;                            // [ww0, ww1, h2...]
     int *flag = reinterpret_cast<int *>(state + 4);
     const int nchunk = (nv + KCH - 1) / KCH;
-    double *V_out = const_cast<double *>(d_V) + int64_t(nv) * ld;
+    T *V_out = const_cast<T *>(d_V) + int64_t(nv) * ld;
     // pass 1: h = V^T w, |w|^2
-    k_mdot<<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, part, refine == 1 ? 1 : 0,
-                                             nullptr);
+    k_mdot<T><<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, part, refine == 1 ? 1 : 0,
+                                                nullptr);
     k_reduce<<<nv, KT, 0, st>>>(part, KNB, d_hcol, 0, nullptr);
     if (refine == 1) k_reduce<<<1, KT, 0, st>>>(part + int64_t(nv) * KNB, KNB, state + 0, 0, nullptr);
     // w -= V h, |w'|^2
-    k_update<<<KNB, KT, 0, st>>>(d_V, ld, nv, d_hcol, d_w, n, part2, nullptr);
+    k_update<T><<<KNB, KT, 0, st>>>(d_V, ld, nv, d_hcol, d_w, n, part2, nullptr);
     k_reduce<<<1, KT, 0, st>>>(part2, KNB, state + 1, 0, nullptr);
     if (refine == 0) {     // classical Gram-Schmidt, one pass (PETSc's default)
-        k_normalize<<<2048, KT, 0, st>>>(d_w, V_out, n, state + 1, d_hcol, nv);
+        k_normalize<T><<<2048, KT, 0, st>>>(d_w, V_out, n, state + 1, d_hcol, nv);
         RTX_HIP(hipGetLastError());
         return RTX_OK;
     }
@@ -262,16 +280,34 @@ extern "C" int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n
     else
         k_set_flag<<<1, 1, 0, st>>>(flag, 0);
     // optional pass 2 (skipped on the device when flag == 1)
-    k_mdot<<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, part, 0, flag);
+    k_mdot<T><<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, part, 0, flag);
     // accumulate the correction into hcol, keep it separately for the update
     double *hcorr = part + int64_t(KMAXV) * KNB;            // nv doubles, after the partials
     k_reduce<<<nv, KT, 0, st>>>(part, KNB, hcorr, 0, flag);
-    k_update<<<KNB, KT, 0, st>>>(d_V, ld, nv, hcorr, d_w, n, part2, flag);
+    k_update<T><<<KNB, KT, 0, st>>>(d_V, ld, nv, hcorr, d_w, n, part2, flag);
     k_reduce<<<1, KT, 0, st>>>(part2, KNB, state + 1, 0, flag);
     k_reduce<<<nv, KT, 0, st>>>(hcorr, 1, d_hcol, 1, flag);  // hcol += hcorr
-    k_normalize<<<2048, KT, 0, st>>>(d_w, V_out, n, state + 1, d_hcol, nv);
+    k_normalize<T><<<2048, KT, 0, st>>>(d_w, V_out, n, state + 1, d_hcol, nv);
     RTX_HIP(hipGetLastError());
     return RTX_OK;
+}
+
+/* One Arnoldi step: orthogonalise w against V_0..V_{nv-1} (columns of length n,
+   stride ld) by classical Gram-Schmidt; refine 0: one pass, 1: a second pass
+   when the norm dropped by more than the "twice is enough" factor, 2: always
+   two passes; normalise into V_nv and write the Hessenberg column
+   h_0..h_{nv-1}, h_nv = |w| to d_hcol (device, nv + 1 doubles).  w is
+   overwritten.  Everything is enqueued on `stream`; no host sync. */
+extern "C" int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n, double *d_w,
+                                double *d_hcol, double *d_work, int refine, void *stream) {
+    return arnoldi_step<double>(d_V, ld, nv, n, d_w, d_hcol, d_work, refine, stream);
+}
+
+/* The same step on an FP32 basis (dot products, norms and the Hessenberg
+   column accumulated in FP64). */
+extern "C" int rtx_arnoldi_step_f32(const float *d_V, int64_t ld, int nv, int64_t n, float *d_w,
+                                    double *d_hcol, double *d_work, int refine, void *stream) {
+    return arnoldi_step<float>(d_V, ld, nv, n, d_w, d_hcol, d_work, refine, stream);
 }
 
 extern "C" int rtx_gmres_state_size(int m, int64_t *n_doubles) {
@@ -311,7 +347,7 @@ extern "C" int rtx_mdot(const double *d_V, int64_t ld, int nv, int64_t n, const 
     RTX_CHECK(d_V && d_w && d_out && d_work && nv >= 1 && nv < KMAXV, "rtx_mdot: bad argument");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int nchunk = (nv + KCH - 1) / KCH;
-    k_mdot<<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, d_work, 0, nullptr);
+    k_mdot<double><<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, d_work, 0, nullptr);
     k_reduce<<<nv, KT, 0, st>>>(d_work, KNB, d_out, 0, nullptr);
     RTX_HIP(hipGetLastError());
     return RTX_OK;
@@ -321,8 +357,43 @@ extern "C" int rtx_mdot(const double *d_V, int64_t ld, int nv, int64_t n, const 
 extern "C" int rtx_combine(const double *d_V, int64_t ld, int nv, const double *d_y, int64_t n,
                            double *d_x, int accumulate, void *stream) {
     RTX_CHECK(d_V && d_y && d_x && nv >= 1 && nv < KMAXV, "rtx_combine: bad argument");
-    k_combine<<<2048, KT, 0, static_cast<hipStream_t>(stream)>>>(d_V, ld, nv, d_y, d_x, n,
-                                                               accumulate);
+    k_combine<double, double><<<2048, KT, 0, static_cast<hipStream_t>(stream)>>>(
+        d_V, ld, nv, d_y, d_x, n, accumulate);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+/* d_x (+)= sum_j y_j V_j for an FP32 basis; d_x is FP64 (x_precision 0) or
+   FP32 (x_precision 1); accumulation in FP64. */
+extern "C" int rtx_combine_f32(const float *d_V, int64_t ld, int nv, const double *d_y,
+                               int64_t n, void *d_x, int x_precision, int accumulate,
+                               void *stream) {
+    RTX_CHECK(d_V && d_y && d_x && nv >= 1 && nv < KMAXV && (x_precision == 0 || x_precision == 1),
+              "rtx_combine_f32: bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (x_precision == 0)
+        k_combine<float, double><<<2048, KT, 0, st>>>(d_V, ld, nv, d_y, (double *)d_x, n,
+                                                      accumulate);
+    else
+        k_combine<float, float><<<2048, KT, 0, st>>>(d_V, ld, nv, d_y, (float *)d_x, n,
+                                                     accumulate);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+/* y = a x + b y with x FP32 and y FP64 (direction 0) or x FP64 and y FP32
+   (direction 1). */
+extern "C" int rtx_axpby_mixed(int direction, int64_t n, double a, const void *d_x, double b,
+                               void *d_y, void *stream) {
+    RTX_CHECK(d_x && d_y && n >= 0 && (direction == 0 || direction == 1),
+              "rtx_axpby_mixed: bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (direction == 0)
+        k_axpby_mixed<float, double><<<2048, 256, 0, st>>>(n, a, (const float *)d_x, b,
+                                                           (double *)d_y);
+    else
+        k_axpby_mixed<double, float><<<2048, 256, 0, st>>>(n, a, (const double *)d_x, b,
+                                                           (float *)d_y);
     RTX_HIP(hipGetLastError());
     return RTX_OK;
 }

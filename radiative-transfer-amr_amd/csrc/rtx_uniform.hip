@@ -50,12 +50,15 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
                                                // face pair; 4 leaves room for hp-free meshes)
     constexpr int TRL = N * NTH;               // trace values per link
     constexpr int XU = (NCD + UTHREADS - 1) / UTHREADS;
+    // LDS rows per spatial node, angle-contiguous, odd strides so 16 lanes on
+    // 16 consecutive nodes hit distinct banks
+    constexpr int NAP = NA | 1, NOAP = NOA | 1;
     static_assert(NTILE <= UTHREADS / 64, "one output-angle tile per wavefront");
     static_assert(NC % H == 0, "parts must split the cells evenly");
     extern __shared__ __align__(16) unsigned char u_lds[];
-    A *xs = reinterpret_cast<A *>(u_lds);      // NCD
-    A *ys = xs + NCD;                          // NOWN
-    A *tr = ys + NOWN;                         // MAXL * TRL
+    A *xs = reinterpret_cast<A *>(u_lds);      // [NSP][NAP]: column x, angle A of node s
+    A *ys = xs + NSP * NAP;                    // [NSP][NOAP]: part outputs
+    A *tr = ys + NSP * NOAP;                   // MAXL * TRL
     __shared__ A cth[NA], sth[NA], cw[NA];
     __shared__ A kap[NSP], sgw[NSP];
     __shared__ A wx_[N], WD[N * N];
@@ -162,11 +165,9 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
         const int t = tid + u * UTHREADS;
         if (t < NCD) {
             A v = xv[u];
-            if (restrict_) {
-                const int kk = t / CD, lt = t % CD;
-                if (cbd[kk] && u_on_bdry(cbd[kk], lt / (N * NTH), (lt / NTH) % N, N)) v = A(0);
-            }
-            xs[t] = v;
+            const int kk = t / CD, lt = t % CD, sn = lt / NTH, a = lt % NTH;
+            if (restrict_ && cbd[kk] && u_on_bdry(cbd[kk], sn / N, sn % N, N)) v = A(0);
+            xs[sn * NAP + kk * NTH + a] = v;
         }
     }
     if (bconv) {
@@ -203,54 +204,67 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     }
     __syncthreads();
 
-    // ---- phase 2: cell-local terms on the part's DOFs; a thread's elements
-    // are computed together (all LDS reads before any LDS write) so their
-    // loads overlap
+    // ---- phase 2: cell-local terms on the part's DOFs, by lines of nodes:
+    // pass x: one (cell, q, angle) line per thread reads x[0..N-1][q] once
+    // and writes mass, outflow faces and the x-derivative term of all N
+    // nodes; pass y: one (cell, p, angle) line adds the y-derivative term
     {
-        constexpr int EU = (NOWN + UTHREADS - 1) / UTHREADS;
-        A accs[EU];
+        constexpr int NLINE = OC * N * NTH;
+        const A cm = A(dx * dy * 0.125);
+        for (int e = tid; e < NLINE; e += UTHREADS) {
+            const int kl = e / (N * NTH), q = (e / NTH) % N, r = e % NTH;
+            const int kk = kk_lo + kl, ca = kk * NTH + r, oa = kl * NTH + r;
+            const A dth = dth_[kk], wr = cw[ca], c_ = cth[ca], s_ = sth[ca];
+            const int quad = cqd[kk];
+            A xl[N];
 #pragma unroll
-        for (int u = 0; u < EU; ++u) {
-            const int e = tid + u * UTHREADS;
-            A acc = A(0);
-            if (e < NOWN) {
-                const int kk = kk_lo + e / CD, lt = e % CD;
-                const int p = lt / (N * NTH), q = (lt / NTH) % N, r = lt % NTH;
-                const int ca = kk * NTH + r;
-                const A dth = dth_[kk], wr = cw[ca];
-                const A *xc = xs + kk * CD;
-                if (ops & RTX_OP_MASS) acc += A(dx * dy * 0.125) * dth * wx_[p] * wx_[q] * wr * kap[p * N + q] * xc[lt];
-                if (ops & RTX_OP_QUADW) acc += A(dx * dy * 0.125) * dth * wx_[p] * wx_[q] * wr * xc[lt];
+            for (int i = 0; i < N; ++i) xl[i] = xs[(i * N + q) * NAP + ca];
+            const A cx = A(dy * 0.25) * dth * wx_[q] * wr * c_;
+#pragma unroll
+            for (int pp = 0; pp < N; ++pp) {
+                A acc = A(0);
+                const A xn = xl[pp];
+                if (ops & RTX_OP_MASS) acc += cm * dth * wx_[pp] * wx_[q] * wr * kap[pp * N + q] * xn;
+                if (ops & RTX_OP_QUADW) acc += cm * dth * wx_[pp] * wx_[q] * wr * xn;
                 if (ops & RTX_OP_ICONV) {
-                    A sx = A(0), sy = A(0);
+                    A sx = A(0);
 #pragma unroll
-                    for (int i = 0; i < N; ++i) sx += WD[p * N + i] * xc[(i * N + q) * NTH + r];
-#pragma unroll
-                    for (int j = 0; j < N; ++j) sy += WD[q * N + j] * xc[(p * N + j) * NTH + r];
-                    acc -= A(dy * 0.25) * dth * wx_[q] * wr * cth[ca] * sx +
-                           A(dx * 0.25) * dth * wx_[p] * wr * sth[ca] * sy;
+                    for (int i = 0; i < N; ++i) sx += WD[pp * N + i] * xl[i];
+                    acc -= cx * sx;
                 }
-                const int quad = cqd[kk];
                 if (bconv && quad >= 0) {
 #pragma unroll
                     for (int hh = 0; hh < 2; ++hh) {
                         const int F = (quad + hh) & 3;
-                        const bool on = (F == 0 && p == N - 1) || (F == 2 && p == 0) ||
+                        const bool on = (F == 0 && pp == N - 1) || (F == 2 && pp == 0) ||
                                         (F == 1 && q == N - 1) || (F == 3 && q == 0);
                         if (on) {
-                            const A c_ = cth[ca], s_ = sth[ca];
                             const A tf = F == 0 ? c_ : (F == 1 ? s_ : (F == 2 ? -c_ : -s_));
-                            acc += A(((F & 1) ? dx : dy) * 0.25) * dth * ((F & 1) ? wx_[p] : wx_[q]) * wr * tf * xc[lt];
+                            acc += A(((F & 1) ? dx : dy) * 0.25) * dth *
+                                   ((F & 1) ? wx_[pp] : wx_[q]) * wr * tf * xn;
                         }
                     }
                 }
+                ys[(pp * N + q) * NOAP + oa] = acc;
             }
-            accs[u] = acc;
         }
+        if (ops & RTX_OP_ICONV) {
+            __syncthreads();
+            for (int e = tid; e < NLINE; e += UTHREADS) {
+                const int kl = e / (N * NTH), pp = (e / NTH) % N, r = e % NTH;
+                const int kk = kk_lo + kl, ca = kk * NTH + r, oa = kl * NTH + r;
+                A xl[N];
 #pragma unroll
-        for (int u = 0; u < EU; ++u) {
-            const int e = tid + u * UTHREADS;
-            if (e < NOWN) ys[e] = accs[u];
+                for (int j = 0; j < N; ++j) xl[j] = xs[(pp * N + j) * NAP + ca];
+                const A cy = A(dx * 0.25) * dth_[kk] * wx_[pp] * cw[ca] * sth[ca];
+#pragma unroll
+                for (int q = 0; q < N; ++q) {
+                    A sy = A(0);
+#pragma unroll
+                    for (int j = 0; j < N; ++j) sy += WD[q * N + j] * xl[j];
+                    ys[(pp * N + q) * NOAP + oa] -= cy * sy;
+                }
+            }
         }
     }
     __syncthreads();
@@ -278,12 +292,13 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
                 for (int a = 0; a < NTH; ++a) inner += et[a] * trl[m1 * NTH + a];
                 acc += es[m1] * inner;
             }
-            int node;
-            if (F == 0) node = ((N - 1) * N + m0) * NTH + r;
-            else if (F == 2) node = m0 * NTH + r;
-            else if (F == 1) node = (m0 * N + N - 1) * NTH + r;
-            else node = (m0 * N) * NTH + r;
-            atomicAdd(&ys[(kk - kk_lo) * CD + node], A(((F & 1) ? dx : dy) * 0.25) * dth_[kk] * acc);
+            int sn;
+            if (F == 0) sn = (N - 1) * N + m0;
+            else if (F == 2) sn = m0;
+            else if (F == 1) sn = m0 * N + N - 1;
+            else sn = m0 * N;
+            atomicAdd(&ys[sn * NOAP + (kk - kk_lo) * NTH + r],
+                      A(((F & 1) ? dx : dy) * 0.25) * dth_[kk] * acc);
         }
     }
     __syncthreads();
@@ -303,11 +318,10 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
 #pragma unroll
             for (int ks = 0; ks < NKS; ++ks) {
                 const int Ak = ks * 4 + lg;
-                const int kcell = Ak / NTH, kloc = Ak % NTH;
 #pragma unroll
                 for (int t = 0; t < MT; ++t) {
                     const int s = t * 16 + l16;
-                    const EB a = (Ak < NA && s < NSP) ? EB(xs[kcell * CD + s * NTH + kloc]) : EB(0);
+                    const EB a = (Ak < NA && s < NSP) ? EB(xs[s * NAP + Ak]) : EB(0);
                     acc4 &dst = (ks & 1) ? acc2[t] : acc[t];
                     if constexpr (PREC == RTX_PREC_F64)
                         dst = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bpre[ks], dst, 0, 0, 0);
@@ -318,13 +332,12 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
 #pragma unroll
             for (int t = 0; t < MT; ++t) acc[t] += acc2[t];
             if (rok) {
-                const int ocell = Rl / NTH, oloc = Rl % NTH;
 #pragma unroll
                 for (int t = 0; t < MT; ++t) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int s = t * 16 + ((PREC == RTX_PREC_F64) ? (lg + 4 * v) : (4 * lg + v));
-                        if (s < NSP) ys[ocell * CD + s * NTH + oloc] -= sgw[s] * A(acc[t][v]);
+                        if (s < NSP) ys[s * NOAP + Rl] -= sgw[s] * A(acc[t][v]);
                     }
                 }
             }
@@ -350,21 +363,19 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const int Ak = ks * 16 + 4 * lg + u;
-                        av[u] = (Ak < NA && s < NSP)
-                                    ? short(u_bf16(float(xs[(Ak / NTH) * CD + s * NTH + Ak % NTH])))
-                                    : short(0);
+                        av[u] = (Ak < NA && s < NSP) ? short(u_bf16(float(xs[s * NAP + Ak])))
+                                                     : short(0);
                     }
                     acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[ks], acc[t], 0, 0, 0);
                 }
             }
             if (rok) {
-                const int ocell = Rl / NTH, oloc = Rl % NTH;
 #pragma unroll
                 for (int t = 0; t < MT; ++t) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int s = t * 16 + 4 * lg + v;
-                        if (s < NSP) ys[ocell * CD + s * NTH + oloc] -= sgw[s] * A(acc[t][v]);
+                        if (s < NSP) ys[s * NOAP + Rl] -= sgw[s] * A(acc[t][v]);
                     }
                 }
             }
@@ -375,12 +386,9 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     // ---- phase 5: write back the part's DOFs
     for (int e = tid; e < NOWN; e += UTHREADS) {
         const int t = own0 + e;
-        bool masked = false;
-        if (restrict_) {
-            const int kk = t / CD, lt = t % CD;
-            masked = cbd[kk] && u_on_bdry(cbd[kk], lt / (N * NTH), (lt / NTH) % N, N);
-        }
-        const A v = masked ? A(0) : ys[e];
+        const int kk = t / CD, lt = t % CD, sn = lt / NTH, a = lt % NTH;
+        const bool masked = restrict_ && cbd[kk] && u_on_bdry(cbd[kk], sn / N, sn % N, N);
+        const A v = masked ? A(0) : ys[sn * NOAP + (kk - kk_lo) * NTH + a];
         if (accumulate) {
             if (!masked) y[cdof + t] = T(A(y[cdof + t]) + v);
         } else {
@@ -391,12 +399,13 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
 
 template <typename T, typename A, int PREC, int N, int NTH, int NC, int H>
 int launch_one(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t st, int acc) {
-    constexpr int NSP = N * N, NA = NC * NTH, NCD = NSP * NA, CD = NSP * NTH;
-    constexpr int NOWN = (NC / H) * CD, MAXL = 4 * (NC / H), TRL = N * NTH;
+    constexpr int NSP = N * N, NA = NC * NTH;
+    constexpr int NAP = NA | 1, NOAP = ((NC / H) * NTH) | 1;
+    constexpr int MAXL = 4 * (NC / H), TRL = N * NTH;
     RTX_CHECK(p->max_part_links[H] <= MAXL, "uniform apply: %d links in a column part (max %d)",
               p->max_part_links[H], MAXL);
-    // trace slots for the largest link count of any part
-    const size_t lds = size_t(NCD + NOWN + p->max_part_links[H] * TRL) * sizeof(A);
+    // column and output rows, then trace slots for the largest link count of any part
+    const size_t lds = size_t(NSP * NAP + NSP * NOAP + p->max_part_links[H] * TRL) * sizeof(A);
     auto kern = k_uniform<T, A, PREC, N, NTH, NC, H>;
     RTX_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 int(lds)));

@@ -74,8 +74,14 @@ SIGNATURES = [
                                    C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     ("rtx_mdot", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_int64, C.c_void_p, C.c_void_p,
                            C.c_void_p, C.c_void_p]),
+    ("rtx_arnoldi_step_f32", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_int64, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     ("rtx_combine", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64,
                               C.c_void_p, C.c_int, C.c_void_p]),
+    ("rtx_combine_f32", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64,
+                                  C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    ("rtx_axpby_mixed", C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_void_p, C.c_double,
+                                  C.c_void_p, C.c_void_p]),
     ("rtx_axpby", C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_void_p, C.c_double,
                             C.c_void_p, C.c_void_p]),
     ("rtx_convert", C.c_int, [C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -120,6 +120,23 @@ class InteriorSystem:
         check(self.lib.rtx_convert(1, r.numel(), _vp(self.y32), _vp(z), st), "convert")
         return z
 
+    def matvec_lo(self, x, y):
+        """y = A_II x on FP32 vectors, the apply in this system's reduced
+        precision (FP32 basis of the inner solves)."""
+        if self.precision == PREC_F64:
+            raise ValueError("an FP32 basis needs a reduced-precision system")
+        self.op.apply(x, y, ops=OP_SYSTEM | OP_RESTRICT, precision=self.precision)
+        return y
+
+    def precondition_lo(self, r, z):
+        """z = M r on FP32 vectors (inverses in the preconditioner precision)."""
+        if self.pc is None:
+            z.copy_(r)
+            return z
+        if self.pc_precision == PREC_F64:
+            raise ValueError("an FP32 basis needs a reduced-precision preconditioner")
+        return self.pc.apply(r, z, self.pc_precision)
+
     def residual(self, b, x, r):
         """r = b - A_II x (always FP64)."""
         self.op.apply(x, r, ops=OP_SYSTEM | OP_RESTRICT, precision=PREC_F64)
@@ -178,18 +195,26 @@ class GMRES:
     the chunk and by ``system.graph_key()``, so a change of preconditioner
     configuration never replays a stale graph.  ``refine`` selects the
     Gram-Schmidt refinement of ``rtx_arnoldi_step`` (0: none, PETSc's
-    default; 1: when needed; 2: always)."""
+    default; 1: when needed; 2: always).  ``basis`` "f32" keeps the basis
+    (V, Z and the work vectors) in FP32 for a reduced-precision system: the
+    operator and preconditioner then act on it directly and the
+    orthogonalisation moves half the bytes; dot products, the Hessenberg
+    matrix, the iterate and the true residual stay FP64."""
 
     def __init__(self, system, n, device, restart=50, flexible=False, chunk=10, graphs=True,
-                 refine=0):
+                 refine=0, basis="f64"):
+        if basis not in ("f64", "f32"):
+            raise ValueError("basis must be 'f64' or 'f32'")
         self.system, self.lib = system, system.lib
         self.n, self.m = int(n), int(restart)
         self.flexible, self.chunk, self.graphs, self.refine = flexible, int(chunk), graphs, refine
+        self.basis = basis
         m, dev = self.m, device
-        self.V = torch.empty((m + 1, n), dtype=torch.float64, device=dev)
-        self.Z = torch.empty((m, n), dtype=torch.float64, device=dev) if flexible else None
-        self.w = torch.empty(n, dtype=torch.float64, device=dev)
-        self.z = torch.empty(n, dtype=torch.float64, device=dev)
+        bt = torch.float32 if basis == "f32" else torch.float64
+        self.V = torch.empty((m + 1, n), dtype=bt, device=dev)
+        self.Z = torch.empty((m, n), dtype=bt, device=dev) if flexible else None
+        self.w = torch.empty(n, dtype=bt, device=dev)
+        self.z = torch.empty(n, dtype=bt, device=dev)
         self.r = torch.empty(n, dtype=torch.float64, device=dev)
         ws = C.c_int64()
         check(self.lib.rtx_arnoldi_workspace(m + 1, C.byref(ws)), "workspace")
@@ -208,14 +233,18 @@ class GMRES:
     def _steps(self, j0, j1):
         lib, st = self.lib, stream_handle()
         V, Z, w, z, n, m = self.V, self.Z, self.w, self.z, self.n, self.m
+        f32 = self.basis == "f32"
+        pre = self.system.precondition_lo if f32 else self.system.precondition
+        mv = self.system.matvec_lo if f32 else self.system.matvec
+        step = lib.rtx_arnoldi_step_f32 if f32 else lib.rtx_arnoldi_step
         for j in range(j0, j1):
             zj = Z[j] if self.flexible else z
-            self.system.precondition(V[j], zj)
-            self.system.matvec(zj, w)
+            pre(V[j], zj)
+            mv(zj, w)
             if self._on_direction is not None:
                 self._on_direction(zj, w)
-            check(lib.rtx_arnoldi_step(_vp(V), n, j + 1, n, _vp(w), _vp(self.hcol),
-                                       _vp(self.work), self.refine, st), "rtx_arnoldi_step")
+            check(step(_vp(V), n, j + 1, n, _vp(w), _vp(self.hcol), _vp(self.work), self.refine,
+                       st), "rtx_arnoldi_step")
             check(lib.rtx_gmres_givens(j, m, _vp(self.hcol), _vp(self.state), st),
                   "rtx_gmres_givens")
 
@@ -257,7 +286,11 @@ class GMRES:
                 reason = -9                                   # KSP_DIVERGED_NANORINF
                 break
             st = stream_handle()
-            check(lib.rtx_axpby(PREC_F64, n, 1.0 / beta, _vp(r), 0.0, _vp(V[0]), st), "axpby")
+            if self.basis == "f32":
+                check(lib.rtx_axpby_mixed(1, n, 1.0 / beta, _vp(r), 0.0, _vp(V[0]), st), "axpby")
+            else:
+                check(lib.rtx_axpby(PREC_F64, n, 1.0 / beta, _vp(r), 0.0, _vp(V[0]), st),
+                      "axpby")
             check(lib.rtx_gmres_begin(m, beta, _vp(self.state), st), "rtx_gmres_begin")
             gkey = system.graph_key() if hasattr(system, "graph_key") else None
             j, k, stop = 0, 0, None
@@ -288,7 +321,16 @@ class GMRES:
             if k > 0:
                 check(lib.rtx_gmres_solve(k, m, _vp(self.state), _vp(self.ydev), st),
                       "rtx_gmres_solve")
-                if self.flexible:
+                if self.basis == "f32":
+                    if self.flexible:
+                        check(lib.rtx_combine_f32(_vp(Z), n, k, _vp(self.ydev), n, _vp(x), 0, 1,
+                                                  st), "combine")
+                    else:
+                        check(lib.rtx_combine_f32(_vp(V), n, k, _vp(self.ydev), n, _vp(w), 1, 0,
+                                                  st), "combine")
+                        system.precondition_lo(w, z)
+                        check(lib.rtx_axpby_mixed(0, n, 1.0, _vp(z), 1.0, _vp(x), st), "axpby")
+                elif self.flexible:
                     check(lib.rtx_combine(_vp(Z), n, k, _vp(self.ydev), n, _vp(x), 1, st),
                           "combine")
                 else:
@@ -319,10 +361,10 @@ class GMRES:
 
 def gmres(system, b, x=None, rtol=1e-8, atol=0.0, restart=50, max_it=5000,
           flexible=False, callback=None, true_residual_every_restart=True, on_restart=None,
-          on_direction=None, chunk=10, graphs=True, refine=0):
+          on_direction=None, chunk=10, graphs=True, refine=0, basis="f64"):
     """One solve with a fresh ``GMRES`` object (see there)."""
     solver = GMRES(system, b.numel(), b.device, restart=restart, flexible=flexible, chunk=chunk,
-                   graphs=graphs, refine=refine)
+                   graphs=graphs, refine=refine, basis=basis)
     return solver.solve(b, x, rtol=rtol, atol=atol, max_it=max_it, callback=callback,
                         on_restart=on_restart, on_direction=on_direction)
 
@@ -338,11 +380,12 @@ def hessenberg_singular_values(H, k):
 
 def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=50,
            max_outer=50, max_inner=200, flexible=False, after_outer=None, chunk=10,
-           graphs=True, inner=None):
+           graphs=True, inner=None, basis="f32"):
     """FP64 iterative refinement: r = b - A x in FP64 (``system_hi``), inner
     correction solve A d = r by GMRES whose operator and preconditioner run in
-    reduced precision (``system_lo``; its Arnoldi is still FP64).  One GMRES
-    object serves every inner solve, so its graphs are captured once.
+    reduced precision (``system_lo``) on a ``basis`` "f32" (default) or "f64"
+    Arnoldi basis.  One GMRES object serves every inner solve, so its graphs
+    are captured once.
     ``flexible`` selects flexible inner GMRES (for a nonlinear
     preconditioner); ``after_outer(outer, |r|)`` sees every FP64 residual;
     ``inner`` may pass a GMRES object on ``system_lo`` to reuse (with its
@@ -361,7 +404,7 @@ def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=
     reason = -3
     if inner is None:
         inner = GMRES(system_lo, n, b.device, restart=restart, flexible=flexible, chunk=chunk,
-                      graphs=graphs)
+                      graphs=graphs, basis=basis)
     d = torch.empty_like(b)
     for outer in range(max_outer + 1):
         system_hi.residual(b, x, r)

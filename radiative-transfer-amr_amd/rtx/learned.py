@@ -524,11 +524,12 @@ class Session:
 
     precision "mixed": FP64 iterative refinement around flexible GMRES with
     FP32 / BF16 applies and block-Jacobi inverses stored in ``bj_precision``
-    (PREC_MIXED: BF16, PREC_F32: FP32); "fp64": GMRES in FP64.  ``pc`` is a
-    TwoLevelPC or None (plain block-Jacobi)."""
+    (PREC_MIXED: BF16, PREC_F32: FP32) on an FP32 (``basis`` "f32") or
+    FP64 Arnoldi basis; "fp64": GMRES in FP64.  ``pc`` is a TwoLevelPC or
+    None (plain block-Jacobi)."""
 
     def __init__(self, op, pc=None, bj=None, precision="mixed", restart=50,
-                 bj_precision=PREC_MIXED):
+                 bj_precision=PREC_MIXED, basis="f32"):
         if precision not in ("mixed", "fp64"):
             raise ValueError("precision must be 'mixed' or 'fp64'")
         if bj is None:
@@ -545,7 +546,8 @@ class Session:
         else:
             self.hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
             self.lo = InteriorSystem(op, PREC_MIXED, the_pc, bj_precision)
-            self.gm = GMRES(self.lo, op.ndof, op.device, restart=restart, flexible=flexible)
+            self.gm = GMRES(self.lo, op.ndof, op.device, restart=restart, flexible=flexible,
+                            basis=basis)
 
     def solve(self, rhs, x0=None, rtol=1e-8, max_it=5000, inner_rtol=1e-2, max_outer=60,
               judge=True):

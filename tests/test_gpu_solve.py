@@ -109,7 +109,7 @@ def test_refinement_mixed_reaches_fp64_residual(cuda):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant", ["plain", "flexible", "mixed"])
+@pytest.mark.parametrize("variant", ["plain", "flexible", "mixed", "mixed_f64_basis"])
 def test_graph_replay_matches_eager(cuda, variant):
     """Chunks of GMRES iterations replayed as HIP graphs reproduce the eager
     run exactly (same kernels, same order): iterate, residual history and
@@ -126,11 +126,13 @@ def test_graph_replay_matches_eager(cuda, variant):
     rhs = op.mask(_dev(np.random.default_rng(4).uniform(-1, 1, op.ndof)))
     runs = []
     for graphs in (False, True, True):
-        if variant == "mixed":
+        if variant.startswith("mixed"):
             hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
             lo = InteriorSystem(op, PREC_MIXED, pc, PREC_F32)
             x, info = refine(hi, lo, rhs, None, rtol=1e-10, inner_rtol=1e-2, restart=7,
-                             max_inner=500, chunk=3, graphs=graphs)
+                             max_inner=500, chunk=3, graphs=graphs,
+                             basis="f64" if variant == "mixed_f64_basis" else "f32")
+            assert info["converged_reason"] > 0
             runs.append((x.cpu().numpy(), info["residual_history"], info["iteration_number"]))
         else:
             system = InteriorSystem(op, PREC_F64, pc, PREC_F64)
@@ -193,3 +195,65 @@ def test_block_jacobi_split_cells_match_oracle(cuda):
     r32 = _dev(r).float()
     z16 = pc.apply(r32, torch.zeros_like(r32), precision=2).cpu().double().numpy()
     assert np.linalg.norm(z16 - z_ref) / np.linalg.norm(z_ref) < 2e-2
+
+
+@pytest.mark.gpu
+def test_arnoldi_fp32_basis(cuda):
+    """Arnoldi steps on an FP32 basis: the basis stays orthonormal to FP32
+    accuracy, every input vector is reproduced by its Hessenberg column
+    (w_j = sum_i h_ij v_i), and the columns agree with the FP64 step's."""
+    import ctypes as C
+    import torch
+    from rtx import _lib
+    from rtx.krylov import _vp
+    lib = _lib.load()
+    n, k = 200_003, 12
+    rng = np.random.default_rng(5)
+    W = rng.normal(size=(k, n))
+    ws = C.c_int64()
+    _lib.check(lib.rtx_arnoldi_workspace(k + 1, C.byref(ws)), "workspace")
+    work = torch.empty(int(ws.value), dtype=torch.float64, device="cuda")
+    out = {}
+    for dt, step in ((torch.float32, lib.rtx_arnoldi_step_f32),
+                     (torch.float64, lib.rtx_arnoldi_step)):
+        V = torch.zeros((k + 1, n), dtype=dt, device="cuda")
+        V[0] = torch.from_numpy(W[0] / np.linalg.norm(W[0])).to(dt)
+        Hs = []
+        for j in range(1, k):
+            w = torch.from_numpy(W[j]).to(dt).cuda()
+            hcol = torch.zeros(k + 2, dtype=torch.float64, device="cuda")
+            _lib.check(step(_vp(V), n, j, n, _vp(w), _vp(hcol), _vp(work), 0, None), "step")
+            Hs.append(hcol[:j + 1].cpu().numpy())
+        torch.cuda.synchronize()
+        Vh = V[:k].double().cpu().numpy()
+        out[dt] = (Vh, Hs)
+    Vh, Hs = out[torch.float32]
+    G = Vh @ Vh.T
+    assert np.abs(G - np.eye(k)).max() < 1e-5
+    for j in range(1, k):
+        rec = Hs[j - 1] @ Vh[:j + 1]
+        assert np.linalg.norm(rec - W[j]) < 1e-5 * np.linalg.norm(W[j])
+        ref = out[torch.float64][1][j - 1]
+        assert np.abs(Hs[j - 1] - ref).max() < 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_refine_fp32_basis_matches_spsolve(cuda):
+    """Mixed iterative refinement with an FP32 inner basis reaches the same
+    discrete solution as spsolve (FP64 outer residual 1e-10)."""
+    from rtx.operator import TransportOperator
+    from rtx.krylov import BlockJacobi, InteriorSystem, refine
+    from rtx._lib import PREC_F64, PREC_MIXED
+    mesh, wl = _c1_small()
+    A, f, b, mask = _oracle_system(mesh, wl)
+    u_ref, _, _ = osys.exact_solve(A, f, b, mask)
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    pc = BlockJacobi(op, keep_f64=False, keep_bf16=True)
+    hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
+    lo = InteriorSystem(op, PREC_MIXED, pc, PREC_MIXED)
+    bb = np.where(mask, 0.0, b)
+    rhs_full = np.where(mask, f - A @ bb, 0.0)
+    x, info = refine(hi, lo, _dev(rhs_full), rtol=1e-10, inner_rtol=1e-3, basis="f32")
+    assert info["converged_reason"] > 0 and info["rel_residual"] <= 1e-10
+    u = np.where(mask, x.cpu().numpy(), b)
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < TOL_SOL
