@@ -208,10 +208,17 @@ int rtx_bjacobi_factor(rtx_plan *plan, int restrict_interior, double *d_inv, flo
    Gauss-Jordan with explicit row interchanges, 1 register-resident sweeps
    (fails for larger blocks). */
 int rtx_bjacobi_set_method(rtx_plan *plan, int method);
+/* Entries (uint16) of the packed BF16 inverse pool: every block's columns
+   padded to a multiple of 8 rows so each starts on 16 bytes. */
+int rtx_bjacobi_size_bf16(rtx_plan *plan, int64_t *n_entries);
+/* Round the FP32 inverses (from rtx_bjacobi_factor) to BF16, nearest even,
+   into the packed pool. */
+int rtx_bjacobi_pack_bf16(rtx_plan *plan, const float *d_inv32, uint16_t *d_inv16,
+                          void *stream);
 /* z = blockdiag(inv) r, precision RTX_PREC_F64 (double inverses, vectors),
-   RTX_PREC_F32 (float inverses, vectors) or RTX_PREC_MIXED (BF16 inverses -
-   rtx_mlp_to_bf16 of the FP32 copy - and float vectors; the factorisation
-   itself is always FP64). */
+   RTX_PREC_F32 (float inverses, vectors) or RTX_PREC_MIXED (the packed BF16
+   pool of rtx_bjacobi_pack_bf16, float vectors, FP32 accumulation; the
+   factorisation itself is always FP64). */
 int rtx_bjacobi_apply(rtx_plan *plan, int precision, const void *d_inv, const void *d_r,
                       void *d_z, void *stream);
 

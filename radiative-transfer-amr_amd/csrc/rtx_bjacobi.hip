@@ -484,11 +484,6 @@ __global__ void k_to_f32_bj(const double *__restrict__ s, float *__restrict__ d,
 // z_K = inv_K r_K, inv column-major.  One workgroup per cell; the columns
 // are split across the four wavefronts' partial sums when n is small.
 // BF16 storage of an inverse entry, widened on load.
-struct bj_bf16 {
-    uint16_t v;
-    __device__ operator double() const { return double(__uint_as_float(uint32_t(v) << 16)); }
-};
-
 template <typename T, typename M>
 __global__ __launch_bounds__(BJ_THREADS) void k_bj_apply(DevMesh m, const int64_t *inv_off,
                                                         const int32_t *__restrict__ blk,
@@ -529,6 +524,81 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_apply(DevMesh m, const int64_
     }
 }
 
+__device__ inline uint16_t bj_to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+__device__ inline int bj_block_n(const DevMesh &m, const int32_t *blk, int b) {
+    const int k = blk[3 * b], g = blk[3 * b + 2];
+    const int c = m.cell_col[k];
+    return m.col_nx[c] * m.col_ny[c] * g;
+}
+
+// Packed BF16 copy: column j of block b at off16[b] + j * ldp, ldp = n
+// rounded up to 8 (padding rows zero), so every column starts on 16 bytes.
+__global__ __launch_bounds__(BJ_THREADS) void k_bj_pack_bf16(DevMesh m, const int64_t *off,
+                                                             const int64_t *off16,
+                                                             const int32_t *__restrict__ blk,
+                                                             const float *__restrict__ src,
+                                                             uint16_t *__restrict__ dst) {
+    const int b = blockIdx.x;
+    const int n = bj_block_n(m, blk, b), ldp = (n + 7) / 8 * 8;
+    const float *S = src + off[b];
+    uint16_t *D = dst + off16[b];
+    for (int e = threadIdx.x; e < n * ldp; e += BJ_THREADS) {
+        const int j = e / ldp, i = e % ldp;
+        D[e] = i < n ? bj_to_bf16(S[size_t(j) * n + i]) : uint16_t(0);
+    }
+}
+
+// z = B r for one block from the packed BF16 inverse: thread (cg, rg) owns
+// rows 8 rg .. 8 rg + 7 and columns cg, cg + CG, ...; one 16-byte load per
+// column; the CG partial sums of each row are added in LDS.
+__global__ __launch_bounds__(BJ_THREADS) void k_bj_apply_bf16(DevMesh m, const int64_t *off16,
+                                                              const int32_t *__restrict__ blk,
+                                                              const uint16_t *__restrict__ inv,
+                                                              const float *__restrict__ r,
+                                                              float *__restrict__ z) {
+    __shared__ float rs[RTX_NMAX * RTX_NMAX * RTX_NMAX];
+    __shared__ float part[BJ_THREADS * 8];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int k = blk[3 * b], a0 = blk[3 * b + 1], g = blk[3 * b + 2];
+    const int c = m.cell_col[k];
+    const int nth = m.cell_nth[k];
+    const int n = m.col_nx[c] * m.col_ny[c] * g;
+    const int ldp = (n + 7) / 8 * 8, RG = ldp / 8, CG = BJ_THREADS / RG;
+    const int off = m.cell_dof_off[k] + a0;
+    for (int i = tid; i < n; i += BJ_THREADS) rs[i] = r[off + (i / g) * nth + i % g];
+    __syncthreads();
+    const int rg = tid % RG, cg = tid / RG;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (cg < CG) {
+        const uint4 *B = reinterpret_cast<const uint4 *>(inv + off16[b]) + rg;
+        const int ldq = ldp / 8;
+#pragma unroll 4
+        for (int j = cg; j < n; j += CG) {
+            const uint4 q = B[size_t(j) * ldq];
+            const float rj = rs[j];
+            const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc[2 * u] += __uint_as_float(w[u] << 16) * rj;
+                acc[2 * u + 1] += __uint_as_float(w[u] & 0xffff0000u) * rj;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) part[cg * ldp + rg * 8 + u] = acc[u];
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += BJ_THREADS) {
+        float s = 0.f;
+        for (int q = 0; q < CG; ++q) s += part[q * ldp + i];
+        z[off + (i / g) * nth + i % g] = s;
+    }
+}
+
 }  // namespace rtx
 
 using namespace rtx;
@@ -542,7 +612,7 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
     RTX_HIP(hipMemcpy(nt.data(), p->m.cell_nth, nt.size() * 4, hipMemcpyDeviceToHost));
     const int split = (p->bj_method != 0) ? 1 : 0;
     std::vector<int32_t> blk;
-    std::vector<int64_t> off(1, 0);
+    std::vector<int64_t> off(1, 0), off16(1, 0);
     int32_t maxb = 0;
     for (int k = 0; k < p->m.ncell; ++k) {
         const int nsp = nx[cc[k]] * ny[cc[k]], nth = nt[k];
@@ -554,16 +624,21 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
             blk.push_back(a0);
             blk.push_back(gg);
             off.push_back(off.back() + n * n);
+            off16.push_back(off16.back() + n * ((n + 7) / 8 * 8));
             maxb = std::max<int32_t>(maxb, int32_t(n));
         }
     }
     if (p->bj_split != split || !p->bj_off) {
         if (p->bj_off) RTX_HIP(hipFree(p->bj_off));
+        if (p->bj_off16) RTX_HIP(hipFree(p->bj_off16));
         if (p->bj_blk) RTX_HIP(hipFree(p->bj_blk));
-        p->bj_off = nullptr;
+        p->bj_off = p->bj_off16 = nullptr;
         p->bj_blk = nullptr;
         RTX_HIP(hipMalloc(&p->bj_off, off.size() * sizeof(int64_t)));
         RTX_HIP(hipMemcpy(p->bj_off, off.data(), off.size() * sizeof(int64_t),
+                          hipMemcpyHostToDevice));
+        RTX_HIP(hipMalloc(&p->bj_off16, off16.size() * sizeof(int64_t)));
+        RTX_HIP(hipMemcpy(p->bj_off16, off16.data(), off16.size() * sizeof(int64_t),
                           hipMemcpyHostToDevice));
         RTX_HIP(hipMalloc(&p->bj_blk, blk.size() * sizeof(int32_t)));
         RTX_HIP(hipMemcpy(p->bj_blk, blk.data(), blk.size() * sizeof(int32_t),
@@ -574,7 +649,31 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
     p->bj_nblk = int32_t(blk.size() / 3);
     p->bj_max_block = maxb;
     p->bj_entries = off.back();
+    p->bj_entries16 = off16.back();
     *n_entries = off.back();
+    return RTX_OK;
+}
+
+/* Entries (uint16) of the packed BF16 inverse pool. */
+extern "C" int rtx_bjacobi_size_bf16(rtx_plan *p, int64_t *n_entries) {
+    RTX_CHECK(p && n_entries, "rtx_bjacobi_size_bf16: null argument");
+    if (!p->bj_off16) {
+        int64_t n;
+        const int rc = rtx_bjacobi_size(p, &n);
+        if (rc) return rc;
+    }
+    *n_entries = p->bj_entries16;
+    return RTX_OK;
+}
+
+/* Round the FP32 inverses to BF16 (nearest even) into the packed layout. */
+extern "C" int rtx_bjacobi_pack_bf16(rtx_plan *p, const float *d_inv32, uint16_t *d_inv16,
+                                     void *stream) {
+    RTX_CHECK(p && d_inv32 && d_inv16, "rtx_bjacobi_pack_bf16: null argument");
+    RTX_CHECK(p->bj_off16 && p->bj_nblk > 0, "rtx_bjacobi_pack_bf16: call rtx_bjacobi_size first");
+    k_bj_pack_bf16<<<p->bj_nblk, BJ_THREADS, 0, static_cast<hipStream_t>(stream)>>>(
+        p->m, p->bj_off, p->bj_off16, p->bj_blk, d_inv32, d_inv16);
+    RTX_HIP(hipGetLastError());
     return RTX_OK;
 }
 
@@ -672,8 +771,9 @@ extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, 
         k_bj_apply<float, float><<<p->bj_nblk, BJ_THREADS, 0, st>>>(
             p->m, p->bj_off, p->bj_blk, (const float *)d_inv, (const float *)d_r, (float *)d_z);
     else if (precision == RTX_PREC_MIXED)
-        k_bj_apply<float, bj_bf16><<<p->bj_nblk, BJ_THREADS, 0, st>>>(
-            p->m, p->bj_off, p->bj_blk, (const bj_bf16 *)d_inv, (const float *)d_r, (float *)d_z);
+        k_bj_apply_bf16<<<p->bj_nblk, BJ_THREADS, 0, st>>>(
+            p->m, p->bj_off16, p->bj_blk, (const uint16_t *)d_inv, (const float *)d_r,
+            (float *)d_z);
     else {
         set_error("rtx_bjacobi_apply: unsupported precision %d", precision);
         return RTX_ERR_ARG;

@@ -145,6 +145,9 @@ struct rtx_plan {
     // block-Jacobi
     int64_t *bj_off = nullptr;   // ncell + 1 offsets of the column-major inverses
     int64_t bj_entries = 0;
+    // packed BF16 copy: columns padded to a multiple of 8 rows (16 bytes)
+    int64_t *bj_off16 = nullptr;
+    int64_t bj_entries16 = 0;
     bool bj_ready = false;
     int bj_method = -1;          // -1 automatic, 0 LDS Gauss-Jordan, 1 register sweeps
     // blocks: (cell, first angle, angle count) triples; a cell of more than

@@ -499,6 +499,7 @@ extern "C" void rtx_plan_destroy(rtx_plan *p) {
     if (p->arena) (void)hipFree(p->arena);
     if (p->coef) (void)hipFree(p->coef);
     if (p->bj_off) (void)hipFree(p->bj_off);
+    if (p->bj_off16) (void)hipFree(p->bj_off16);
     if (p->bj_blk) (void)hipFree(p->bj_blk);
     if (p->coarse) (void)hipFree(p->coarse);
     if (p->bins) (void)hipFree(p->bins);

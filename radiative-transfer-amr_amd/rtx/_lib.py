@@ -67,6 +67,8 @@ SIGNATURES = [
     ("rtx_bjacobi_factor", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p]),
     ("rtx_bjacobi_set_method", C.c_int, [C.c_void_p, C.c_int]),
+    ("rtx_bjacobi_size_bf16", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("rtx_bjacobi_pack_bf16", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rtx_bjacobi_apply", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p]),
     ("rtx_arnoldi_workspace", C.c_int, [C.c_int, C.POINTER(C.c_int64)]),

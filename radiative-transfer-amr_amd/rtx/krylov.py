@@ -56,10 +56,15 @@ class BlockJacobi:
             del scratch
         check(rc, "rtx_bjacobi_factor")
         self.inv16 = None
+        self.entries16 = 0
         if keep_bf16:
-            self.inv16 = torch.empty(self.entries, dtype=torch.int16, device=dev)
-            check(self.lib.rtx_mlp_to_bf16(self.entries, _vp(self.inv32), _vp(self.inv16), st),
-                  "rtx_mlp_to_bf16")
+            n16 = C.c_int64()
+            check(self.lib.rtx_bjacobi_size_bf16(op.plan, C.byref(n16)), "rtx_bjacobi_size_bf16")
+            self.entries16 = int(n16.value)
+            # packed: columns padded to a multiple of 8 rows for 16-byte loads
+            self.inv16 = torch.empty(self.entries16, dtype=torch.int16, device=dev)
+            check(self.lib.rtx_bjacobi_pack_bf16(op.plan, _vp(self.inv32), _vp(self.inv16), st),
+                  "rtx_bjacobi_pack_bf16")
             if not keep_f32:
                 torch.cuda.current_stream().synchronize()
                 self.inv32 = None
@@ -67,7 +72,9 @@ class BlockJacobi:
             self.inv = None
 
     def bytes_per_apply(self, precision=PREC_F64):
-        return self.entries * {PREC_F64: 8, PREC_F32: 4, PREC_MIXED: 2}[precision]
+        if precision == PREC_MIXED:
+            return self.entries16 * 2
+        return self.entries * {PREC_F64: 8, PREC_F32: 4}[precision]
 
     def apply(self, r, z, precision=PREC_F64, stream=None):
         """precision PREC_F64: FP64 inverses and vectors; PREC_F32: FP32;

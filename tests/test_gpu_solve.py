@@ -257,3 +257,43 @@ def test_refine_fp32_basis_matches_spsolve(cuda):
     assert info["converged_reason"] > 0 and info["rel_residual"] <= 1e-10
     u = np.where(mask, x.cpu().numpy(), b)
     assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < TOL_SOL
+
+
+def _bf16_round(a):
+    """Round float32 values to BF16 (nearest even), returned as float64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", [0, 1], ids=["whole-cells", "split"])
+def test_block_jacobi_bf16_packed_apply(cuda, method):
+    """The packed BF16 block-Jacobi apply equals the oracle's inverses
+    rounded to BF16 applied in high precision (to FP32 accumulation), on an
+    hp mesh of 80..150-DOF cells (split into angle groups for method 1)."""
+    import torch
+    from rtx.operator import TransportOperator
+    from rtx.krylov import BlockJacobi
+    from rtx._lib import PREC_MIXED, check
+    mesh = problems.uniform_mesh([1.0, 1.0], [4, 4, 5], 1, 2)
+    keys = sorted(mesh.cols)
+    mesh.ref_col(keys[3], "spt", "p")
+    mesh.ref_cell(keys[-1], sorted(mesh.cols[keys[-1]].cells)[0], "p")
+    wl = problems.two_scatterers()
+    A, f, b, mask = _oracle_system(mesh, wl)
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    check(op.lib.rtx_bjacobi_set_method(op.plan, method), "set method")
+    pc = BlockJacobi(op, keep_f32=True, keep_bf16=True)
+    split = 128 if method == 1 else None
+    invs = osys.block_jacobi_inverses(A, mesh, mask, split)
+    r = np.random.default_rng(7).uniform(-1, 1, op.ndof).astype(np.float32)
+    z_ref = np.zeros(op.ndof)
+    for idx, inv in zip(osys.bj_block_dofs(mesh, split), invs):
+        z_ref[idx] = _bf16_round(inv.astype(np.float32)) @ r[idx].astype(np.float64)
+    z = pc.apply(_dev(r), torch.zeros(op.ndof, dtype=torch.float32, device="cuda"),
+                 PREC_MIXED).cpu().numpy()
+    # the device rounds its own FP32 inverse: the two roundings may differ by
+    # one BF16 ulp where the FP32 values sit on a rounding boundary
+    assert np.linalg.norm(z - z_ref) / np.linalg.norm(z_ref) < 2e-4
+    assert pc.bytes_per_apply(PREC_MIXED) >= 2 * pc.entries
