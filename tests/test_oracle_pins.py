@@ -228,6 +228,48 @@ def test_E_space_row_and_column_sums(n0, n1, dlv, lower):
         assert abs(E.sum() - 1.0) < 1e-13
 
 
+@pytest.mark.parametrize("n0,n1", [(2, 3), (3, 5), (4, 7), (3, 2), (5, 3), (7, 4), (6, 5), (8, 9)])
+@pytest.mark.parametrize("dlv", [0, 1, -1])
+@pytest.mark.parametrize("lower", [True, False])
+def test_E_space_mismatched_orders_entrywise(n0, n1, dlv, lower):
+    """With n0 != n1 every branch's rule has max(n0, n1) LGL nodes, exact to
+    degree 2 max - 3 >= n0 + n1 - 2, so each entry is the exact integral
+    E[i, p] = (2 / |own|) int_{own & neighbour} phi1_i phi0_p dx of the
+    physical Lagrange functions, computed here by numpy polynomial algebra
+    on the physical nodes."""
+    if dlv == 0 and not lower:
+        pytest.skip("one geometry for same-level faces")
+    a, b = _half(lower)
+    if dlv == 0:
+        own, nbr = (0.0, 1.0), (0.0, 1.0)
+        E = emat.E_space(n0, 0.0, 1.0, 2, n1, 0.0, 1.0, 2)
+    elif dlv == 1:
+        own, nbr = (a, b), (0.0, 1.0)
+        E = emat.E_space(n0, a, b, 3, n1, 0.0, 1.0, 2)
+    else:
+        own, nbr = (0.0, 1.0), (a, b)
+        E = emat.E_space(n0, 0.0, 1.0, 2, n1, a, b, 3)
+    X0 = own[0] + (own[1] - own[0]) * 0.5 * (quad.lgl(n0)[0] + 1.0)
+    X1 = nbr[0] + (nbr[1] - nbr[0]) * 0.5 * (quad.lgl(n1)[0] + 1.0)
+    lo, hi = max(own[0], nbr[0]), min(own[1], nbr[1])
+    # a 16-point Gauss-Legendre rule on the overlap integrates degree 31
+    # exactly; the Lagrange functions in product form on the physical nodes
+    g, gw = np.polynomial.legendre.leggauss(16)
+    xs = lo + (hi - lo) * 0.5 * (g + 1.0)
+    gw = gw * 0.5 * (hi - lo)
+
+    def lag_prod(nodes, i, x):
+        v = np.ones_like(x)
+        for j, xj in enumerate(nodes):
+            if j != i:
+                v *= (x - xj) / (nodes[i] - xj)
+        return v
+    ref = np.array([[2.0 / (own[1] - own[0]) *
+                     np.sum(gw * lag_prod(X1, i, xs) * lag_prod(X0, p, xs))
+                     for p in range(n0)] for i in range(n1)])
+    np.testing.assert_allclose(E, ref, atol=1e-13)
+
+
 def test_E_space_rejects_level_jump_of_two():
     with pytest.raises(ValueError):
         emat.E_space(3, 0.0, 0.25, 4, 3, 0.0, 1.0, 2)
