@@ -21,6 +21,8 @@ Modules
              to the rows of a subset of columns (large meshes)
   system   - interior/boundary split, right-hand side, exact and Krylov
              solves, per-cell block-Jacobi, MLP reference
+  error    - analytic L2 error of a nodal solution (interpolation to
+             n + 3 nodes per direction and weighted squared error)
 
 Parity pins (what fixes each function) are listed in each module header.
 Functions without an independent pin say "parity unpinned" there.

@@ -24,9 +24,11 @@ A x = 0 from a random start, for random kappa / sigma fields.  There the
 error of every iterate is exactly -x and its residual -A x, so each restart
 yields a (residual, correction) pair without a reference solution.
 
-Fallback: ``solve`` judges every outer step by the FP64 true residual; if a
-step with the coarse correction does not reduce it, the correction is
-switched off and the solve continues as plain block-Jacobi GMRES.
+Fallback: ``Session.solve`` judges outer steps by the FP64 true residual:
+one step with the correction and one without, keeping whichever reduced the
+residual faster per second; afterwards a step with the correction that does
+not cut the residual by 10 % switches it off, and the solve continues as
+plain block-Jacobi GMRES.
 """
 import ctypes as C
 import time
