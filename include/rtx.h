@@ -272,6 +272,22 @@ int rtx_axpby_mixed(int direction, int64_t n, double a, const void *d_x, double 
 /* direction 0: FP64 -> FP32, 1: FP32 -> FP64. */
 int rtx_convert(int direction, int64_t n, const void *d_src, void *d_dst, void *stream);
 
+/* ---- analytic L2 error (src/amr/Error_Indicator/error/error_analytic.py:22-149:
+   interpolation of every cell to n + 3 nodes per direction, weighted squared
+   error against the analytic solution at those nodes). */
+#define RTX_ERR_TAB_STRIDE (2 * RTX_NMAX * (RTX_NMAX + 3) + 2 * (RTX_NMAX + 3))
+/* d_tab: (RTX_NMAX + 1) * RTX_ERR_TAB_STRIDE doubles; for n = 1..RTX_NMAX
+   at n * RTX_ERR_TAB_STRIDE: the LGL interpolation matrix n -> n + 3
+   (P[i (n + 3) + p] = l_i(xi_p), error_analytic.py:57-80), at
+   + RTX_NMAX (RTX_NMAX + 3) the Gauss-Legendre one (:100-111), at
+   + 2 RTX_NMAX (RTX_NMAX + 3) the n + 3 LGL weights, then the n + 3 LG
+   weights.  d_u_hr: the analytic solution at each cell's (nx + 3)(ny + 3)
+   (nth + 3) nodes, (p, q, r) row-major, cell k from d_hr_off[k].
+   d_cell_err[k] = (dx dy dth / 8) sum w_p w_q w_r (u - I uh)^2 (:136-137). */
+int rtx_error_l2(rtx_plan *plan, const double *d_uh, const double *d_u_hr,
+                 const int64_t *d_hr_off, const double *d_tab, double *d_cell_err,
+                 void *stream);
+
 /* ---- learned coarse scalar-flux correction.  No counterpart in the
    reference: its preconditioners are PETSc bjacobi / kaczmarz
    (src/rt/solve.py:86-95).  A coarse grid of n_coarse cells covers the

@@ -87,6 +87,8 @@ SIGNATURES = [
     ("rtx_axpby", C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_void_p, C.c_double,
                             C.c_void_p, C.c_void_p]),
     ("rtx_convert", C.c_int, [C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtx_error_l2", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p]),
     ("rtx_gmres_state_size", C.c_int, [C.c_int, C.POINTER(C.c_int64)]),
     ("rtx_gmres_begin", C.c_int, [C.c_int, C.c_double, C.c_void_p, C.c_void_p]),
     ("rtx_gmres_givens", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
