@@ -240,6 +240,14 @@ int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n, double *d
    counterpart in the reference (PETSc's basis is FP64). */
 int rtx_arnoldi_step_f32(const float *d_V, int64_t ld, int nv, int64_t n, float *d_w,
                          double *d_hcol, double *d_work, int refine, void *stream);
+/* Arnoldi step j of a GMRES cycle of m steps (nv = j + 1 basis vectors,
+   FP64 basis for basis_precision 0, FP32 for 1) fused with its Givens
+   update: the same results as rtx_arnoldi_step (or _f32) followed by
+   rtx_gmres_givens(j, m, d_hcol, d_state).  No counterpart in the
+   reference (PETSc keeps the least-squares problem on the host). */
+int rtx_gmres_arnoldi(int basis_precision, const void *d_V, int64_t ld, int64_t n, void *d_w,
+                      double *d_hcol, double *d_work, int refine, int j, int m,
+                      double *d_state, void *stream);
 /* Least-squares state of one restart cycle of m steps kept on the device
    (Givens QR of the Hessenberg matrix; PETSc keeps it on the host, here it
    stays on the GPU so whole chunks of iterations can run as one HIP graph).

@@ -33,9 +33,38 @@ __device__ inline void block_sum(double (&v)[N], double *sh) {
     }
 }
 
+// Four consecutive values as doubles (16-byte loads; the caller checks the
+// alignment).
+__device__ inline void ld4(const float *p, double (&o)[4]) {
+    const float4 f = *reinterpret_cast<const float4 *>(p);
+    o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = f.w;
+}
+__device__ inline void ld4(const double *p, double (&o)[4]) {
+    const double2 a = *reinterpret_cast<const double2 *>(p);
+    const double2 b = *reinterpret_cast<const double2 *>(p + 2);
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ inline void st4(float *p, const float (&v)[4]) {
+    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ inline void st4(double *p, const double (&v)[4]) {
+    *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2 *>(p + 2) = make_double2(v[2], v[3]);
+}
+
+// Row range of block b; with VEC the range starts on a multiple of 4.
+template <bool VEC>
+__device__ inline void row_range(int64_t n, int b, int nb, int64_t &i0, int64_t &i1) {
+    int64_t per = (n + nb - 1) / nb;
+    if (VEC) per = (per + 3) & ~int64_t(3);
+    i0 = int64_t(b) * per;
+    i1 = (i0 + per < n) ? i0 + per : n;
+    if (i0 > n) i0 = n;
+}
+
 // partial[j * nb + b] = sum over block b's rows of V_j[i] w[i], j < nv;
 // if with_ww, partial[nv * nb + b] = sum w[i]^2 (computed by chunk 0).
-template <typename T>
+template <typename T, bool VEC>
 __global__ __launch_bounds__(KT) void k_mdot(const T *__restrict__ V, int64_t ld, int nv,
                                              const T *__restrict__ w, int64_t n,
                                              double *__restrict__ partial, int with_ww,
@@ -44,12 +73,26 @@ __global__ __launch_bounds__(KT) void k_mdot(const T *__restrict__ V, int64_t ld
     __shared__ double sh[KT * (KCH + 1)];
     const int b = blockIdx.x, nb = gridDim.x;
     const int j0 = blockIdx.y * KCH;
-    const int64_t per = (n + nb - 1) / nb;
-    const int64_t i0 = int64_t(b) * per, i1 = (i0 + per < n) ? i0 + per : n;
+    int64_t i0, i1;
+    row_range<VEC>(n, b, nb, i0, i1);
     double acc[KCH + 1];
 #pragma unroll
     for (int j = 0; j <= KCH; ++j) acc[j] = 0.0;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += KT) {
+    const int64_t nq = VEC ? (i1 - i0) / 4 : 0;
+    for (int64_t q = threadIdx.x; q < nq; q += KT) {
+        const int64_t i = i0 + 4 * q;
+        double wv[4];
+        ld4(w + i, wv);
+#pragma unroll
+        for (int j = 0; j < KCH; ++j)
+            if (j0 + j < nv) {
+                double vv[4];
+                ld4(V + int64_t(j0 + j) * ld + i, vv);
+                acc[j] += vv[0] * wv[0] + vv[1] * wv[1] + vv[2] * wv[2] + vv[3] * wv[3];
+            }
+        acc[KCH] += wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2] + wv[3] * wv[3];
+    }
+    for (int64_t i = i0 + 4 * nq + threadIdx.x; i < i1; i += KT) {
         const double wi = double(w[i]);
 #pragma unroll
         for (int j = 0; j < KCH; ++j)
@@ -79,7 +122,7 @@ __global__ __launch_bounds__(KT) void k_reduce(const double *__restrict__ partia
 
 // w -= V h (nv vectors); partial[b] = sum over block b of the new w^2
 // (accumulated in FP64 for either storage type).
-template <typename T>
+template <typename T, bool VEC>
 __global__ __launch_bounds__(KT) void k_update(const T *__restrict__ V, int64_t ld, int nv,
                                                const double *__restrict__ h,
                                                T *__restrict__ w, int64_t n,
@@ -91,10 +134,30 @@ __global__ __launch_bounds__(KT) void k_update(const T *__restrict__ V, int64_t 
     for (int j = threadIdx.x; j < nv; j += KT) hs[j] = h[j];
     __syncthreads();
     const int b = blockIdx.x, nb = gridDim.x;
-    const int64_t per = (n + nb - 1) / nb;
-    const int64_t i0 = int64_t(b) * per, i1 = (i0 + per < n) ? i0 + per : n;
+    int64_t i0, i1;
+    row_range<VEC>(n, b, nb, i0, i1);
     double ss[1] = {0.0};
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += KT) {
+    const int64_t nq = VEC ? (i1 - i0) / 4 : 0;
+    for (int64_t q = threadIdx.x; q < nq; q += KT) {
+        const int64_t i = i0 + 4 * q;
+        double acc[4];
+        ld4(w + i, acc);
+        for (int j = 0; j < nv; ++j) {
+            double vv[4];
+            ld4(V + int64_t(j) * ld + i, vv);
+            const double hj = hs[j];
+            acc[0] -= vv[0] * hj; acc[1] -= vv[1] * hj;
+            acc[2] -= vv[2] * hj; acc[3] -= vv[3] * hj;
+        }
+        T out[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            out[u] = T(acc[u]);
+            ss[0] += double(out[u]) * double(out[u]);
+        }
+        st4(w + i, out);
+    }
+    for (int64_t i = i0 + 4 * nq + threadIdx.x; i < i1; i += KT) {
         double acc = double(w[i]);
         int j = 0;
         for (; j + 4 <= nv; j += 4) {
@@ -210,6 +273,58 @@ __global__ void k_gmres_givens(int j, int m, const double *__restrict__ hcol, do
     s.est[j] = fabs(s.g[j + 1]);
 }
 
+// v_out = w / |w| (as k_normalize) and, in block 0, the Givens update of
+// column j (as k_gmres_givens) with the rotations staged in LDS, so the
+// serial recurrence runs on LDS values while the other blocks normalise.
+template <typename T>
+__global__ __launch_bounds__(KT) void k_normalize_givens(const T *__restrict__ w,
+                                                         T *__restrict__ v_out, int64_t n,
+                                                         const double *__restrict__ nrm2,
+                                                         double *__restrict__ hcol, int nv,
+                                                         int j, int m, double *state) {
+    const double beta = sqrt(*nrm2);
+    const double s = beta > 0.0 ? 1.0 / beta : 0.0;
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i < n; i += int64_t(gridDim.x) * blockDim.x) v_out[i] = T(double(w[i]) * s);
+    if (blockIdx.x != 0) return;
+    __shared__ double hc[KMAXV + 1], cs[KMAXV], sn[KMAXV];
+    GState g(state, m);
+    const int tid = threadIdx.x;
+    for (int t = tid; t <= j; t += KT) hc[t] = hcol[t];
+    if (tid == 0) hc[j + 1] = beta;
+    for (int t = tid; t < j; t += KT) {
+        cs[t] = g.cs[t];
+        sn[t] = g.sn[t];
+    }
+    __syncthreads();
+    double *Hj = g.H + int64_t(j) * (m + 1);
+    for (int t = tid; t <= j + 1; t += KT) Hj[t] = hc[t];
+    __syncthreads();                 // hc is rotated in place below
+    if (tid == 0) {
+        hcol[nv] = beta;
+        double carry = hc[0];
+        for (int t = 0; t < j; ++t) {
+            const double b = hc[t + 1];
+            hc[t] = cs[t] * carry + sn[t] * b;
+            carry = -sn[t] * carry + cs[t] * b;
+        }
+        const double hj1 = beta;
+        const double den = hypot(carry, hj1);
+        const double c = den != 0.0 ? carry / den : 1.0;
+        const double sv = den != 0.0 ? hj1 / den : 0.0;
+        g.cs[j] = c;
+        g.sn[j] = sv;
+        hc[j] = den;
+        const double gj = g.g[j];
+        g.g[j + 1] = -sv * gj;
+        g.g[j] = c * gj;
+        g.est[j] = fabs(g.g[j + 1]);
+    }
+    __syncthreads();
+    double *Rj = g.R + int64_t(j) * m;
+    for (int t = tid; t <= j; t += KT) Rj[t] = hc[t];
+}
+
 // y = R_k^{-1} g_k (back substitution over the first k columns).
 __global__ void k_gmres_solve(int k, int m, const double *state, double *__restrict__ y) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -248,8 +363,34 @@ extern "C" int rtx_arnoldi_workspace(int max_vecs, int64_t *n_doubles) {
    h_0..h_{nv-1}, h_nv = |w| to d_hcol (device, nv + 1 doubles).  w is
    overwritten.  Everything is enqueued on `stream`; no host sync. */
 template <typename T>
+static bool aligned16(const T *p) {
+    return (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
+
+template <typename T>
+static void launch_mdot(bool vec, dim3 grid, hipStream_t st, const T *V, int64_t ld, int nv,
+                        const T *w, int64_t n, double *part, int with_ww, const int *flag) {
+    if (vec)
+        k_mdot<T, true><<<grid, KT, 0, st>>>(V, ld, nv, w, n, part, with_ww, flag);
+    else
+        k_mdot<T, false><<<grid, KT, 0, st>>>(V, ld, nv, w, n, part, with_ww, flag);
+}
+
+template <typename T>
+static void launch_update(bool vec, hipStream_t st, const T *V, int64_t ld, int nv,
+                          const double *h, T *w, int64_t n, double *part, const int *flag) {
+    if (vec)
+        k_update<T, true><<<KNB, KT, 0, st>>>(V, ld, nv, h, w, n, part, flag);
+    else
+        k_update<T, false><<<KNB, KT, 0, st>>>(V, ld, nv, h, w, n, part, flag);
+}
+
+// One Arnoldi step; with d_state, the final normalisation also folds the
+// new Hessenberg column j into the least-squares state (rtx_gmres_givens).
+template <typename T>
 static int arnoldi_step(const T *d_V, int64_t ld, int nv, int64_t n, T *d_w, double *d_hcol,
-                        double *d_work, int refine, void *stream) {
+                        double *d_work, int refine, void *stream, int gj = -1, int gm = 0,
+                        double *d_state = nullptr) {
     RTX_CHECK(d_V && d_w && d_hcol && d_work, "rtx_arnoldi_step: null argument");
     RTX_CHECK(refine >= 0 && refine <= 2, "rtx_arnoldi_step: refine must be 0, 1 or 2");
     RTX_CHECK(nv >= 1 && nv < KMAXV - 1, "rtx_arnoldi_step: nv=%d outside [1, %d)", nv,
@@ -262,16 +403,24 @@ static int arnoldi_step(const T *d_V, int64_t ld, int nv, int64_t n, T *d_w, dou
     int *flag = reinterpret_cast<int *>(state + 4);
     const int nchunk = (nv + KCH - 1) / KCH;
     T *V_out = const_cast<T *>(d_V) + int64_t(nv) * ld;
+    const bool vec = (ld % 4 == 0) && aligned16(d_V) && aligned16(d_w);
+    auto normalize = [&]() {
+        if (d_state)
+            k_normalize_givens<T><<<2048, KT, 0, st>>>(d_w, V_out, n, state + 1, d_hcol, nv, gj,
+                                                       gm, d_state);
+        else
+            k_normalize<T><<<2048, KT, 0, st>>>(d_w, V_out, n, state + 1, d_hcol, nv);
+    };
     // pass 1: h = V^T w, |w|^2
-    k_mdot<T><<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, part, refine == 1 ? 1 : 0,
-                                                nullptr);
+    launch_mdot<T>(vec, dim3(KNB, nchunk), st, d_V, ld, nv, d_w, n, part, refine == 1 ? 1 : 0,
+                   nullptr);
     k_reduce<<<nv, KT, 0, st>>>(part, KNB, d_hcol, 0, nullptr);
     if (refine == 1) k_reduce<<<1, KT, 0, st>>>(part + int64_t(nv) * KNB, KNB, state + 0, 0, nullptr);
     // w -= V h, |w'|^2
-    k_update<T><<<KNB, KT, 0, st>>>(d_V, ld, nv, d_hcol, d_w, n, part2, nullptr);
+    launch_update<T>(vec, st, d_V, ld, nv, d_hcol, d_w, n, part2, nullptr);
     k_reduce<<<1, KT, 0, st>>>(part2, KNB, state + 1, 0, nullptr);
     if (refine == 0) {     // classical Gram-Schmidt, one pass (PETSc's default)
-        k_normalize<T><<<2048, KT, 0, st>>>(d_w, V_out, n, state + 1, d_hcol, nv);
+        normalize();
         RTX_HIP(hipGetLastError());
         return RTX_OK;
     }
@@ -280,14 +429,14 @@ static int arnoldi_step(const T *d_V, int64_t ld, int nv, int64_t n, T *d_w, dou
     else
         k_set_flag<<<1, 1, 0, st>>>(flag, 0);
     // optional pass 2 (skipped on the device when flag == 1)
-    k_mdot<T><<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, part, 0, flag);
+    launch_mdot<T>(vec, dim3(KNB, nchunk), st, d_V, ld, nv, d_w, n, part, 0, flag);
     // accumulate the correction into hcol, keep it separately for the update
     double *hcorr = part + int64_t(KMAXV) * KNB;            // nv doubles, after the partials
     k_reduce<<<nv, KT, 0, st>>>(part, KNB, hcorr, 0, flag);
-    k_update<T><<<KNB, KT, 0, st>>>(d_V, ld, nv, hcorr, d_w, n, part2, flag);
+    launch_update<T>(vec, st, d_V, ld, nv, hcorr, d_w, n, part2, flag);
     k_reduce<<<1, KT, 0, st>>>(part2, KNB, state + 1, 0, flag);
     k_reduce<<<nv, KT, 0, st>>>(hcorr, 1, d_hcol, 1, flag);  // hcol += hcorr
-    k_normalize<T><<<2048, KT, 0, st>>>(d_w, V_out, n, state + 1, d_hcol, nv);
+    normalize();
     RTX_HIP(hipGetLastError());
     return RTX_OK;
 }
@@ -301,6 +450,27 @@ static int arnoldi_step(const T *d_V, int64_t ld, int nv, int64_t n, T *d_w, dou
 extern "C" int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n, double *d_w,
                                 double *d_hcol, double *d_work, int refine, void *stream) {
     return arnoldi_step<double>(d_V, ld, nv, n, d_w, d_hcol, d_work, refine, stream);
+}
+
+/* Arnoldi step j of a GMRES cycle of m steps fused with its Givens update:
+   rtx_arnoldi_step (FP64 basis, basis_precision 0, or FP32, 1) on
+   nv = j + 1 vectors, then rtx_gmres_givens(j, m, d_hcol, d_state) folded
+   into the final normalisation.  No counterpart in the reference (PETSc
+   keeps the least-squares problem on the host). */
+extern "C" int rtx_gmres_arnoldi(int basis_precision, const void *d_V, int64_t ld, int64_t n,
+                                 void *d_w, double *d_hcol, double *d_work, int refine, int j,
+                                 int m, double *d_state, void *stream) {
+    RTX_CHECK(d_state && m >= 1 && m < KMAXV - 1 && j >= 0 && j < m,
+              "rtx_gmres_arnoldi: bad argument (j=%d, m=%d)", j, m);
+    RTX_CHECK(basis_precision == 0 || basis_precision == 1,
+              "rtx_gmres_arnoldi: basis_precision must be 0 (FP64) or 1 (FP32)");
+    if (basis_precision == 0)
+        return arnoldi_step<double>(static_cast<const double *>(d_V), ld, j + 1, n,
+                                    static_cast<double *>(d_w), d_hcol, d_work, refine, stream,
+                                    j, m, d_state);
+    return arnoldi_step<float>(static_cast<const float *>(d_V), ld, j + 1, n,
+                               static_cast<float *>(d_w), d_hcol, d_work, refine, stream, j, m,
+                               d_state);
 }
 
 /* The same step on an FP32 basis (dot products, norms and the Hessenberg
@@ -347,7 +517,8 @@ extern "C" int rtx_mdot(const double *d_V, int64_t ld, int nv, int64_t n, const 
     RTX_CHECK(d_V && d_w && d_out && d_work && nv >= 1 && nv < KMAXV, "rtx_mdot: bad argument");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int nchunk = (nv + KCH - 1) / KCH;
-    k_mdot<double><<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, d_work, 0, nullptr);
+    launch_mdot<double>((ld % 4 == 0) && aligned16(d_V) && aligned16(d_w), dim3(KNB, nchunk), st,
+                        d_V, ld, nv, d_w, n, d_work, 0, nullptr);
     k_reduce<<<nv, KT, 0, st>>>(d_work, KNB, d_out, 0, nullptr);
     RTX_HIP(hipGetLastError());
     return RTX_OK;

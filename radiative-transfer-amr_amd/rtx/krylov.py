@@ -243,17 +243,16 @@ class GMRES:
         f32 = self.basis == "f32"
         pre = self.system.precondition_lo if f32 else self.system.precondition
         mv = self.system.matvec_lo if f32 else self.system.matvec
-        step = lib.rtx_arnoldi_step_f32 if f32 else lib.rtx_arnoldi_step
         for j in range(j0, j1):
             zj = Z[j] if self.flexible else z
             pre(V[j], zj)
             mv(zj, w)
             if self._on_direction is not None:
                 self._on_direction(zj, w)
-            check(step(_vp(V), n, j + 1, n, _vp(w), _vp(self.hcol), _vp(self.work), self.refine,
-                       st), "rtx_arnoldi_step")
-            check(lib.rtx_gmres_givens(j, m, _vp(self.hcol), _vp(self.state), st),
-                  "rtx_gmres_givens")
+            # Arnoldi step with the Givens update of column j folded in
+            check(lib.rtx_gmres_arnoldi(1 if f32 else 0, _vp(V), n, n, _vp(w), _vp(self.hcol),
+                                        _vp(self.work), self.refine, j, m, _vp(self.state), st),
+                  "rtx_gmres_arnoldi")
 
     def solve(self, b, x=None, rtol=1e-8, atol=0.0, max_it=5000, callback=None,
               on_restart=None, on_direction=None):

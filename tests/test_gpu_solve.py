@@ -297,3 +297,49 @@ def test_block_jacobi_bf16_packed_apply(cuda, method):
     # one BF16 ulp where the FP32 values sit on a rounding boundary
     assert np.linalg.norm(z - z_ref) / np.linalg.norm(z_ref) < 2e-4
     assert pc.bytes_per_apply(PREC_MIXED) >= 2 * pc.entries
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("basis", [0, 1], ids=["f64", "f32"])
+def test_fused_arnoldi_givens_matches_separate(cuda, basis):
+    """rtx_gmres_arnoldi (Arnoldi step with the Givens update folded into the
+    normalisation) leaves the basis, the Hessenberg column and the whole
+    least-squares state equal (basis bit for bit, state to rounding of the
+    rotation recurrence) to rtx_arnoldi_step + rtx_gmres_givens over a full
+    cycle of m steps; also on a length not divisible by 4 (the scalar
+    path)."""
+    import ctypes as C
+    import torch
+    from rtx import _lib
+    from rtx.krylov import _vp
+    lib = _lib.load()
+    m = 9
+    for n in (100_004, 77_777):
+        rng = np.random.default_rng(11)
+        W = rng.normal(size=(m, n))
+        dt = torch.float32 if basis else torch.float64
+        ws, ss = C.c_int64(), C.c_int64()
+        _lib.check(lib.rtx_arnoldi_workspace(m + 1, C.byref(ws)), "ws")
+        _lib.check(lib.rtx_gmres_state_size(m, C.byref(ss)), "ss")
+        out = []
+        for fused in (False, True):
+            V = torch.zeros((m + 1, n), dtype=dt, device="cuda")
+            V[0] = torch.from_numpy(W[0] / np.linalg.norm(W[0])).to(dt)
+            work = torch.zeros(int(ws.value), dtype=torch.float64, device="cuda")
+            state = torch.zeros(int(ss.value), dtype=torch.float64, device="cuda")
+            hcol = torch.zeros(m + 2, dtype=torch.float64, device="cuda")
+            _lib.check(lib.rtx_gmres_begin(m, 2.5, _vp(state), None), "begin")
+            for j in range(m - 1):
+                w = torch.from_numpy(W[j + 1]).to(dt).cuda()
+                if fused:
+                    _lib.check(lib.rtx_gmres_arnoldi(basis, _vp(V), n, n, _vp(w), _vp(hcol),
+                                                     _vp(work), 0, j, m, _vp(state), None), "f")
+                else:
+                    step = lib.rtx_arnoldi_step_f32 if basis else lib.rtx_arnoldi_step
+                    _lib.check(step(_vp(V), n, j + 1, n, _vp(w), _vp(hcol), _vp(work), 0, None),
+                               "step")
+                    _lib.check(lib.rtx_gmres_givens(j, m, _vp(hcol), _vp(state), None), "g")
+            torch.cuda.synchronize()
+            out.append((V.cpu().numpy(), state.cpu().numpy()))
+        assert np.array_equal(out[0][0], out[1][0])
+        np.testing.assert_allclose(out[1][1], out[0][1], rtol=1e-13, atol=1e-15)
