@@ -214,7 +214,7 @@ def _rhs(op, wl):
     return op.mask(f - op.apply(bb, ops=OP_SYSTEM), keep_interior=True)
 
 
-def _timed_solve(op, rhs, precision, make_pc=None, inner_rtol=1e-3):
+def _timed_solve(op, rhs, precision, make_pc=None, inner_rtol=1e-2):
     """Set up a solver session (block-Jacobi factorisation, and the coarse
     preconditioner when ``make_pc(bj)`` is given), then solve twice: the
     first ("cold") solve also runs each chunk of iterations eagerly once and
@@ -227,7 +227,8 @@ def _timed_solve(op, rhs, precision, make_pc=None, inner_rtol=1e-3):
     t0 = time.perf_counter()
     bj = BlockJacobi(op, keep_f64=precision == "fp64", keep_bf16=precision != "fp64")
     pc = make_pc(bj) if make_pc is not None else None
-    sess = learned.Session(op, pc=pc, bj=bj, precision=precision, restart=50)
+    # fp64: GMRES(50) as the reference's KSP; mixed: inner GMRES(20)
+    sess = learned.Session(op, pc=pc, bj=bj, precision=precision)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     out = {}

@@ -570,24 +570,37 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_apply_bf16(DevMesh m, const i
     const int n = m.col_nx[c] * m.col_ny[c] * g;
     const int ldp = (n + 7) / 8 * 8, RG = ldp / 8, CG = BJ_THREADS / RG;
     const int off = m.cell_dof_off[k] + a0;
+    const int rg = tid % RG, cg = tid / RG;
+    const uint4 *B = reinterpret_cast<const uint4 *>(inv + off16[b]) + rg;
+    const int ldq = ldp / 8;
+    // the first 8 columns of this thread are fetched before the wait on r,
+    // so the inverse loads and the r gather are in flight together
+    constexpr int PF = 8;
+    uint4 q[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        const int j = cg + u * CG;
+        q[u] = (cg < CG && j < n) ? B[size_t(j) * ldq] : make_uint4(0u, 0u, 0u, 0u);
+    }
     for (int i = tid; i < n; i += BJ_THREADS) rs[i] = r[off + (i / g) * nth + i % g];
     __syncthreads();
-    const int rg = tid % RG, cg = tid / RG;
     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (cg < CG) {
-        const uint4 *B = reinterpret_cast<const uint4 *>(inv + off16[b]) + rg;
-        const int ldq = ldp / 8;
-#pragma unroll 4
-        for (int j = cg; j < n; j += CG) {
-            const uint4 q = B[size_t(j) * ldq];
-            const float rj = rs[j];
-            const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+    auto fma8 = [&](const uint4 &v, float rj) {
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                acc[2 * u] += __uint_as_float(w[u] << 16) * rj;
-                acc[2 * u + 1] += __uint_as_float(w[u] & 0xffff0000u) * rj;
-            }
+        for (int u = 0; u < 4; ++u) {
+            acc[2 * u] += __uint_as_float(w[u] << 16) * rj;
+            acc[2 * u + 1] += __uint_as_float(w[u] & 0xffff0000u) * rj;
         }
+    };
+    if (cg < CG) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int j = cg + u * CG;
+            if (j < n) fma8(q[u], rs[j]);
+        }
+#pragma unroll 4
+        for (int j = cg + PF * CG; j < n; j += CG) fma8(B[size_t(j) * ldq], rs[j]);
 #pragma unroll
         for (int u = 0; u < 8; ++u) part[cg * ldp + rg * 8 + u] = acc[u];
     }

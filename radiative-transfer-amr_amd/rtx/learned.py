@@ -530,8 +530,13 @@ class Session:
     FP64 Arnoldi basis; "fp64": GMRES in FP64.  ``pc`` is a TwoLevelPC or
     None (plain block-Jacobi)."""
 
-    def __init__(self, op, pc=None, bj=None, precision="mixed", restart=50,
+    def __init__(self, op, pc=None, bj=None, precision="mixed", restart=None,
                  bj_precision=PREC_MIXED, basis="f32"):
+        # GMRES(50) as the reference's KSP (src/rt/solve.py:87) for "fp64";
+        # the inner solves of mixed refinement restart after 20 steps
+        # (shorter Gram-Schmidt, fewer total iterations on these problems)
+        if restart is None:
+            restart = 50 if precision == "fp64" else 20
         if precision not in ("mixed", "fp64"):
             raise ValueError("precision must be 'mixed' or 'fp64'")
         if bj is None:
