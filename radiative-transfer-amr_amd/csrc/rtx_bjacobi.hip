@@ -385,6 +385,8 @@ __global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restric
                                      (col / g) * x.nth + a0 + col % g)
                           : 0.0;
         }
+    // Every step is written without per-element branches (selects around
+    // one FMA per element), so the wave runs straight-line code.
     for (int kk = 0; kk < n; ++kk) {
         const int buf = kk & 1, vk = kk / TJ;
         if (tj == kk % TJ) {           // the 16 owners of column kk
@@ -392,19 +394,16 @@ __global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restric
             int bi = n;
 #pragma unroll
             for (int u = 0; u < RR; ++u) {
-                double val = 0.0;
+                double val = A[u][0];
 #pragma unroll
-                for (int v = 0; v < CC; ++v)
-                    if (v == vk) val = A[u][v];
+                for (int v = 1; v < CC; ++v) val = (v == vk) ? A[u][v] : val;
                 const int row = ti + 16 * u;
-                if (row < n) {
-                    colk[buf][row] = val;
-                    const double av = fabs(val);
-                    if (!used[row] && (av > best || (av == best && row < bi))) {
-                        best = av;
-                        bi = row;
-                    }
-                }
+                const bool ok = row < n;
+                if (ok) colk[buf][row] = val;
+                const double av = fabs(val);
+                const bool better = ok && !used[ok ? row : 0] && (av > best || (av == best && row < bi));
+                best = better ? av : best;
+                bi = better ? row : bi;
             }
             red_v[ti] = best;
             red_i[ti] = bi;
@@ -427,39 +426,42 @@ __global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restric
         if (ti == (pr & 15)) {
             const int up = pr >> 4;
 #pragma unroll
-            for (int u = 0; u < RR; ++u)
-                if (u == up)
+            for (int v = 0; v < CC; ++v) {
+                double pval = A[0][v];
 #pragma unroll
-                    for (int v = 0; v < CC; ++v) {
-                        const int col = tj + TJ * v;
-                        if (col < n) prow[col] = A[u][v];
-                    }
+                for (int u = 1; u < RR; ++u) pval = (u == up) ? A[u][v] : pval;
+                const int col = tj + TJ * v;
+                if (col < n) prow[col] = pval;
+            }
         }
         if (tid == 0) {
             used[pr] = 1;
             perm[kk] = pr;
         }
         __syncthreads();
+        // in-place Gauss-Jordan: pivot row -> row / p with 1 / p in column
+        // kk; other rows -> row - c_r (pivot row / p) with -c_r / p in column
+        // kk.  With the pivot row's kk entry taken as 1 and the other rows'
+        // kk entries as 0 this is one FMA per element.
         const double invp = 1.0 / colk[buf][pr];
         double pv[CC];
 #pragma unroll
         for (int v = 0; v < CC; ++v) {
             const int col = tj + TJ * v;
-            pv[v] = (col < n) ? prow[col] * invp : 0.0;
+            const double pr_v = (col == kk) ? 1.0 : ((col < n) ? prow[col] : 0.0);
+            pv[v] = pr_v * invp;
         }
 #pragma unroll
         for (int u = 0; u < RR; ++u) {
             const int row = ti + 16 * u;
+            const bool isp = row == pr;
             const double ck = (row < n) ? colk[buf][row] : 0.0;
 #pragma unroll
             for (int v = 0; v < CC; ++v) {
                 const int col = tj + TJ * v;
-                if (row == pr)
-                    A[u][v] = (col == kk) ? invp : A[u][v] * invp;
-                else if (col == kk)
-                    A[u][v] = -ck * invp;
-                else
-                    A[u][v] = A[u][v] - ck * pv[v];
+                const double base = (col == kk) ? 0.0 : A[u][v];
+                const double upd = fma(-ck, pv[v], base);
+                A[u][v] = isp ? pv[v] : upd;
             }
         }
     }
