@@ -245,7 +245,8 @@ def _timed_solve(op, rhs, precision, make_pc=None, inner_rtol=1e-2):
                "iterations": int(info["iteration_number"]),
                "rel_residual": float(info["rel_residual"]),
                "converged": bool(info["converged_reason"] > 0),
-               "fallback": bool(info.get("fallback", False))}
+               "fallback": bool(info.get("fallback", False)),
+               "gated": bool(info.get("gated", False))}
 
 
 def _speedups(res, a, b, key):
@@ -318,6 +319,7 @@ def learned_section(fields=17, eval_seed=10_000, kappa_scale=1.0, log=None):
 
     def make_pc(bj):
         model = learned.CoarseModel(mlp, 3 * nc, xs, ys)
+        model.validation_error = hist[-1][1]
         model.set_coefficients(grid.coefficient_channels(wl.kappa, wl.sigma))
         return learned.TwoLevelPC(op, bj, grid, model, apply_precision=PREC_MIXED)
 

@@ -268,6 +268,9 @@ class CoarseModel:
         self.b0 = torch.empty(mlp.dims[1], dtype=torch.float32, device=dev)
         self.h = [torch.empty(d, dtype=torch.float32, device=dev) for d in mlp.dims[1:]]
         self.coef = None
+        # relative error on held-out pairs (set by the trainer's caller);
+        # Session gates the correction off when it is too large
+        self.validation_error = None
 
     def set_coefficients(self, channels):
         """b0' = b0 + W0[:, moments:] c for this problem's coefficient channels."""
@@ -528,10 +531,19 @@ class Session:
     FP32 / BF16 applies and block-Jacobi inverses stored in ``bj_precision``
     (PREC_MIXED: BF16, PREC_F32: FP32) on an FP32 (``basis`` "f32") or
     FP64 Arnoldi basis; "fp64": GMRES in FP64.  ``pc`` is a TwoLevelPC or
-    None (plain block-Jacobi)."""
+    None (plain block-Jacobi).  A coarse model whose held-out relative error
+    (``model.validation_error``) exceeds ``gate`` is not used at all: the
+    session runs plain block-Jacobi and reports ``gated``."""
 
     def __init__(self, op, pc=None, bj=None, precision="mixed", restart=None,
-                 bj_precision=PREC_MIXED, basis="f32"):
+                 bj_precision=PREC_MIXED, basis="f32", gate=0.5):
+        self.gated = False
+        if pc is not None:
+            ve = getattr(getattr(pc, "model", None), "validation_error", None)
+            if ve is not None and gate is not None and ve > gate:
+                self.gated = True
+                bj = pc.bj if bj is None else bj
+                pc = None
         # GMRES(50) as the reference's KSP (src/rt/solve.py:87) for "fp64";
         # the inner solves of mixed refinement restart after 20 steps
         # (shorter Gram-Schmidt, fewer total iterations on these problems)
@@ -574,17 +586,19 @@ class Session:
             x, info = refine(self.hi, self.lo, rhs, x0, rtol=rtol, inner_rtol=inner_rtol,
                              restart=self.restart, max_outer=max_outer, max_inner=max_it,
                              flexible=pc is not None, after_outer=jd.outer_hook, inner=self.gm)
-        info["fallback"] = jd.fell_back
+        info["fallback"] = jd.fell_back or self.gated
+        info["gated"] = self.gated
         return x, info
 
 
-def solve(op, rhs, x0=None, pc=None, bj=None, precision="mixed", rtol=1e-8, restart=50,
-          max_it=5000, inner_rtol=1e-2, max_outer=60, bj_precision=PREC_MIXED, judge=True):
+def solve(op, rhs, x0=None, pc=None, bj=None, precision="mixed", rtol=1e-8, restart=None,
+          max_it=5000, inner_rtol=1e-2, max_outer=60, bj_precision=PREC_MIXED, judge=True,
+          gate=0.5):
     """One solve with a fresh ``Session`` (see there)."""
     return Session(op, pc=pc, bj=bj, precision=precision, restart=restart,
-                   bj_precision=bj_precision).solve(rhs, x0, rtol=rtol, max_it=max_it,
-                                                    inner_rtol=inner_rtol,
-                                                    max_outer=max_outer, judge=judge)
+                   bj_precision=bj_precision, gate=gate).solve(rhs, x0, rtol=rtol, max_it=max_it,
+                                                               inner_rtol=inner_rtol,
+                                                               max_outer=max_outer, judge=judge)
 
 
 class _Judge:

@@ -319,3 +319,32 @@ def test_untrained_network_falls_back_safely(cuda, trained):
     assert info["fallback"]
     x_bj, _ = learned.solve(op, rhs, None, pc=None, bj=bj, precision="fp64", rtol=1e-10)
     assert float((x - x_bj).norm() / x_bj.norm()) < 1e-6
+
+
+@pytest.mark.gpu
+def test_model_with_large_validation_error_is_gated(cuda, trained):
+    """A coarse model whose held-out error exceeds the gate is not used: the
+    session solves with block-Jacobi alone (no trial step) and says so; a
+    model under the gate goes to the FP64 judge as before."""
+    from rtx import learned
+    from rtx.krylov import BlockJacobi
+    mesh, op, grid, _, _, _, _ = trained
+    wl = _diffusive(10_002, 16)
+    op.set_coeffs(wl.kappa, wl.sigma)
+    rhs = _rhs(op, wl)
+    bj = BlockJacobi(op, keep_f64=True, keep_bf16=True)
+    raw = learned.MLP([5 * grid.nc, 128, 128, 128, grid.nc], "cuda", seed=7)
+    model = learned.CoarseModel(raw, 3 * grid.nc, 1.0, 1.0)
+    model.set_coefficients(grid.coefficient_channels(wl.kappa, wl.sigma))
+    model.validation_error = 0.9
+    pc = learned.TwoLevelPC(op, bj, grid, model)
+    sess = learned.Session(op, pc=pc, bj=bj, precision="mixed")
+    x, info = sess.solve(rhs, rtol=1e-8)
+    assert info["gated"] and info["fallback"]
+    assert info["converged_reason"] > 0 and info["rel_residual"] <= 1e-8
+    assert pc.calls == 0
+    ref = learned.Session(op, bj=bj, precision="mixed").solve(rhs, rtol=1e-8)[0]
+    assert float((x - ref).norm() / ref.norm()) < 1e-7
+    model.validation_error = 0.1
+    x, info = learned.Session(op, pc=pc, bj=bj, precision="mixed").solve(rhs, rtol=1e-8)
+    assert not info["gated"] and info["converged_reason"] > 0
