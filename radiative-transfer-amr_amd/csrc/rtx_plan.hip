@@ -284,6 +284,12 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
                  d->col_G_off[c] == d->col_G_off[0];
         }
         for (int k = 0; k < nk && ok; ++k) ok = d->cell_nth[k] == NTH;
+        // the uniform kernel computes a column's first cell, DOF, spatial
+        // node and angle from the column index
+        for (int c = 0; c < nc && ok; ++c)
+            ok = d->col_cell_start[c] == c * NC &&
+                 int64_t(d->cell_dof_off[c * NC]) == int64_t(c) * N * N * NTH * NC &&
+                 d->col_sp_off[c] == c * N * N && d->col_ang_start[c] == c * NTH * NC;
         // every link must couple equal shapes (trace of N x NTH values)
         for (int l = 0; l < nl && ok; ++l) {
             const int s = d->link_src[l];

@@ -72,8 +72,10 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     if (c >= m.ncol) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int k0 = m.col_cell_start[c];
-    const int cdof = m.cell_dof_off[k0];
+    // the plan admits a uniform bin only when columns are laid out
+    // contiguously in index order, so no table lookup precedes the x loads
+    const int k0 = c * NC;
+    const int cdof = c * NCD;
     const int kk_lo = h * OC;
     const int own0 = kk_lo * CD;
     const int L0 = m.link_start[k0 + kk_lo];
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     const bool bconv = (ops & RTX_OP_BCONV) != 0;
     const double dx = m.col_geom[4 * c + 2] - m.col_geom[4 * c + 0];
     const double dy = m.col_geom[4 * c + 3] - m.col_geom[4 * c + 1];
-    const int ga0 = m.col_ang_start[c];
+    const int ga0 = c * NA;
 
     // ---- batched loads issued before any use
     typedef typename std::conditional<PREC == RTX_PREC_F64, double, float>::type EB;
@@ -115,8 +117,8 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
         r_w = m.ang_trig[3 * (ga0 + tid) + 2];
     }
     if (tid < NSP) {
-        r_kap = m.kappa[m.col_sp_off[c] + tid];
-        r_sig = m.sigma[m.col_sp_off[c] + tid];
+        r_kap = m.kappa[c * NSP + tid];
+        r_sig = m.sigma[c * NSP + tid];
     }
     if (tid < N * N) {
         const int p = tid / N, i = tid % N;
