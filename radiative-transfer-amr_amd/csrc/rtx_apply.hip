@@ -21,6 +21,8 @@ namespace rtx {
 constexpr int NM = RTX_NMAX;
 constexpr int SCAT_THREADS = 256;
 constexpr int SC_KC = 32;                // angles per K step of the tiled scattering
+constexpr int TT = 64;                   // threads per cell of k_transport (one wavefront;
+                                         // four per cell measured slower)
 constexpr int MAX_NANG = 1024;           // angular nodes per column
 
 __device__ inline bool on_bdry(int bd, int i, int j, int nx, int ny) {
@@ -34,7 +36,7 @@ __device__ inline double theta_f(int F, double c, double s) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int accumulate,
+__global__ __launch_bounds__(TT) void k_transport(DevMesh m, uint32_t ops, int accumulate,
                                                   const int32_t *__restrict__ cells,
                                                   const T *__restrict__ x, T *__restrict__ y) {
     // x and y of the cell (dynamic: twice the plan's largest cell), the
@@ -59,21 +61,21 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
     const int nyt = ny * nth;
     const int sp0 = m.col_sp_off[c];
 
-    for (int t = lane; t < n; t += 64) {
+    for (int t = lane; t < n; t += TT) {
         const int i = t / nyt, j = (t / nth) % ny;
         double v = double(x[off + t]);
         if (bd && on_bdry(bd, i, j, nx, ny)) v = 0.0;
         xs[t] = v;
     }
-    for (int t = lane; t < nx * nx; t += 64) {
+    for (int t = lane; t < nx * nx; t += TT) {
         const int pp = t / nx, i = t % nx;
         Dxw[t] = m.lgl_w[nx * NM + i] * m.lgl_D[(size_t(nx) * NM + pp) * NM + i];
     }
-    for (int t = lane; t < ny * ny; t += 64) {
+    for (int t = lane; t < ny * ny; t += TT) {
         const int q = t / ny, j = t % ny;
         Dyw[t] = m.lgl_w[ny * NM + j] * m.lgl_D[(size_t(ny) * NM + q) * NM + j];
     }
-    for (int t = lane; t < nx * ny; t += 64) kap[t] = m.kappa[sp0 + t];
+    for (int t = lane; t < nx * ny; t += TT) kap[t] = m.kappa[sp0 + t];
     if (lane < nx) wxs[lane] = m.lgl_w[nx * NM + lane];
     if (lane < ny) wys[lane] = m.lgl_w[ny * NM + lane];
     if (lane < nth) {
@@ -88,7 +90,7 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
 
     const int quad = m.cell_quad[k];
     const double vol = dx * dy * dth * 0.125;
-    for (int t = lane; t < n; t += 64) {
+    for (int t = lane; t < n; t += TT) {
         const int p = t / nyt, q = (t / nth) % ny, r = t % nth;
         double acc = 0.0;
         const double w3 = vol * wxs[p] * wys[q] * wts[r];
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
             const double *Es = m.Es_pool + m.link_Es_off[l];
             const double *Et = m.Eth_pool + m.link_Eth_off[l];
             __syncthreads();                      // tr, inn, Ets, Ess reuse
-            for (int t = lane; t < n1 * nt1; t += 64) {
+            for (int t = lane; t < n1 * nt1; t += TT) {
                 const int m1 = t / nt1, a = t % nt1;
                 int i1, j1;
                 if (F == 0) { i1 = 0; j1 = m1; }
@@ -142,11 +144,11 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
                 if (bd1 && on_bdry(bd1, i1, j1, nx1, ny1)) v = 0.0;
                 tr[t] = v;
             }
-            for (int t = lane; t < nt1 * nth; t += 64) Ets[t] = Et[t];
-            for (int t = lane; t < n1 * n0; t += 64) Ess[t] = Es[t];
+            for (int t = lane; t < nt1 * nth; t += TT) Ets[t] = Et[t];
+            for (int t = lane; t < n1 * n0; t += TT) Ess[t] = Es[t];
             __syncthreads();
             // sum-factorised: angle first (n1 x nth), then space (n0 x nth)
-            for (int t = lane; t < n1 * nth; t += 64) {
+            for (int t = lane; t < n1 * nth; t += TT) {
                 const int m1 = t / nth, r = t % nth;
                 double v = 0.0;
                 for (int a = 0; a < nt1; ++a) v += Ets[a * nth + r] * tr[m1 * nt1 + a];
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
             }
             __syncthreads();
             const double cf = ((F & 1) ? dx : dy) * dth * 0.25;
-            for (int t = lane; t < n0 * nth; t += 64) {
+            for (int t = lane; t < n0 * nth; t += TT) {
                 const int m0 = t / nth, r = t % nth;
                 double acc = 0.0;
                 for (int m1 = 0; m1 < n1; ++m1) acc += Ess[m1 * n0 + m0] * inn[m1 * nth + r];
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(64) void k_transport(DevMesh m, uint32_t ops, int a
         }
     }
     __syncthreads();
-    for (int t = lane; t < n; t += 64) {
+    for (int t = lane; t < n; t += TT) {
         const int i = t / nyt, j = (t / nth) % ny;
         const bool masked = restrict_ && on_bdry(m.cell_bdry[k], i, j, nx, ny);
         double v = masked ? 0.0 : ys[t];
@@ -337,7 +339,7 @@ static int launch_apply(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hip
         return RTX_OK;
     }
     if (tp && ncells > 0)
-        k_transport<T><<<ncells, 64, size_t(2) * p->max_cell_ndof * sizeof(double), st>>>(
+        k_transport<T><<<ncells, TT, size_t(2) * p->max_cell_ndof * sizeof(double), st>>>(
             p->m, ops, acc ? 1 : 0, cells, x, y);
     if (sc && nitems > 0)
         k_scatter_mma<T><<<nitems, SCAT_THREADS, 0, st>>>(p->m, ops, (acc || tp) ? 1 : 0, items,
