@@ -121,51 +121,89 @@ __global__ __launch_bounds__(TT) void k_transport(DevMesh m, uint32_t ops, int a
     }
 
     if (ops & RTX_OP_BCONV) {
-        const int l0 = m.link_start[k], l1 = m.link_start[k + 1];
-        for (int l = l0; l < l1; ++l) {
-            const int s = m.link_src[l], F = m.link_face[l];
-            const int c1 = m.cell_col[s];
-            const int nx1 = m.col_nx[c1], ny1 = m.col_ny[c1], nt1 = m.cell_nth[s];
-            const int off1 = m.cell_dof_off[s];
-            const int bd1 = restrict_ ? m.cell_bdry[s] : 0;
-            const int n1 = (F & 1) ? nx1 : ny1;   // neighbour trace length
-            const int n0 = (F & 1) ? nx : ny;     // own trace length
-            const double *Es = m.Es_pool + m.link_Es_off[l];
-            const double *Et = m.Eth_pool + m.link_Eth_off[l];
-            __syncthreads();                      // tr, inn, Ets, Ess reuse
-            for (int t = lane; t < n1 * nt1; t += TT) {
-                const int m1 = t / nt1, a = t % nt1;
-                int i1, j1;
-                if (F == 0) { i1 = 0; j1 = m1; }
-                else if (F == 2) { i1 = nx1 - 1; j1 = m1; }
-                else if (F == 1) { i1 = m1; j1 = 0; }
-                else { i1 = m1; j1 = ny1 - 1; }
-                double v = double(x[off1 + (i1 * ny1 + j1) * nt1 + a]);
-                if (bd1 && on_bdry(bd1, i1, j1, nx1, ny1)) v = 0.0;
-                tr[t] = v;
+        // Upwind links, up to 64 at a time: lane li holds link li's metadata
+        // (one round of loads for all links), and the next link's trace and
+        // E blocks are fetched into registers while the current one computes.
+        static_assert(TT == 64, "link metadata is held one link per lane of the wavefront");
+        const int l0 = m.link_start[k], nlk = m.link_start[k + 1] - l0;
+        for (int c0 = 0; c0 < nlk; c0 += TT) {
+            const int ncl = (nlk - c0) < TT ? (nlk - c0) : TT;
+            int mF = 0, moff = 0, mnx = 1, mny = 1, mnt = 1, mbd = 0, mes = 0, met = 0;
+            if (lane < ncl) {
+                const int l = l0 + c0 + lane;
+                const int s = m.link_src[l];
+                mF = m.link_face[l];
+                mes = m.link_Es_off[l];
+                met = m.link_Eth_off[l];
+                const int c1 = m.cell_col[s];
+                mnx = m.col_nx[c1];
+                mny = m.col_ny[c1];
+                mnt = m.cell_nth[s];
+                moff = m.cell_dof_off[s];
+                mbd = restrict_ ? m.cell_bdry[s] : 0;
             }
-            for (int t = lane; t < nt1 * nth; t += TT) Ets[t] = Et[t];
-            for (int t = lane; t < n1 * n0; t += TT) Ess[t] = Es[t];
-            __syncthreads();
-            // sum-factorised: angle first (n1 x nth), then space (n0 x nth)
-            for (int t = lane; t < n1 * nth; t += TT) {
-                const int m1 = t / nth, r = t % nth;
-                double v = 0.0;
-                for (int a = 0; a < nt1; ++a) v += Ets[a * nth + r] * tr[m1 * nt1 + a];
-                inn[t] = v;
-            }
-            __syncthreads();
-            const double cf = ((F & 1) ? dx : dy) * dth * 0.25;
-            for (int t = lane; t < n0 * nth; t += TT) {
-                const int m0 = t / nth, r = t % nth;
-                double acc = 0.0;
-                for (int m1 = 0; m1 < n1; ++m1) acc += Ess[m1 * n0 + m0] * inn[m1 * nth + r];
-                int node;
-                if (F == 0) node = (nx - 1) * nyt + m0 * nth + r;
-                else if (F == 2) node = m0 * nth + r;
-                else if (F == 1) node = m0 * nyt + (ny - 1) * nth + r;
-                else node = m0 * nyt + r;
-                ys[node] += cf * acc;   // one lane per node: no conflict
+            constexpr int PU = (NM * NM + TT - 1) / TT;    // register slots per lane
+            double pt[PU], pe[PU], ps[PU];
+            auto fetch = [&](int li) {
+                const int F = __shfl(mF, li), nx1 = __shfl(mnx, li), ny1 = __shfl(mny, li);
+                const int nt1 = __shfl(mnt, li), off1 = __shfl(moff, li), bd1 = __shfl(mbd, li);
+                const int es = __shfl(mes, li), et = __shfl(met, li);
+                const int n1 = (F & 1) ? nx1 : ny1, n0 = (F & 1) ? nx : ny;
+#pragma unroll
+                for (int u = 0; u < PU; ++u) {
+                    const int t = lane + u * TT;
+                    double v = 0.0;
+                    if (t < n1 * nt1) {
+                        const int m1 = t / nt1, a = t % nt1;
+                        int i1, j1;
+                        if (F == 0) { i1 = 0; j1 = m1; }
+                        else if (F == 2) { i1 = nx1 - 1; j1 = m1; }
+                        else if (F == 1) { i1 = m1; j1 = 0; }
+                        else { i1 = m1; j1 = ny1 - 1; }
+                        v = double(x[off1 + (i1 * ny1 + j1) * nt1 + a]);
+                        if (bd1 && on_bdry(bd1, i1, j1, nx1, ny1)) v = 0.0;
+                    }
+                    pt[u] = v;
+                    pe[u] = (t < nt1 * nth) ? m.Eth_pool[et + t] : 0.0;
+                    ps[u] = (t < n1 * n0) ? m.Es_pool[es + t] : 0.0;
+                }
+            };
+            if (ncl > 0) fetch(0);
+            for (int li = 0; li < ncl; ++li) {
+                const int F = __shfl(mF, li), nx1 = __shfl(mnx, li), ny1 = __shfl(mny, li);
+                const int nt1 = __shfl(mnt, li);
+                const int n1 = (F & 1) ? nx1 : ny1;   // neighbour trace length
+                const int n0 = (F & 1) ? nx : ny;     // own trace length
+                __syncthreads();                      // tr, inn, Ets, Ess reuse
+#pragma unroll
+                for (int u = 0; u < PU; ++u) {
+                    const int t = lane + u * TT;
+                    if (t < n1 * nt1) tr[t] = pt[u];
+                    if (t < nt1 * nth) Ets[t] = pe[u];
+                    if (t < n1 * n0) Ess[t] = ps[u];
+                }
+                __syncthreads();
+                if (li + 1 < ncl) fetch(li + 1);      // in flight during this link's sums
+                // sum-factorised: angle first (n1 x nth), then space (n0 x nth)
+                for (int t = lane; t < n1 * nth; t += TT) {
+                    const int m1 = t / nth, r = t % nth;
+                    double v = 0.0;
+                    for (int a = 0; a < nt1; ++a) v += Ets[a * nth + r] * tr[m1 * nt1 + a];
+                    inn[t] = v;
+                }
+                __syncthreads();
+                const double cf = ((F & 1) ? dx : dy) * dth * 0.25;
+                for (int t = lane; t < n0 * nth; t += TT) {
+                    const int m0 = t / nth, r = t % nth;
+                    double acc = 0.0;
+                    for (int m1 = 0; m1 < n1; ++m1) acc += Ess[m1 * n0 + m0] * inn[m1 * nth + r];
+                    int node;
+                    if (F == 0) node = (nx - 1) * nyt + m0 * nth + r;
+                    else if (F == 2) node = m0 * nth + r;
+                    else if (F == 1) node = m0 * nyt + (ny - 1) * nth + r;
+                    else node = m0 * nyt + r;
+                    ys[node] += cf * acc;   // one lane per node: no conflict
+                }
             }
         }
     }
