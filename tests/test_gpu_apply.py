@@ -265,3 +265,31 @@ def test_uniform_parts_and_autotune(cuda):
         y = op.apply(_dev(x), ops=OP_SYSTEM | OP_RESTRICT).cpu().numpy()
         assert _rel(y, ref) < TOL64, (flags, _rel(y, ref))
     assert op.plan_info()["bins_on_generic"] == []
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pbcs", [(True, False), (True, True)], ids=["periodic-x", "periodic-xy"])
+@pytest.mark.parametrize("path", [-1, 0], ids=["auto", "generic"])
+def test_apply_periodic_matches_oracle(cuda, pbcs, path):
+    """Periodic meshes (``Mesh(pbcs=...)``, class_Mesh.py:46-53): columns on
+    opposite walls are face neighbours and no wall in a periodic direction
+    is an inflow boundary; every part against the oracle's matrices."""
+    from rtx.operator import (TransportOperator, OP_MASS, OP_SCAT, OP_ICONV, OP_BCONV,
+                              OP_SYSTEM, OP_RESTRICT)
+    mesh = problems.uniform_mesh([1.0, 1.0], [4, 4, 4], 2, 2, pbcs=pbcs)
+    if path == 0:
+        keys = sorted(mesh.cols)
+        mesh.ref_col(keys[0], "spt", "p")
+    wl = problems.manufactured()
+    V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi)
+    A = (V["BC"] - V["IC"] + V["M"] + V["S"]).tocsr()
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    op.set_path(path)
+    x = np.random.default_rng(9).uniform(-1.0, 1.0, op.ndof)
+    xd = _dev(x)
+    for name, ops, ref in [("BC", OP_BCONV, V["BC"] @ x), ("A", OP_SYSTEM, A @ x)]:
+        y = op.apply(xd, ops=ops).cpu().numpy()
+        assert _rel(y, ref) < TOL64, (name, _rel(y, ref))
+    mask = asm.intr_mask(mesh)
+    y = op.apply(xd, ops=OP_SYSTEM | OP_RESTRICT).cpu().numpy()
+    assert _rel(y, mask * (A @ (mask * x))) < TOL64

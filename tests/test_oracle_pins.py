@@ -802,3 +802,22 @@ def test_coarse_average_and_prolong_closed_forms():
         inside = np.array([int(0.5 * (x0 + x1) * n) * n + int(0.5 * (y0 + y1) * n) == g
                            for (x0, y0, x1, y1) in geo])[colid]
         assert np.allclose(z, np.where(inside & mask, 0.7, 0.0), atol=1e-14)
+
+
+@pytest.mark.parametrize("pbcs", [(True, False), (False, True), (True, True)])
+def test_periodic_boundary_convection_loops_vs_vectorised(pbcs):
+    """Periodic meshes: the literal loops and the vectorised assembly agree
+    on BC, a periodic direction has no inflow boundary, and on a conforming
+    periodic mesh every cell's 1^T (BC - IC) 1 vanishes (no wall terms
+    remain; opposite faces cancel)."""
+    mesh = problems.uniform_mesh([1.0, 1.0], [3, 3, 3], 1, 2, pbcs=pbcs)
+    wl = problems.manufactured()
+    V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi)
+    BCl = loops.boundary_conv(mesh)
+    assert abs(V["BC"] - BCl).max() < 1e-13
+    if all(pbcs):
+        assert asm.intr_mask(mesh).all()
+        T = (V["BC"] - V["IC"]).tocsr()
+        for o, n in system.cell_blocks(mesh):
+            tot = T[o:o + n].sum()
+            assert abs(tot) < 1e-12, tot
