@@ -21,6 +21,10 @@ namespace rtx {
 
 constexpr int NMq = RTX_NMAX;
 constexpr int BJ_THREADS = 256;
+#ifndef BJ_APPLY16_THREADS
+#define BJ_APPLY16_THREADS 512   // threads per block of the packed BF16 apply (measured
+                                 // best of 128, 256, 512, 1024 on 125-DOF blocks)
+#endif
 constexpr int BJ_COLS = 128;     // column threads in the elimination
 constexpr int BJ_SPLIT = 128;    // larger cells are split into groups of angles
 
@@ -558,33 +562,34 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_pack_bf16(DevMesh m, const in
 // z = B r for one block from the packed BF16 inverse: thread (cg, rg) owns
 // rows 8 rg .. 8 rg + 7 and columns cg, cg + CG, ...; one 16-byte load per
 // column; the CG partial sums of each row are added in LDS.
-__global__ __launch_bounds__(BJ_THREADS) void k_bj_apply_bf16(DevMesh m, const int64_t *off16,
+template <int NT>
+__global__ __launch_bounds__(NT) void k_bj_apply_bf16(DevMesh m, const int64_t *off16,
                                                               const int32_t *__restrict__ blk,
                                                               const uint16_t *__restrict__ inv,
                                                               const float *__restrict__ r,
                                                               float *__restrict__ z) {
     __shared__ float rs[RTX_NMAX * RTX_NMAX * RTX_NMAX];
-    __shared__ float part[BJ_THREADS * 8];
+    __shared__ float part[NT * 8];
     const int b = blockIdx.x, tid = threadIdx.x;
     const int k = blk[3 * b], a0 = blk[3 * b + 1], g = blk[3 * b + 2];
     const int c = m.cell_col[k];
     const int nth = m.cell_nth[k];
     const int n = m.col_nx[c] * m.col_ny[c] * g;
-    const int ldp = (n + 7) / 8 * 8, RG = ldp / 8, CG = BJ_THREADS / RG;
+    const int ldp = (n + 7) / 8 * 8, RG = ldp / 8, CG = NT / RG;
     const int off = m.cell_dof_off[k] + a0;
     const int rg = tid % RG, cg = tid / RG;
     const uint4 *B = reinterpret_cast<const uint4 *>(inv + off16[b]) + rg;
     const int ldq = ldp / 8;
     // the first 8 columns of this thread are fetched before the wait on r,
     // so the inverse loads and the r gather are in flight together
-    constexpr int PF = 8;
+    constexpr int PF = 2048 / NT;
     uint4 q[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
         const int j = cg + u * CG;
         q[u] = (cg < CG && j < n) ? B[size_t(j) * ldq] : make_uint4(0u, 0u, 0u, 0u);
     }
-    for (int i = tid; i < n; i += BJ_THREADS) rs[i] = r[off + (i / g) * nth + i % g];
+    for (int i = tid; i < n; i += NT) rs[i] = r[off + (i / g) * nth + i % g];
     __syncthreads();
     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     auto fma8 = [&](const uint4 &v, float rj) {
@@ -607,7 +612,7 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_apply_bf16(DevMesh m, const i
         for (int u = 0; u < 8; ++u) part[cg * ldp + rg * 8 + u] = acc[u];
     }
     __syncthreads();
-    for (int i = tid; i < n; i += BJ_THREADS) {
+    for (int i = tid; i < n; i += NT) {
         float s = 0.f;
         for (int q = 0; q < CG; ++q) s += part[q * ldp + i];
         z[off + (i / g) * nth + i % g] = s;
@@ -786,7 +791,7 @@ extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, 
         k_bj_apply<float, float><<<p->bj_nblk, BJ_THREADS, 0, st>>>(
             p->m, p->bj_off, p->bj_blk, (const float *)d_inv, (const float *)d_r, (float *)d_z);
     else if (precision == RTX_PREC_MIXED)
-        k_bj_apply_bf16<<<p->bj_nblk, BJ_THREADS, 0, st>>>(
+        k_bj_apply_bf16<BJ_APPLY16_THREADS><<<p->bj_nblk, BJ_APPLY16_THREADS, 0, st>>>(
             p->m, p->bj_off16, p->bj_blk, (const uint16_t *)d_inv, (const float *)d_r,
             (float *)d_z);
     else {
