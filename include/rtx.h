@@ -47,8 +47,10 @@ enum {
 enum {
     RTX_PREC_F64 = 0,      /* everything in FP64                                    */
     RTX_PREC_F32 = 1,      /* x, y FP32; FP32 arithmetic                            */
-    RTX_PREC_MIXED = 2     /* x, y FP32; convection FP32, scattering BF16 on MFMA
+    RTX_PREC_MIXED = 2,    /* x, y FP32; convection FP32, scattering BF16 on MFMA
                               with FP32 accumulation                                */
+    RTX_PREC_FP8 = 3       /* block-Jacobi apply only: FP8 (e4m3) inverses with an
+                              FP32 scale per column, FP32 vectors and arithmetic    */
 };
 
 /* Largest supported number of nodes per direction in a cell. */
@@ -215,10 +217,19 @@ int rtx_bjacobi_size_bf16(rtx_plan *plan, int64_t *n_entries);
    into the packed pool. */
 int rtx_bjacobi_pack_bf16(rtx_plan *plan, const float *d_inv32, uint16_t *d_inv16,
                           void *stream);
+/* Bytes of the FP8 inverse pool: e4m3 codes with every block's columns
+   padded to a multiple of 16 rows (16-byte columns), then one FP32 scale
+   per column (at a 16-byte aligned offset). */
+int rtx_bjacobi_size_fp8(rtx_plan *plan, int64_t *n_bytes);
+/* Quantise the FP32 inverses to FP8 e4m3 (nearest even) with the scale
+   max |column| / 448 per column, into the FP8 pool.  No counterpart in the
+   reference. */
+int rtx_bjacobi_pack_fp8(rtx_plan *plan, const float *d_inv32, uint8_t *d_inv8, void *stream);
 /* z = blockdiag(inv) r, precision RTX_PREC_F64 (double inverses, vectors),
-   RTX_PREC_F32 (float inverses, vectors) or RTX_PREC_MIXED (the packed BF16
-   pool of rtx_bjacobi_pack_bf16, float vectors, FP32 accumulation; the
-   factorisation itself is always FP64). */
+   RTX_PREC_F32 (float inverses, vectors), RTX_PREC_MIXED (the packed BF16
+   pool of rtx_bjacobi_pack_bf16, float vectors, FP32 accumulation) or
+   RTX_PREC_FP8 (the pool of rtx_bjacobi_pack_fp8, float vectors); the
+   factorisation itself is always FP64. */
 int rtx_bjacobi_apply(rtx_plan *plan, int precision, const void *d_inv, const void *d_r,
                       void *d_z, void *stream);
 

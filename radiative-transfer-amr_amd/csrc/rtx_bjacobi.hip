@@ -21,6 +21,9 @@ namespace rtx {
 
 constexpr int NMq = RTX_NMAX;
 constexpr int BJ_THREADS = 256;
+#ifndef BJ_APPLY8_THREADS
+#define BJ_APPLY8_THREADS 256    // threads per block of the FP8 apply
+#endif
 #ifndef BJ_APPLY16_THREADS
 #define BJ_APPLY16_THREADS 512   // threads per block of the packed BF16 apply (measured
                                  // best of 128, 256, 512, 1024 on 125-DOF blocks)
@@ -619,6 +622,127 @@ __global__ __launch_bounds__(NT) void k_bj_apply_bf16(DevMesh m, const int64_t *
     }
 }
 
+// FP8 e4m3 (bias 7, largest finite 448, no infinities): nearest code,
+// ties to even; |v| above 448 saturates.
+__device__ inline uint8_t bj_to_e4m3(float v) {
+    const uint8_t sgn = v < 0.f ? 0x80 : 0;
+    const float a = fabsf(v);
+    if (!(a > 0.f)) return sgn;
+    if (a >= 448.f) return sgn | 0x7E;
+    int e;
+    (void)frexpf(a, &e);                       // a = f 2^e, f in [0.5, 1)
+    int E = e - 1;                             // a = (1 + m / 8) 2^E for normals
+    if (E < -6) {                              // subnormals: m 2^-9
+        const int mq = int(rintf(a * 512.f));
+        return sgn | uint8_t(mq >= 8 ? 8 : mq);
+    }
+    int mq = int(rintf((ldexpf(a, -E) - 1.f) * 8.f));
+    if (mq == 8) {
+        mq = 0;
+        ++E;
+    }
+    const int eb = E + 7;
+    if (eb > 15 || (eb == 15 && mq > 6)) return sgn | 0x7E;
+    return sgn | uint8_t((eb << 3) | mq);
+}
+
+// FP8 copy of block b: column j of codes at off8[b] + j * ldp (ldp = n
+// rounded up to 16, padding zero), scale_j = max_i |B_ij| / 448 at
+// scales[soff[b] + j]; code = e4m3(B_ij / scale_j).
+__global__ __launch_bounds__(BJ_THREADS) void k_bj_pack_fp8(DevMesh m, const int64_t *off,
+                                                            const int64_t *off8,
+                                                            const int64_t *soff,
+                                                            const int32_t *__restrict__ blk,
+                                                            const float *__restrict__ src,
+                                                            uint8_t *__restrict__ dst,
+                                                            float *__restrict__ scales) {
+    __shared__ float inv_s[RTX_NMAX * RTX_NMAX * RTX_NMAX];
+    const int b = blockIdx.x;
+    const int n = bj_block_n(m, blk, b), ldp = (n + 15) / 16 * 16;
+    const float *S = src + off[b];
+    uint8_t *D = dst + off8[b];
+    for (int j = threadIdx.x; j < n; j += BJ_THREADS) {
+        float mx = 0.f;
+        for (int i = 0; i < n; ++i) mx = fmaxf(mx, fabsf(S[size_t(j) * n + i]));
+        const float sc = mx > 0.f ? mx / 448.f : 1.f;
+        scales[soff[b] + j] = sc;
+        inv_s[j] = 1.f / sc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * ldp; e += BJ_THREADS) {
+        const int j = e / ldp, i = e % ldp;
+        D[e] = i < n ? bj_to_e4m3(S[size_t(j) * n + i] * inv_s[j]) : uint8_t(0);
+    }
+}
+
+// z = B r from the FP8 pool: thread (cg, rg) owns rows 16 rg .. 16 rg + 15
+// and columns cg, cg + CG, ...; one 16-byte load of codes per column,
+// decoded by the hardware conversion; the column scales are folded into r.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *off8,
+                                                     const int64_t *soff,
+                                                     const int32_t *__restrict__ blk,
+                                                     const uint8_t *__restrict__ inv,
+                                                     const float *__restrict__ scales,
+                                                     const float *__restrict__ r,
+                                                     float *__restrict__ z) {
+    __shared__ float rs[RTX_NMAX * RTX_NMAX * RTX_NMAX];
+    __shared__ float part[NT * 16];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int k = blk[3 * b], a0 = blk[3 * b + 1], g = blk[3 * b + 2];
+    const int c = m.cell_col[k];
+    const int nth = m.cell_nth[k];
+    const int n = m.col_nx[c] * m.col_ny[c] * g;
+    const int ldp = (n + 15) / 16 * 16, RG = ldp / 16, CG = NT / RG;
+    const int off = m.cell_dof_off[k] + a0;
+    const int rg = tid % RG, cg = tid / RG;
+    const uint4 *B = reinterpret_cast<const uint4 *>(inv + off8[b]) + rg;
+    const int ldq = ldp / 16;
+    constexpr int PF = 4;
+    uint4 q[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        const int j = cg + u * CG;
+        q[u] = (cg < CG && j < n) ? B[size_t(j) * ldq] : make_uint4(0u, 0u, 0u, 0u);
+    }
+    const float *sc = scales + soff[b];
+    for (int i = tid; i < n; i += NT) rs[i] = r[off + (i / g) * nth + i % g] * sc[i];
+    __syncthreads();
+    float acc[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) acc[u] = 0.f;
+    // the hardware FP8 (e4m3) to FP32 conversion, two codes per instruction
+    typedef float bj_f2 __attribute__((ext_vector_type(2)));
+    auto fma16 = [&](const uint4 &v, float rj) {
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bj_f2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(int(w[u]), false);
+            const bj_f2 hi = __builtin_amdgcn_cvt_pk_f32_fp8(int(w[u]), true);
+            acc[4 * u + 0] += lo.x * rj;
+            acc[4 * u + 1] += lo.y * rj;
+            acc[4 * u + 2] += hi.x * rj;
+            acc[4 * u + 3] += hi.y * rj;
+        }
+    };
+    if (cg < CG) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int j = cg + u * CG;
+            if (j < n) fma16(q[u], rs[j]);
+        }
+        for (int j = cg + PF * CG; j < n; j += CG) fma16(B[size_t(j) * ldq], rs[j]);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) part[cg * ldp + rg * 16 + u] = acc[u];
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) {
+        float s = 0.f;
+        for (int qq = 0; qq < CG; ++qq) s += part[qq * ldp + i];
+        z[off + (i / g) * nth + i % g] = s;
+    }
+}
+
 }  // namespace rtx
 
 using namespace rtx;
@@ -632,7 +756,7 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
     RTX_HIP(hipMemcpy(nt.data(), p->m.cell_nth, nt.size() * 4, hipMemcpyDeviceToHost));
     const int split = (p->bj_method != 0) ? 1 : 0;
     std::vector<int32_t> blk;
-    std::vector<int64_t> off(1, 0), off16(1, 0);
+    std::vector<int64_t> off(1, 0), off16(1, 0), off8(1, 0), soff(1, 0);
     int32_t maxb = 0;
     for (int k = 0; k < p->m.ncell; ++k) {
         const int nsp = nx[cc[k]] * ny[cc[k]], nth = nt[k];
@@ -645,20 +769,30 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
             blk.push_back(gg);
             off.push_back(off.back() + n * n);
             off16.push_back(off16.back() + n * ((n + 7) / 8 * 8));
+            off8.push_back(off8.back() + n * ((n + 15) / 16 * 16));
+            soff.push_back(soff.back() + n);
             maxb = std::max<int32_t>(maxb, int32_t(n));
         }
     }
     if (p->bj_split != split || !p->bj_off) {
         if (p->bj_off) RTX_HIP(hipFree(p->bj_off));
         if (p->bj_off16) RTX_HIP(hipFree(p->bj_off16));
+        if (p->bj_off8) RTX_HIP(hipFree(p->bj_off8));
+        if (p->bj_soff) RTX_HIP(hipFree(p->bj_soff));
         if (p->bj_blk) RTX_HIP(hipFree(p->bj_blk));
-        p->bj_off = p->bj_off16 = nullptr;
+        p->bj_off = p->bj_off16 = p->bj_off8 = p->bj_soff = nullptr;
         p->bj_blk = nullptr;
         RTX_HIP(hipMalloc(&p->bj_off, off.size() * sizeof(int64_t)));
         RTX_HIP(hipMemcpy(p->bj_off, off.data(), off.size() * sizeof(int64_t),
                           hipMemcpyHostToDevice));
         RTX_HIP(hipMalloc(&p->bj_off16, off16.size() * sizeof(int64_t)));
         RTX_HIP(hipMemcpy(p->bj_off16, off16.data(), off16.size() * sizeof(int64_t),
+                          hipMemcpyHostToDevice));
+        RTX_HIP(hipMalloc(&p->bj_off8, off8.size() * sizeof(int64_t)));
+        RTX_HIP(hipMemcpy(p->bj_off8, off8.data(), off8.size() * sizeof(int64_t),
+                          hipMemcpyHostToDevice));
+        RTX_HIP(hipMalloc(&p->bj_soff, soff.size() * sizeof(int64_t)));
+        RTX_HIP(hipMemcpy(p->bj_soff, soff.data(), soff.size() * sizeof(int64_t),
                           hipMemcpyHostToDevice));
         RTX_HIP(hipMalloc(&p->bj_blk, blk.size() * sizeof(int32_t)));
         RTX_HIP(hipMemcpy(p->bj_blk, blk.data(), blk.size() * sizeof(int32_t),
@@ -670,7 +804,31 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
     p->bj_max_block = maxb;
     p->bj_entries = off.back();
     p->bj_entries16 = off16.back();
+    p->bj_codes8 = (off8.back() + 15) / 16 * 16;
+    p->bj_bytes8 = p->bj_codes8 + soff.back() * int64_t(sizeof(float));
     *n_entries = off.back();
+    return RTX_OK;
+}
+
+extern "C" int rtx_bjacobi_size_fp8(rtx_plan *p, int64_t *n_bytes) {
+    RTX_CHECK(p && n_bytes, "rtx_bjacobi_size_fp8: null argument");
+    if (!p->bj_off8) {
+        int64_t n;
+        const int rc = rtx_bjacobi_size(p, &n);
+        if (rc) return rc;
+    }
+    *n_bytes = p->bj_bytes8;
+    return RTX_OK;
+}
+
+extern "C" int rtx_bjacobi_pack_fp8(rtx_plan *p, const float *d_inv32, uint8_t *d_inv8,
+                                    void *stream) {
+    RTX_CHECK(p && d_inv32 && d_inv8, "rtx_bjacobi_pack_fp8: null argument");
+    RTX_CHECK(p->bj_off8 && p->bj_nblk > 0, "rtx_bjacobi_pack_fp8: call rtx_bjacobi_size first");
+    k_bj_pack_fp8<<<p->bj_nblk, BJ_THREADS, 0, static_cast<hipStream_t>(stream)>>>(
+        p->m, p->bj_off, p->bj_off8, p->bj_soff, p->bj_blk, d_inv32, d_inv8,
+        reinterpret_cast<float *>(d_inv8 + p->bj_codes8));
+    RTX_HIP(hipGetLastError());
     return RTX_OK;
 }
 
@@ -794,6 +952,11 @@ extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, 
         k_bj_apply_bf16<BJ_APPLY16_THREADS><<<p->bj_nblk, BJ_APPLY16_THREADS, 0, st>>>(
             p->m, p->bj_off16, p->bj_blk, (const uint16_t *)d_inv, (const float *)d_r,
             (float *)d_z);
+    else if (precision == RTX_PREC_FP8)
+        k_bj_apply_fp8<BJ_APPLY8_THREADS><<<p->bj_nblk, BJ_APPLY8_THREADS, 0, st>>>(
+            p->m, p->bj_off8, p->bj_soff, p->bj_blk, (const uint8_t *)d_inv,
+            reinterpret_cast<const float *>((const uint8_t *)d_inv + p->bj_codes8),
+            (const float *)d_r, (float *)d_z);
     else {
         set_error("rtx_bjacobi_apply: unsupported precision %d", precision);
         return RTX_ERR_ARG;

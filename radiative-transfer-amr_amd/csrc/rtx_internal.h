@@ -148,6 +148,11 @@ struct rtx_plan {
     // packed BF16 copy: columns padded to a multiple of 8 rows (16 bytes)
     int64_t *bj_off16 = nullptr;
     int64_t bj_entries16 = 0;
+    // FP8 copy: codes with columns padded to a multiple of 16 rows, then an
+    // FP32 scale per column; per block the code offset (bytes) and the
+    // scale offset (floats from the start of the scales)
+    int64_t *bj_off8 = nullptr, *bj_soff = nullptr;
+    int64_t bj_codes8 = 0, bj_bytes8 = 0;
     bool bj_ready = false;
     int bj_method = -1;          // -1 automatic, 0 LDS Gauss-Jordan, 1 register sweeps
     // blocks: (cell, first angle, angle count) triples; a cell of more than

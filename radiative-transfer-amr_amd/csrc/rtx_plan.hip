@@ -506,6 +506,8 @@ extern "C" void rtx_plan_destroy(rtx_plan *p) {
     if (p->coef) (void)hipFree(p->coef);
     if (p->bj_off) (void)hipFree(p->bj_off);
     if (p->bj_off16) (void)hipFree(p->bj_off16);
+    if (p->bj_off8) (void)hipFree(p->bj_off8);
+    if (p->bj_soff) (void)hipFree(p->bj_soff);
     if (p->bj_blk) (void)hipFree(p->bj_blk);
     if (p->coarse) (void)hipFree(p->coarse);
     if (p->bins) (void)hipFree(p->bins);

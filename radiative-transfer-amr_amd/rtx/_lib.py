@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(_HERE, "librtx.so")
 
 OP_MASS, OP_SCAT, OP_ICONV, OP_BCONV, OP_RESTRICT, OP_ACCUM, OP_QUADW = 1, 2, 4, 8, 16, 32, 64
 OP_SYSTEM = OP_MASS | OP_SCAT | OP_ICONV | OP_BCONV
-PREC_F64, PREC_F32, PREC_MIXED = 0, 1, 2
+PREC_F64, PREC_F32, PREC_MIXED, PREC_FP8 = 0, 1, 2, 3
 NMAX = 10
 
 _i32p = C.POINTER(C.c_int32)
@@ -68,6 +68,8 @@ SIGNATURES = [
                                      C.c_void_p]),
     ("rtx_bjacobi_set_method", C.c_int, [C.c_void_p, C.c_int]),
     ("rtx_bjacobi_size_bf16", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("rtx_bjacobi_size_fp8", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("rtx_bjacobi_pack_fp8", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rtx_bjacobi_pack_bf16", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rtx_bjacobi_apply", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p]),

@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, PREC_F64, PREC_F32, PREC_MIXED, OP_SYSTEM, OP_RESTRICT
+from ._lib import check, PREC_F64, PREC_F32, PREC_MIXED, PREC_FP8, OP_SYSTEM, OP_RESTRICT
 from .operator import stream_handle
 
 
@@ -36,14 +36,15 @@ def _vp(t):
 class BlockJacobi:
     """Dense inverses of every cell's interior diagonal block, on the device."""
 
-    def __init__(self, op, restrict=True, keep_f32=False, keep_f64=True, keep_bf16=False):
+    def __init__(self, op, restrict=True, keep_f32=False, keep_f64=True, keep_bf16=False,
+                 keep_fp8=False):
         self.op, self.lib = op, op.lib
         n = C.c_int64()
         check(self.lib.rtx_bjacobi_size(op.plan, C.byref(n)), "rtx_bjacobi_size")
         self.entries = int(n.value)
         dev = op.device
         self.inv = torch.empty(self.entries, dtype=torch.float64, device=dev)
-        need32 = keep_f32 or keep_bf16       # the BF16 copy is rounded from the FP32 one
+        need32 = keep_f32 or keep_bf16 or keep_fp8   # BF16 / FP8 are rounded from FP32
         self.inv32 = torch.empty(self.entries, dtype=torch.float32, device=dev) if need32 else None
         st = stream_handle()
         rc = self.lib.rtx_bjacobi_factor(op.plan, 1 if restrict else 0, _vp(self.inv),
@@ -65,6 +66,17 @@ class BlockJacobi:
             self.inv16 = torch.empty(self.entries16, dtype=torch.int16, device=dev)
             check(self.lib.rtx_bjacobi_pack_bf16(op.plan, _vp(self.inv32), _vp(self.inv16), st),
                   "rtx_bjacobi_pack_bf16")
+        self.inv8 = None
+        self.bytes8 = 0
+        if keep_fp8:
+            n8 = C.c_int64()
+            check(self.lib.rtx_bjacobi_size_fp8(op.plan, C.byref(n8)), "rtx_bjacobi_size_fp8")
+            self.bytes8 = int(n8.value)
+            # e4m3 codes (16-byte columns) followed by one FP32 scale per column
+            self.inv8 = torch.empty(self.bytes8, dtype=torch.uint8, device=dev)
+            check(self.lib.rtx_bjacobi_pack_fp8(op.plan, _vp(self.inv32), _vp(self.inv8), st),
+                  "rtx_bjacobi_pack_fp8")
+        if keep_bf16 or keep_fp8:
             if not keep_f32:
                 torch.cuda.current_stream().synchronize()
                 self.inv32 = None
@@ -74,12 +86,16 @@ class BlockJacobi:
     def bytes_per_apply(self, precision=PREC_F64):
         if precision == PREC_MIXED:
             return self.entries16 * 2
+        if precision == PREC_FP8:
+            return self.bytes8
         return self.entries * {PREC_F64: 8, PREC_F32: 4}[precision]
 
     def apply(self, r, z, precision=PREC_F64, stream=None):
         """precision PREC_F64: FP64 inverses and vectors; PREC_F32: FP32;
-        PREC_MIXED: BF16 inverses, FP32 vectors."""
-        inv = {PREC_F64: self.inv, PREC_F32: self.inv32, PREC_MIXED: self.inv16}[precision]
+        PREC_MIXED: BF16 inverses, FP32 vectors; PREC_FP8: FP8 (e4m3)
+        inverses with per-column scales, FP32 vectors."""
+        inv = {PREC_F64: self.inv, PREC_F32: self.inv32, PREC_MIXED: self.inv16,
+               PREC_FP8: self.inv8}[precision]
         if inv is None:
             raise ValueError("block-Jacobi inverses for this precision were not kept")
         check(self.lib.rtx_bjacobi_apply(self.op.plan, precision, _vp(inv), _vp(r), _vp(z),
