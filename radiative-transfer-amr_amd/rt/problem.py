@@ -10,6 +10,7 @@ from dg.projection import Projection, push_forward, pull_back
 from dg.quadrature import quad_xyth, lag_eval
 from rtx import _lib
 from rtx.operator import TransportOperator, OP_QUADW, OP_SYSTEM, PREC_F64, PREC_F32, PREC_MIXED
+from rtx._lib import PREC_FP8
 from rtx import krylov
 
 _PRECISIONS = {"fp64": PREC_F64, "fp32": PREC_F32, "mixed": PREC_MIXED}
@@ -88,10 +89,14 @@ class Problem:
         ("fgmres" accepted), pc_type="bjacobi" ("kaczmarz" and "none"
         accepted; "kaczmarz" falls back to block-Jacobi and is reported in
         convergence_info["pc_type_used"]), rtol=EPS, atol=EPS, max_it=5000,
-        restart=50, precision="fp64" | "mixed" (FP64 iterative refinement
-        around FP32/BF16 applies), inner_rtol=1e-2 (mixed only)."""
+        restart=50, precision="fp64" | "fp32" | "mixed" (FP64 iterative
+        refinement around reduced-precision applies: "mixed" runs FP32
+        convection, BF16 scattering and FP8 block-Jacobi inverses with
+        inner GMRES(20) unless ``restart`` is given; "fp32" runs FP32
+        throughout), inner_rtol=1e-2 (refinement only)."""
         kw = {"verbose": False, "ksp_type": "gmres", "pc_type": "bjacobi",
-              "rtol": consts.EPS, "atol": consts.EPS, "max_it": 5000, "restart": 50,
+              "rtol": consts.EPS, "atol": consts.EPS, "max_it": 5000,
+              "restart": 20 if kwargs.get("precision") == "mixed" else 50,
               "precision": "fp64", "inner_rtol": 1e-2, **kwargs}
         t_start = time.perf_counter()
         op = self.operator(mesh)
@@ -109,15 +114,19 @@ class Problem:
             raise ValueError("unsupported pc_type {!r}".format(pc_type))
         prec = _PRECISIONS[kw["precision"]]
         t_setup = time.perf_counter()
-        pc = krylov.BlockJacobi(op, keep_f32=prec != PREC_F64) if pc_used == "bjacobi" else None
+        pc = (krylov.BlockJacobi(op, keep_f64=prec == PREC_F64, keep_f32=prec == PREC_F32,
+                                 keep_fp8=prec == PREC_MIXED)
+              if pc_used == "bjacobi" else None)
         t_pc = time.perf_counter()
-        hi = krylov.InteriorSystem(op, PREC_F64, pc, PREC_F64)
+        # the refinement's FP64 system only forms residuals: no preconditioner
+        hi = krylov.InteriorSystem(op, PREC_F64, pc if prec == PREC_F64 else None, PREC_F64)
         if prec == PREC_F64:
             x, info = krylov.gmres(hi, rhs, x0, rtol=kw["rtol"], atol=kw["atol"],
                                    restart=kw["restart"], max_it=kw["max_it"],
                                    flexible=kw["ksp_type"] == "fgmres")
         else:
-            lo = krylov.InteriorSystem(op, prec, pc, PREC_F32 if pc is not None else prec)
+            pc_prec = {PREC_F32: PREC_F32, PREC_MIXED: PREC_FP8}[prec] if pc is not None else prec
+            lo = krylov.InteriorSystem(op, prec, pc, pc_prec)
             x, info = krylov.refine(hi, lo, rhs, x0, rtol=max(kw["rtol"], 1e-14),
                                     inner_rtol=kw["inner_rtol"], restart=kw["restart"],
                                     max_inner=kw["max_it"])

@@ -68,10 +68,14 @@ def test_problem_vectors_and_solve(cuda):
                 "res_best", "rtol", "atol", "divtol", "max_it"):
         assert key in conv
     assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-8
-    # mixed precision (FP64 refinement around FP32 applies) reaches the same solution
-    uh2, conv2, _ = prob.solve(mesh, rtol=1e-11, atol=0.0, precision="mixed")
-    assert conv2["converged_reason"] > 0
-    assert np.linalg.norm(uh2.to_vector() - u_ref) / np.linalg.norm(u_ref) < 1e-8
+    # FP64 refinement around reduced-precision inner solves ("mixed": FP32
+    # convection, BF16 scattering, FP8 block-Jacobi; "fp32": FP32 throughout)
+    # reaches the same solution
+    for precision in ("mixed", "fp32"):
+        uh2, conv2, _ = prob.solve(mesh, rtol=1e-11, atol=0.0, precision=precision)
+        assert conv2["converged_reason"] > 0, precision
+        assert conv2["precision"] == precision
+        assert np.linalg.norm(uh2.to_vector() - u_ref) / np.linalg.norm(u_ref) < 1e-8, precision
 
 
 @pytest.mark.gpu

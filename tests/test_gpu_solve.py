@@ -373,18 +373,53 @@ def test_c4_mixed_refinement_matches_fp64(cuda):
     assert float((xm - x64).norm() / x64.norm()) < 1e-6
 
 
-def _e4m3_round(a):
-    """Nearest FP8 e4m3 value (ties to even), |a| <= 448, as float64."""
+def _e4m3_values():
+    """Magnitudes of the finite OCP e4m3 codes 0 .. 0x7E (ascending, 0 .. 448):
+    exponent bits 6-3 with bias 7, mantissa bits 2-0, subnormals at exponent 0."""
     vals = []
     for c in range(127):
         e, m = c >> 3, c & 7
         vals.append(m / 512.0 if e == 0 else (1 + m / 8.0) * 2.0 ** (e - 7))
-    vals = np.array(vals)           # ascending, 0 .. 448
+    return np.array(vals)
+
+
+def _e4m3_code(a):
+    """Nearest e4m3 code (ties to the even code), |a| <= 448: sign in bit 7."""
+    vals = _e4m3_values()
     x = np.abs(np.asarray(a, dtype=np.float64))
     i = np.clip(np.searchsorted(vals, x), 1, len(vals) - 1)
     lo, hi = vals[i - 1], vals[i]
     pick_hi = (x - lo > hi - x) | ((x - lo == hi - x) & (i % 2 == 0))
-    return np.sign(a) * np.where(pick_hi, hi, lo)
+    mag = np.where(pick_hi, i, i - 1)
+    return (mag | np.where((np.asarray(a) < 0) & (mag > 0), 0x80, 0)).astype(np.uint8)
+
+
+def _e4m3_decode(codes):
+    codes = np.asarray(codes, dtype=np.int64)
+    mag = codes & 0x7F
+    assert (mag < 127).all(), "NaN code in the pool"
+    return np.where(codes & 0x80, -1.0, 1.0) * _e4m3_values()[mag]
+
+
+def _e4m3_round(a):
+    """Nearest FP8 e4m3 value (ties to even), |a| <= 448, as float64."""
+    return _e4m3_decode(_e4m3_code(a))
+
+
+def test_e4m3_helpers():
+    """The numpy e4m3 reference: a few code points of the OCP table, ties to
+    even, sign, and round trips of every finite code."""
+    vals = _e4m3_values()
+    assert vals[0x7E] == 448.0 and vals[0x08] == 2.0 ** -6 and vals[0x01] == 2.0 ** -9
+    assert vals[0x38] == 1.0 and vals[0x3F] == 1.875
+    assert (np.diff(vals) > 0).all()
+    codes = np.arange(256, dtype=np.uint8)
+    codes = codes[(codes & 0x7F) != 0x7F]
+    back = _e4m3_code(_e4m3_decode(codes))
+    assert ((back == codes) | ((codes & 0x7F) == 0)).all()       # -0 encodes as +0
+    # 1 + 1/16 lies halfway between 1 (code 0x38) and 1.125 (0x39): even wins
+    assert _e4m3_code(1.0625) == 0x38 and _e4m3_code(1.1875) == 0x3A
+    assert _e4m3_code(-3.0) == 0x80 | 0x44 and _e4m3_round(-3.0) == -3.0
 
 
 @pytest.mark.gpu
@@ -418,6 +453,43 @@ def test_block_jacobi_fp8_apply(cuda):
     # codes on a rounding boundary may differ by one step between the FP32
     # device scaling and the numpy FP64 one
     assert np.linalg.norm(z - z_ref) / np.linalg.norm(z_ref) < 5e-3
+    # the pool itself: codes of block b at off8[b] + j ldp (column j, rows
+    # padded to ldp = 16 ceil(n / 16) with code 0), then the FP32 scales
+    # after the codes rounded up to 16 bytes, at soff[b] + j
+    pool = pc.inv8.cpu().numpy()
+    blocks = osys.bj_block_dofs(mesh, 128)
+    ns = np.array([len(i) for i in blocks])
+    ldps = (ns + 15) // 16 * 16
+    off8 = np.concatenate([[0], np.cumsum(ns * ldps)])
+    soff = np.concatenate([[0], np.cumsum(ns)])
+    codes8 = (off8[-1] + 15) // 16 * 16
+    assert pool.size == codes8 + 4 * soff[-1]
+    scales = pool[codes8:].view(np.float32).astype(np.float64)
+    z_pool = np.zeros(op.ndof)
+    n_diff = n_all = 0
+    for b, (idx, inv) in enumerate(zip(blocks, invs)):
+        n, ldp = ns[b], ldps[b]
+        cd = pool[off8[b]:off8[b + 1]].reshape(n, ldp)           # [column][row]
+        assert not cd[:, n:].any(), "padding rows must hold code 0"
+        sc = scales[soff[b]:soff[b + 1]]
+        B = inv.astype(np.float32).astype(np.float64)
+        sc_ref = np.abs(B).max(axis=0) / 448.0
+        sc_ref = np.where(sc_ref > 0, sc_ref, 1.0)
+        assert np.allclose(sc, sc_ref, rtol=1e-6, atol=0)
+        c_dev = cd[:, :n].T.astype(np.int64)                       # [row][column]
+        c_ref = _e4m3_code(np.clip(B / sc[None, :], -448.0, 448.0)).astype(np.int64)
+        # signed code index: the codes ordered along the real line
+        signed = lambda c: np.where(c & 0x80, -1, 1) * (c & 0x7F)
+        step = np.abs(signed(c_dev) - signed(c_ref))
+        # a code may differ from the numpy one only by one step, where the
+        # device's FP32 inverse and quotient sit on a rounding boundary
+        assert (step <= 1).all()
+        n_diff += int((step != 0).sum())
+        n_all += step.size
+        z_pool[idx] = (_e4m3_decode(c_dev) * sc[None, :]) @ r[idx].astype(np.float64)
+    assert n_diff <= max(2, n_all // 1000), (n_diff, n_all)
+    # the apply against the device's own pool: FP32 accumulation only
+    assert np.linalg.norm(z - z_pool) / np.linalg.norm(z_pool) < 2e-6
     fd = op.apply(torch.from_numpy(op.nodal(wl.f)).cuda(), ops=OP_QUADW)
     bd = op.mask(torch.from_numpy(op.nodal(wl.bcs)).cuda(), keep_interior=False)
     rhs = op.mask(fd - op.apply(bd, ops=OP_SYSTEM), keep_interior=True)
