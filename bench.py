@@ -94,33 +94,65 @@ def measured_traffic():
     return int(m.group(1)) if m else None
 
 
-def cpu_baseline(mesh, wl, target_s=12.0):
-    """Oracle assembled CSR SpMV (scipy, one thread) on the rows of a sample
-    of columns; DOF/s = rows / time per SpMV."""
+def _cpu_threads():
+    """Host threads for the multi-core baseline: the cores this process may
+    run on, capped by OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
+
+
+def _time_reps(fn, budget_s, max_reps=2000):
+    fn()
+    reps, t = 0, 0.0
+    t0 = time.perf_counter()
+    while t < budget_s and reps < max_reps:
+        fn()
+        reps += 1
+        t = time.perf_counter() - t0
+    return t / reps, reps
+
+
+def cpu_baseline(mesh, wl, ncols=128, budget_s=4.0):
+    """The oracle's assembled FP64 CSR A = (BC - IC) + M + S on the rows of a
+    sample of columns, multiplied by the oracle's C CSR product on every host
+    thread (the value) and by scipy on one thread; DOF/s = rows / time per
+    SpMV."""
     import numpy as np
-    from oracle import assemble as asm
+    from oracle import assemble as asm, build_c
     keys = sorted(mesh.cols)
-    sample = keys[:: max(1, len(keys) // 32)][:32]
+    sample = keys[:: max(1, len(keys) // ncols)][:ncols]
     t0 = time.perf_counter()
     V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi, cols=sample)
     A = asm.system_matrix(V)
     rows = np.unique(A.nonzero()[0])
     A = A[rows].tocsr()
+    A.indices = A.indices.astype(np.int32)
     t_asm = time.perf_counter() - t0
     x = np.random.default_rng(0).uniform(-1, 1, A.shape[1])
-    A @ x
-    reps, t = 0, 0.0
-    t0 = time.perf_counter()
-    while t < target_s * 0.25 and reps < 1000:
-        A @ x
-        reps += 1
-        t = time.perf_counter() - t0
-    per = t / reps
-    return {"value": rows.size / per, "unit": "DOF/s", "cores": 1, "kind": "port",
-            "sample": "rows of {} of {} columns ({} rows, {} nnz) of the oracle's assembled "
-                      "FP64 CSR A = (BC - IC) + M + S, scipy SpMV, {} reps; assembly of the "
-                      "sample took {:.1f} s".format(len(sample), len(keys), rows.size, A.nnz,
-                                                    reps, t_asm)}
+    per1, reps1 = _time_reps(lambda: A @ x, budget_s)
+    lib = build_c.load()
+    nt = _cpu_threads()
+    if not np.array_equal(build_c.csr_spmv(lib, A, x, nt), A @ x):
+        raise RuntimeError("the C CSR product differs from scipy's")
+    indptr = np.ascontiguousarray(A.indptr, dtype=np.int64)
+    y = np.empty(A.shape[0])
+    per, reps = _time_reps(lambda: lib.oracle_csr_spmv(
+        A.shape[0], indptr.ctypes.data, A.indices.ctypes.data, A.data.ctypes.data,
+        x.ctypes.data, y.ctypes.data, nt), budget_s)
+    return {"value": rows.size / per, "unit": "DOF/s", "cores": nt, "kind": "port",
+            "sample": "rows of {} of {} columns ({} rows, {} nnz, {:.0f} MB CSR) of the "
+                      "oracle's assembled FP64 CSR A = (BC - IC) + M + S; the oracle's C CSR "
+                      "product (OpenMP, {} threads, {} reps, bit-identical to scipy); "
+                      "assembly of the sample took {:.1f} s".format(
+                          len(sample), len(keys), rows.size, A.nnz,
+                          (A.data.nbytes + A.indices.nbytes + indptr.nbytes) / 1e6, nt, reps,
+                          t_asm),
+            "scipy_1thread": {"value": rows.size / per1, "unit": "DOF/s", "cores": 1,
+                              "reps": reps1}}
 
 
 def main():
