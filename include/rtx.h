@@ -307,6 +307,55 @@ int rtx_error_l2(rtx_plan *plan, const double *d_uh, const double *d_u_hr,
                  const int64_t *d_hr_off, const double *d_tab, double *d_cell_err,
                  void *stream);
 
+/* The same error against the nodal solution of a p-refined mesh with the
+   same column and cell keys (error_high_resolution.py:103-203): cell k is
+   interpolated to d_hr_dims[3k + 0..2] = (mx, my, mth) nodes, each at most
+   RTX_NMAX; its high-resolution values start at d_u_hr + d_hr_off[k].
+   d_tab: (RTX_NMAX + 1)(RTX_NMAX + 4) * RTX_ERR_TAB_STRIDE doubles; the
+   block of (n, m) at (n (RTX_NMAX + 4) + m) * RTX_ERR_TAB_STRIDE has the
+   layout of rtx_error_l2's block of n with m nodes in place of n + 3
+   (:122-131, 143-155).  Also d_hr_norm2[k] = (dx dy dth / 8) sum w w w
+   u_hr^2, the cell's share of the normalisation (:66-97). */
+int rtx_error_l2_nodal(rtx_plan *plan, const double *d_uh, const double *d_u_hr,
+                       const int64_t *d_hr_off, const int32_t *d_hr_dims, const double *d_tab,
+                       double *d_cell_err, double *d_hr_norm2, void *stream);
+
+/* ---- error indicators without an analytic solution
+   (src/amr/Error_Indicator/error/error_angular_jump.py,
+   error_spatial_jump.py) and the projection integrals hp steering reads
+   (src/dg/projection/cell_intg_xy.py, col_intg_th.py). */
+
+/* d_intg_xy[col_ang_start + cell_ang_off + a] = (dx dy / 4) sum_ij w_i w_j
+   u[i, j, a] for every cell (cell_intg_xy.py:13-36);
+   d_intg_th[col_sp_off + i ny + j] = sum over the column's cells (dth / 2)
+   sum_a w_a u[i, j, a] (col_intg_th.py:13-37); d_norm2 (may be NULL):
+   (dx dy / 4)(dth / 2) sum w w w u^2 per cell. */
+int rtx_indicator_moments(rtx_plan *plan, const double *d_u, double *d_intg_xy,
+                          double *d_intg_th, double *d_norm2, void *stream);
+
+/* Angular-jump error per cell (error_angular_jump.py:26-81):
+   d_cell_err[k] = (1/4) sum_ij w_i w_j (1/2)((bot_k - top_lo)^2 +
+   (top_k - bot_up)^2), the traces at theta pulled back to -1 (bot) and +1
+   (top); d_nbr[2k], d_nbr[2k + 1]: global cell indices of the lower and
+   upper angular neighbours (Cell.nhbr_keys); d_lend[(n 2 + s) RTX_NMAX + a]
+   = l_a(2 s - 1) of the n-point Gauss-Legendre basis.  The caller takes the
+   square roots and column sums (:69-79). */
+int rtx_error_angular_jump(rtx_plan *plan, const double *d_u, const int32_t *d_nbr,
+                           const double *d_lend, double *d_cell_err, void *stream);
+
+/* Spatial-jump error per column (error_spatial_jump.py:30-227).  The pairs
+   (own cell, neighbour cell) of column c across each face are
+   d_pair[RTX_JUMP_PAIR_INTS p ..] for p in d_col_pair_start[c] ..
+   d_col_pair_start[c + 1] - 1: own cell, neighbour cell, face F, nB, nH,
+   and d_pool offsets of the row-major projections psi_0 (n_0 x nB), psi_1
+   (n_1 x nB), xsi_0 (nth_0 x nH), xsi_1 (nth_1 x nH) (:84-207);
+   d_scale[p] = d_A dth_G / 4 (:212, 222).  d_col_err[c] = sum_p d_scale[p]
+   sum wB wH (U_0 - U_1)^2 / ((2 dx + 2 dy) 2 pi) (:227). */
+#define RTX_JUMP_PAIR_INTS 9
+int rtx_error_spatial_jump(rtx_plan *plan, const double *d_u, const int32_t *d_col_pair_start,
+                           const int32_t *d_pair, const double *d_scale, const double *d_pool,
+                           double *d_col_err, void *stream);
+
 /* ---- learned coarse scalar-flux correction.  No counterpart in the
    reference: its preconditioners are PETSc bjacobi / kaczmarz
    (src/rt/solve.py:86-95).  A coarse grid of n_coarse cells covers the

@@ -91,6 +91,10 @@ SIGNATURES = [
     ("rtx_convert", C.c_int, [C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rtx_error_l2", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                C.c_void_p, C.c_void_p]),
+    ("rtx_error_l2_nodal", C.c_int, [C.c_void_p] * 9),
+    ("rtx_indicator_moments", C.c_int, [C.c_void_p] * 6),
+    ("rtx_error_angular_jump", C.c_int, [C.c_void_p] * 6),
+    ("rtx_error_spatial_jump", C.c_int, [C.c_void_p] * 8),
     ("rtx_gmres_state_size", C.c_int, [C.c_int, C.POINTER(C.c_int64)]),
     ("rtx_gmres_begin", C.c_int, [C.c_int, C.c_double, C.c_void_p, C.c_void_p]),
     ("rtx_gmres_givens", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -67,9 +67,25 @@ def test_error_indicator_matches_oracle_and_flags(cuda, tmp_path):
     ind.to_file(str(f), write_projection=False, write_mesh=False)
     d = json.loads(f.read_text())
     assert abs(d["error"] - ind.error) < 1e-15 and len(d["cols"]) == len(ind.cols)
-    # hp steering is not built: a flagged refinement with ref_form "hp" raises
-    with pytest.raises(NotImplementedError):
-        Error_Indicator(proj, ref_tol=[0.5, 0.5]).error_analytic(_u, intg=intg)
-    # default tolerances flag nothing, so "hp" needs no steering
+    # ref_form "hp": every flagged column and cell is steered as the oracle's
+    # steering rules decide on the oracle's integrals
+    from oracle import indicator as oind
+    ind = Error_Indicator(proj, ref_tol=[0.5, 0.5]).error_analytic(_u, intg=intg)
+    u = proj.to_vector()
+    steered = 0
+    for ck, col in ind.cols.items():
+        if col.do_ref:
+            nx, ny = mesh.cols[ck].ndofs
+            _, _, lp = oind.col_steer_lp(nx, ny, oind.col_intg_th(mesh, u, ck))
+            assert col.ref_form == ("p" if lp - 0.5 >= 0.5 * (nx + ny) else "h")
+            steered += 1
+        for kk, cell in col.cells.items():
+            if cell.do_ref:
+                nth = mesh.cols[ck].cells[kk].ndofs[0]
+                _, lp = oind.cell_steer_lp(nth, oind.cell_intg_xy(mesh, u, ck, kk))
+                assert cell.ref_form == ("p" if lp - 0.5 >= nth else "h")
+                steered += 1
+    assert steered > 0
+    # default tolerances flag nothing
     ind = Error_Indicator(proj).error_analytic(_u, intg=intg)
     assert not any(c.do_ref for c in ind.cols.values())
