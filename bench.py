@@ -14,12 +14,13 @@ relative residual of 1e-8 with (a) plain FP64 GMRES + block-Jacobi and
 (b) FP64 iterative refinement around reduced-precision applies; trains the
 learned coarse preconditioner on solver-generated pairs of the random-field
 workload (64 x 64 x 16 cells, p = 4, 8,192,000 DOF) and times plain FP64,
-mixed and learned+mixed solves of an unseen field there; and times the
-oracle's assembled CSR SpMV on a bounded sample on the host CPU.
+mixed and learned+mixed solves of an unseen field there; times the error
+indicator kernels on the same mesh; and times the oracle's assembled CSR
+SpMV on a bounded sample on the host CPU.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve]
-                       [--no-cpu] [--no-learned] [--learned-fields N]
-                       [--learned-kappa-scale S] [--calib]
+                       [--no-cpu] [--no-learned] [--no-indicators]
+                       [--learned-fields N] [--learned-kappa-scales S,...] [--calib]
 """
 import argparse
 import json
@@ -43,6 +44,7 @@ def parse():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-indicators", action="store_true")
     ap.add_argument("--learned-fields", type=int, default=17)
     ap.add_argument("--learned-kappa-scales", default="1,30",
                     help="comma-separated kappa scales of the random-field workload "
@@ -224,6 +226,8 @@ def main():
     }
     if not args.no_solve:
         out["solve"] = solve_section(op, mesh, wl)
+    if not args.no_indicators:
+        out["indicators"] = indicators_section(op, mesh, wl, cpu=not args.no_cpu)
     if not args.no_learned:
         del op
         torch.cuda.empty_cache()
@@ -233,6 +237,91 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, wl)
     print(json.dumps(out), flush=True)
+
+
+def indicators_section(op, mesh, wl, reps=20, cpu=True):
+    """The error-indicator kernels on the same mesh, on a random nodal field:
+    average device time per call (HIP events on the launch stream) and the
+    algorithmic bytes each reads; beside them, the oracle's loops on a
+    4 x 4-column mesh of the same order (per DOF)."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from rtx.error import ErrorEvaluator
+    from rtx.operator import stream_handle
+    t0 = time.perf_counter()
+    ev = ErrorEvaluator(mesh, device=op.device, tables=op.t)
+    t = ev.t
+    u = torch.rand(t.ndof, dtype=torch.float64, device=op.device)
+    start, pair, scale, pool = ev.jump_pairs()
+    ev.spatial_jump(u)                          # uploads the pair tables
+    ev.angular_jump(u)
+    t_setup = time.perf_counter() - t0
+    vp = lambda x: C.c_void_p(x.data_ptr())
+    f64 = lambda n: torch.empty(max(int(n), 1), dtype=torch.float64, device=op.device)
+    xy, th, n2, ce, cc = f64(t.nang), f64(t.nsp), f64(t.ncell), f64(t.ncell), f64(t.ncol)
+    hr = torch.rand(int(ev.hr_off[-1]), dtype=torch.float64, device=op.device)
+    ds, dp, dsc, dpl = ev._d_jump
+    st = stream_handle()
+    calls = {
+        "moments": lambda: ev.lib.rtx_indicator_moments(ev.plan, vp(u), vp(xy), vp(th), vp(n2), st),
+        "angular_jump": lambda: ev.lib.rtx_error_angular_jump(ev.plan, vp(u), vp(ev._ang_nbr),
+                                                              vp(ev._d_lend), vp(ce), st),
+        "spatial_jump": lambda: ev.lib.rtx_error_spatial_jump(ev.plan, vp(u), vp(ds), vp(dp),
+                                                              vp(dsc), vp(dpl), vp(cc), st),
+        "analytic_l2": lambda: ev.lib.rtx_error_l2(ev.plan, vp(u), vp(hr), vp(ev.d_hr_off),
+                                                   vp(ev.d_tab), vp(ce), st),
+    }
+    P = pair.reshape(-1, 9)
+    c0, c1 = t.cell_col[P[:, 0]], t.cell_col[P[:, 1]]
+    n0 = np.where(P[:, 2] % 2 == 0, t.col_ny[c0], t.col_nx[c0])
+    n1 = np.where(P[:, 2] % 2 == 0, t.col_ny[c1], t.col_nx[c1])
+    trace_vals = int(np.sum(n0 * t.cell_nth[P[:, 0]] + n1 * t.cell_nth[P[:, 1]]))
+    alg = {"moments": 16 * t.ndof, "angular_jump": 8 * t.ndof,
+           "spatial_jump": 8 * trace_vals, "analytic_l2": 8 * (t.ndof + int(ev.hr_off[-1]))}
+    stream = torch.cuda.current_stream()
+    res = {"workload": "C2 mesh, random nodal field", "setup_seconds": t_setup,
+           "spatial_jump_pairs": int(P.shape[0])}
+    for name, fn in calls.items():
+        for _ in range(3):
+            _lib_check(fn(), name)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / reps * 1e3
+        res[name] = {"us": us, "algorithmic_bytes": alg[name],
+                     "gbs": alg[name] / (us * 1e-6) / 1e9}
+    if cpu:
+        res["cpu_oracle"] = _indicator_cpu_sample()
+    return res
+
+
+def _lib_check(rc, what):
+    if rc != 0:
+        raise RuntimeError("{} failed ({})".format(what, rc))
+
+
+def _indicator_cpu_sample():
+    """The oracle's loops (angular and spatial jumps) on a 4 x 4-column,
+    16-cell, p = 4 mesh: seconds per DOF, one thread."""
+    import numpy as np
+    from oracle import indicator as oind
+    from rtx import problems
+    m = problems.uniform_mesh([1.0, 1.0], [5, 5, 5], nref_ang=4, nref_spt=2)
+    n = m.get_ndof()
+    u = np.random.default_rng(0).uniform(-1, 1, n)
+    out = {"sample": "4 x 4 columns x 16 cells, p = 4 ({} DOF), oracle/indicator.py, one thread".format(n),
+           "cores": 1}
+    for name, fn in (("angular_jump", oind.angular_jump), ("spatial_jump", oind.spatial_jump)):
+        t0 = time.perf_counter()
+        fn(m, u)
+        dt = time.perf_counter() - t0
+        out[name] = {"seconds": dt, "dof_per_s": n / dt}
+    return out
 
 
 def _rhs(op, wl):
