@@ -200,9 +200,17 @@ def main():
     # the binding resource: the larger of bytes / HBM peak and flops / FP64 peak
     fp64_bound = flops / (FP64_PEAK_TFLOPS * 1e12) > bytes_ / (HBM_PEAK_GBS * 1e9)
     if args.calib:
+        # two copies of 1 GiB for the traffic counters: the runtime's blit,
+        # then rtx_copy_f64 (plain per-lane FP64 loads and stores, the access
+        # type of the operator kernels)
+        import ctypes as C
+        from rtx import _lib
+        from rtx.operator import stream_handle
         big = torch.empty(1 << 27, dtype=torch.float64, device=dev)   # 1 GiB
         big2 = torch.empty_like(big)
         big2.copy_(big)
+        _lib.check(op.lib.rtx_copy_f64(C.c_void_p(big.data_ptr()), C.c_void_p(big2.data_ptr()),
+                                       big.numel(), stream_handle()), "rtx_copy_f64")
         torch.cuda.synchronize()
 
     out = {

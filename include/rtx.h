@@ -356,6 +356,11 @@ int rtx_error_spatial_jump(rtx_plan *plan, const double *d_u, const int32_t *d_c
                            const int32_t *d_pair, const double *d_scale, const double *d_pool,
                            double *d_col_err, void *stream);
 
+/* ---- measurement.  No counterpart in the reference: d_dst[i] = d_src[i]
+   for i < n, one double per lane per iteration with ordinary vector loads
+   and stores (a copy of known size for calibrating HBM traffic counters). */
+int rtx_copy_f64(const double *d_src, double *d_dst, int64_t n, void *stream);
+
 /* ---- learned coarse scalar-flux correction.  No counterpart in the
    reference: its preconditioners are PETSc bjacobi / kaczmarz
    (src/rt/solve.py:86-95).  A coarse grid of n_coarse cells covers the

@@ -290,10 +290,21 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
             ok = d->col_cell_start[c] == c * NC &&
                  int64_t(d->cell_dof_off[c * NC]) == int64_t(c) * N * N * NTH * NC &&
                  d->col_sp_off[c] == c * N * N && d->col_ang_start[c] == c * NTH * NC;
-        // every link must couple equal shapes (trace of N x NTH values)
+        // every link must couple equal shapes (trace of N x NTH values), and a
+        // cell may have at most one link on an x-face (F even) and one on a
+        // y-face (F odd): the kernel adds a corner node's two face fluxes in
+        // one thread, without atomics
         for (int l = 0; l < nl && ok; ++l) {
             const int s = d->link_src[l];
             ok = d->cell_nth[s] == NTH && d->col_nx[d->cell_col[s]] == N;
+        }
+        for (int k = 0; k < nk && ok; ++k) {
+            int par = 0;
+            for (int l = d->link_start[k]; l < d->link_start[k + 1] && ok; ++l) {
+                const int bit = 1 << (d->link_face[l] & 1);
+                ok = (par & bit) == 0;
+                par |= bit;
+            }
         }
         p->uni_ok = ok && uniform_shape_compiled(N, NTH, NC);
         p->uni_n = N;

@@ -7,7 +7,7 @@
 // computation as rtx_fused.hip (same reference formulas), specialised:
 // no searches, constant divisions, batched register prefetch of the column,
 // the part's neighbour traces and the MFMA B operands, and XCD-aware
-// (column, part) blocks.
+// (column, part) blocks: each XCD owns a contiguous run of columns.
 #include <type_traits>
 
 #include "rtx_internal.h"
@@ -22,7 +22,6 @@ typedef short u_sht4 __attribute__((ext_vector_type(4)));
 
 constexpr int UNM = RTX_NMAX;
 constexpr int UXCD = 8;
-constexpr int UTHREADS = 256;
 
 __device__ inline bool u_on_bdry(int bd, int i, int j, int n) {
     return ((bd & 1) && i == n - 1) || ((bd & 2) && j == n - 1) || ((bd & 4) && i == 0) ||
@@ -35,10 +34,18 @@ __device__ inline uint16_t u_bf16(float f) {
     return uint16_t(u >> 16);
 }
 
-template <typename T, typename A, int PREC, int N, int NTH, int NC, int H>
+// Threads per workgroup: one wavefront per 16-wide output-angle tile of the
+// part, at least four.
+template <int NTH, int NC, int H>
+constexpr int u_threads() {
+    return ((NC / H) * NTH + 15) / 16 > 4 ? 512 : 256;
+}
+
+template <typename T, typename A, int PREC, int N, int NTH, int NC, int H,
+          int UTHREADS = u_threads<NTH, NC, H>()>
 __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops, int accumulate,
-                                                     const T *__restrict__ x,
-                                                     T *__restrict__ y) {
+                                                      const T *__restrict__ x,
+                                                      T *__restrict__ y, int nlink_lds) {
     constexpr int NSP = N * N, NA = NC * NTH, NCD = NSP * NA, CD = NSP * NTH;
     constexpr int OC = NC / H;                 // cells per part
     constexpr int NOWN = OC * CD, NOA = OC * NTH;
@@ -58,17 +65,25 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     extern __shared__ __align__(16) unsigned char u_lds[];
     A *xs = reinterpret_cast<A *>(u_lds);      // [NSP][NAP]: column x, angle A of node s
     A *ys = xs + NSP * NAP;                    // [NSP][NOAP]: part outputs
-    A *tr = ys + NSP * NOAP;                   // MAXL * TRL
+    A *tr = ys + NSP * NOAP;                   // links * TRL: neighbour traces
+    A *fl = tr + nlink_lds * TRL;              // links * TRL: face fluxes (phase 3)
     __shared__ A cth[NA], sth[NA], cw[NA];
     __shared__ A kap[NSP], sgw[NSP];
     __shared__ A wx_[N], WD[N * N];
     __shared__ A dth_[NC];
     __shared__ int cbd[NC], cqd[NC];
     __shared__ int l_src[MAXL], l_meta[MAXL], l_es[MAXL], l_et[MAXL];
+    __shared__ int cfx[OC], cfy[OC];           // per part cell: 4 link + F of its x- / y-face
+                                               // link, -1 without one
 
+    // workgroups go to the XCDs round-robin (b mod 8); XCD j takes the
+    // contiguous columns [j cpx, (j + 1) cpx) in order, all parts of a column
+    // together, so a column's neighbours in index order (its y-neighbours,
+    // and its x-neighbours inside the run) are read through the same L2
     const int b = blockIdx.x;
-    const int grp = b / (UXCD * H), rem = b % (UXCD * H);
-    const int h = rem / UXCD, c = grp * UXCD + rem % UXCD;
+    const int cpx = (m.ncol + UXCD - 1) / UXCD;
+    const int slot = b / UXCD;
+    const int h = slot % H, c = (b % UXCD) * cpx + slot / H;
     if (c >= m.ncol) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -93,16 +108,22 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     const int Rl = wave * 16 + l16;            // part-local output angle of this lane
     const bool rok = (wave < NTILE) && (Rl < NOA);
     const int R = kk_lo * NTH + Rl;
-    if constexpr (PREC != RTX_PREC_MIXED) {
-        const EB *Gp = (PREC == RTX_PREC_F64) ? (const EB *)(m.G + m.col_G_off[c])
-                                              : (const EB *)(m.G_f + m.col_G_off[c]);
-        const bool scat = (ops & RTX_OP_SCAT) != 0;
+    // FP64 runs the scattering MFMA before the vector phases (phase 2); its
+    // B operands (G) are loaded here with the column in every precision
+    constexpr bool MFMA_FIRST = PREC == RTX_PREC_F64;
+    auto load_b = [&]() {
+        if constexpr (PREC != RTX_PREC_MIXED) {
+            const EB *Gp = (PREC == RTX_PREC_F64) ? (const EB *)(m.G + m.col_G_off[c])
+                                                  : (const EB *)(m.G_f + m.col_G_off[c]);
+            const bool scat = (ops & RTX_OP_SCAT) != 0;
 #pragma unroll
-        for (int u = 0; u < NKS; ++u) {
-            const int Ak = u * 4 + lg;
-            bpre[u] = (scat && rok && Ak < NA) ? Gp[size_t(Ak) * NA + R] : EB(0);
+            for (int u = 0; u < NKS; ++u) {
+                const int Ak = u * 4 + lg;
+                bpre[u] = (scat && rok && Ak < NA) ? Gp[size_t(Ak) * NA + R] : EB(0);
+            }
         }
-    }
+    };
+    load_b();
     A xv[XU];
 #pragma unroll
     for (int u = 0; u < XU; ++u) {
@@ -159,7 +180,16 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
         l_es[tid] = r_es;
         l_et[tid] = r_et;
     }
+    if (tid < OC) {
+        cfx[tid] = -1;
+        cfy[tid] = -1;
+    }
     __syncthreads();
+    if (bconv && tid < nl) {
+        const int kl = (r_meta >> 18) - kk_lo, F = r_meta & 3;
+        if (F & 1) cfy[kl] = 4 * tid + F;
+        else cfx[kl] = 4 * tid + F;
+    }
 
     // ---- phase 1: column input (masked) and the part's neighbour traces
 #pragma unroll
@@ -205,6 +235,73 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
         }
     }
     __syncthreads();
+
+    // ---- scattering for the part's output angles on MFMA, into registers,
+    // subtracted from the outputs in phase 4.  In FP64 it is issued here,
+    // before phases 2 and 3, so the matrix pipe runs while the cell-local and
+    // face terms go through the vector units (measured faster); in FP32 and
+    // BF16, whose MFMA work is short and whose operand conversion competes
+    // with phase 2, it runs after phase 3 (measured faster).
+    typedef typename std::conditional<PREC == RTX_PREC_F64, u_dbl4, u_flt4>::type acc4;
+    constexpr bool EARLY = MFMA_FIRST;
+    acc4 sacc[MT];
+    const bool scat_w = (ops & RTX_OP_SCAT) && wave < NTILE;
+#pragma unroll
+    for (int t = 0; t < MT; ++t) sacc[t] = acc4{0, 0, 0, 0};
+    auto scatter = [&]() {
+        if constexpr (PREC == RTX_PREC_F64 || PREC == RTX_PREC_F32) {
+            // two accumulator chains per tile (even / odd k-steps) halve the
+            // dependent MFMA chain
+            constexpr bool TWO = !EARLY;
+            acc4 acc2[MT];
+#pragma unroll
+            for (int t = 0; t < MT; ++t) acc2[t] = acc4{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks) {
+                const int Ak = ks * 4 + lg;
+#pragma unroll
+                for (int t = 0; t < MT; ++t) {
+                    const int s = t * 16 + l16;
+                    const EB a = (Ak < NA && s < NSP) ? EB(xs[s * NAP + Ak]) : EB(0);
+                    acc4 &dst = (TWO && (ks & 1)) ? acc2[t] : sacc[t];
+                    if constexpr (PREC == RTX_PREC_F64)
+                        dst = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bpre[ks], dst, 0, 0, 0);
+                    else
+                        dst = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bpre[ks], dst, 0, 0, 0);
+                }
+            }
+            if constexpr (TWO) {
+#pragma unroll
+                for (int t = 0; t < MT; ++t) sacc[t] += acc2[t];
+            }
+        } else {   // MIXED: BF16 operands, FP32 accumulation
+            const uint16_t *G = m.G_bf16 + m.col_G_off[c];
+            u_sht4 bv[NKB];
+#pragma unroll
+            for (int ks = 0; ks < NKB; ++ks)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int Ak = ks * 16 + 4 * lg + u;
+                    bv[ks][u] = (rok && Ak < NA) ? short(G[size_t(Ak) * NA + R]) : short(0);
+                }
+#pragma unroll
+            for (int ks = 0; ks < NKB; ++ks) {
+#pragma unroll
+                for (int t = 0; t < MT; ++t) {
+                    const int s = t * 16 + l16;
+                    u_sht4 av;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int Ak = ks * 16 + 4 * lg + u;
+                        av[u] = (Ak < NA && s < NSP) ? short(u_bf16(float(xs[s * NAP + Ak])))
+                                                     : short(0);
+                    }
+                    sacc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[ks], sacc[t], 0, 0, 0);
+                }
+            }
+        }
+    };
+    if (EARLY && scat_w) scatter();
 
     // ---- phase 2: cell-local terms on the part's DOFs, by lines of nodes:
     // pass x: one (cell, q, angle) line per thread reads x[0..N-1][q] once
@@ -271,7 +368,11 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     }
     __syncthreads();
 
-    // ---- phase 3: upwind inflow couplings (one (link, face node) per thread)
+    // ---- phase 3: upwind inflow couplings, one (link, face node, angle) per
+    // thread, into a flux slot of its own; phase 5 adds a DOF's slots (its
+    // cell has at most one x-face and one y-face link, the plan admits no
+    // other mesh here) in a fixed order: bit-reproducible without atomics.
+    // Phase 4 does not read the slots, so no barrier separates them.
     if (bconv) {
         for (int e = tid; e < nl * N * NTH; e += UTHREADS) {
             const int li = e / (N * NTH), le = e % (N * NTH);
@@ -294,92 +395,19 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
                 for (int a = 0; a < NTH; ++a) inner += et[a] * trl[m1 * NTH + a];
                 acc += es[m1] * inner;
             }
-            int sn;
-            if (F == 0) sn = (N - 1) * N + m0;
-            else if (F == 2) sn = m0;
-            else if (F == 1) sn = m0 * N + N - 1;
-            else sn = m0 * N;
-            atomicAdd(&ys[sn * NOAP + (kk - kk_lo) * NTH + r],
-                      A(((F & 1) ? dx : dy) * 0.25) * dth_[kk] * acc);
+            fl[e] = A(((F & 1) ? dx : dy) * 0.25) * dth_[kk] * acc;
         }
     }
-    __syncthreads();
 
-    // ---- phase 4: scattering for the part's output angles (MFMA)
-    if ((ops & RTX_OP_SCAT) && wave < NTILE) {
-        if constexpr (PREC == RTX_PREC_F64 || PREC == RTX_PREC_F32) {
-            typedef typename std::conditional<PREC == RTX_PREC_F64, u_dbl4, u_flt4>::type acc4;
-            // two accumulator chains per tile (even / odd k-steps) halve the
-            // dependent MFMA chain
-            acc4 acc[MT], acc2[MT];
+    // ---- phase 4: the scattering, subtracted from the outputs
+    if (!EARLY && scat_w) scatter();
+    if (scat_w && rok) {
 #pragma unroll
-            for (int t = 0; t < MT; ++t) {
-                acc[t] = acc4{0, 0, 0, 0};
-                acc2[t] = acc4{0, 0, 0, 0};
-            }
+        for (int t = 0; t < MT; ++t) {
 #pragma unroll
-            for (int ks = 0; ks < NKS; ++ks) {
-                const int Ak = ks * 4 + lg;
-#pragma unroll
-                for (int t = 0; t < MT; ++t) {
-                    const int s = t * 16 + l16;
-                    const EB a = (Ak < NA && s < NSP) ? EB(xs[s * NAP + Ak]) : EB(0);
-                    acc4 &dst = (ks & 1) ? acc2[t] : acc[t];
-                    if constexpr (PREC == RTX_PREC_F64)
-                        dst = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bpre[ks], dst, 0, 0, 0);
-                    else
-                        dst = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bpre[ks], dst, 0, 0, 0);
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < MT; ++t) acc[t] += acc2[t];
-            if (rok) {
-#pragma unroll
-                for (int t = 0; t < MT; ++t) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int s = t * 16 + ((PREC == RTX_PREC_F64) ? (lg + 4 * v) : (4 * lg + v));
-                        if (s < NSP) ys[s * NOAP + Rl] -= sgw[s] * A(acc[t][v]);
-                    }
-                }
-            }
-        } else {   // MIXED: BF16 operands, FP32 accumulation
-            const uint16_t *G = m.G_bf16 + m.col_G_off[c];
-            u_flt4 acc[MT];
-#pragma unroll
-            for (int t = 0; t < MT; ++t) acc[t] = u_flt4{0, 0, 0, 0};
-            u_sht4 bv[NKB];
-#pragma unroll
-            for (int ks = 0; ks < NKB; ++ks)
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int Ak = ks * 16 + 4 * lg + u;
-                    bv[ks][u] = (rok && Ak < NA) ? short(G[size_t(Ak) * NA + R]) : short(0);
-                }
-#pragma unroll
-            for (int ks = 0; ks < NKB; ++ks) {
-#pragma unroll
-                for (int t = 0; t < MT; ++t) {
-                    const int s = t * 16 + l16;
-                    u_sht4 av;
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int Ak = ks * 16 + 4 * lg + u;
-                        av[u] = (Ak < NA && s < NSP) ? short(u_bf16(float(xs[s * NAP + Ak])))
-                                                     : short(0);
-                    }
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[ks], acc[t], 0, 0, 0);
-                }
-            }
-            if (rok) {
-#pragma unroll
-                for (int t = 0; t < MT; ++t) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int s = t * 16 + 4 * lg + v;
-                        if (s < NSP) ys[s * NOAP + Rl] -= sgw[s] * A(acc[t][v]);
-                    }
-                }
+            for (int v = 0; v < 4; ++v) {
+                const int s = t * 16 + ((PREC == RTX_PREC_F64) ? (lg + 4 * v) : (4 * lg + v));
+                if (s < NSP) ys[s * NOAP + Rl] -= sgw[s] * A(sacc[t][v]);
             }
         }
     }
@@ -390,7 +418,15 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
         const int t = own0 + e;
         const int kk = t / CD, lt = t % CD, sn = lt / NTH, a = lt % NTH;
         const bool masked = restrict_ && cbd[kk] && u_on_bdry(cbd[kk], sn / N, sn % N, N);
-        const A v = masked ? A(0) : ys[sn * NOAP + (kk - kk_lo) * NTH + a];
+        A v = ys[sn * NOAP + (kk - kk_lo) * NTH + a];
+        if (bconv) {
+            // inflow face fluxes: x-face, then y-face
+            const int i = sn / N, j = sn % N;
+            const int fx = cfx[kk - kk_lo], fy = cfy[kk - kk_lo];
+            if (fx >= 0 && i == ((fx & 3) == 0 ? N - 1 : 0)) v += fl[((fx >> 2) * N + j) * NTH + a];
+            if (fy >= 0 && j == ((fy & 3) == 1 ? N - 1 : 0)) v += fl[((fy >> 2) * N + i) * NTH + a];
+        }
+        if (masked) v = A(0);
         if (accumulate) {
             if (!masked) y[cdof + t] = T(A(y[cdof + t]) + v);
         } else {
@@ -406,22 +442,25 @@ int launch_one(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t 
     constexpr int MAXL = 4 * (NC / H), TRL = N * NTH;
     RTX_CHECK(p->max_part_links[H] <= MAXL, "uniform apply: %d links in a column part (max %d)",
               p->max_part_links[H], MAXL);
-    // column and output rows, then trace slots for the largest link count of any part
-    const size_t lds = size_t(NSP * NAP + NSP * NOAP + p->max_part_links[H] * TRL) * sizeof(A);
+    // column and output rows, then trace and flux slots for the largest link
+    // count of any part
+    const size_t lds = size_t(NSP * NAP + NSP * NOAP + 2 * p->max_part_links[H] * TRL) * sizeof(A);
     auto kern = k_uniform<T, A, PREC, N, NTH, NC, H>;
     RTX_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 int(lds)));
-    const int nblocks = ((p->m.ncol + UXCD - 1) / UXCD) * UXCD * H;
-    kern<<<nblocks, UTHREADS, lds, st>>>(p->m, ops, acc, (const T *)dx, (T *)dy);
+    const int nblocks = ((p->m.ncol + UXCD - 1) / UXCD) * UXCD * H;   // 8 cpx H
+    kern<<<nblocks, u_threads<NTH, NC, H>(), lds, st>>>(p->m, ops, acc, (const T *)dx, (T *)dy,
+                                                        p->max_part_links[H]);
     RTX_HIP(hipGetLastError());
     return RTX_OK;
 }
 
 // A part count H is valid for a shape when it divides the cells and the
-// part's output angles fit one 16-wide tile per wavefront.
+// part's output angles fit one 16-wide tile per wavefront of the largest
+// workgroup.
 template <int NTH, int NC, int H>
 constexpr bool u_valid() {
-    return NC % H == 0 && ((NC / H) * NTH + 15) / 16 <= UTHREADS / 64;
+    return NC % H == 0 && ((NC / H) * NTH + 15) / 16 <= 512 / 64;
 }
 
 template <int N, int NTH, int NC, int H>
@@ -442,10 +481,10 @@ int launch_h(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipS
 template <int N, int NTH, int NC>
 int launch_shape(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
                  int acc) {
-    // default: two parts per column (LDS near 30 KB in FP64) when valid;
-    // rtx_autotune may pick another valid part count
+    // default: one part per column (x read once, no padded output tiles
+    // between parts) when valid; rtx_autotune may pick another valid count
     int H = p->uni_H;
-    if (H == 0) H = u_valid<NTH, NC, 2>() ? 2 : (u_valid<NTH, NC, 4>() ? 4 : 1);
+    if (H == 0) H = u_valid<NTH, NC, 1>() ? 1 : (u_valid<NTH, NC, 2>() ? 2 : 4);
     if (H == 1) return launch_h<N, NTH, NC, 1>(p, ops, prec, dx, dy, st, acc);
     if (H == 2) return launch_h<N, NTH, NC, 2>(p, ops, prec, dx, dy, st, acc);
     return launch_h<N, NTH, NC, 4>(p, ops, prec, dx, dy, st, acc);
@@ -480,7 +519,7 @@ int launch_uniform(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy
 int uniform_default_parts(const rtx_plan *p) {
     if (p->uni_H) return p->uni_H;
     const int mask = uniform_valid_parts(p);
-    return (mask & 2) ? 2 : ((mask & 4) ? 4 : 1);
+    return (mask & 1) ? 1 : ((mask & 2) ? 2 : 4);
 }
 
 int uniform_valid_parts(const rtx_plan *p) {

@@ -92,6 +92,7 @@ SIGNATURES = [
     ("rtx_error_l2", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                C.c_void_p, C.c_void_p]),
     ("rtx_error_l2_nodal", C.c_int, [C.c_void_p] * 9),
+    ("rtx_copy_f64", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     ("rtx_indicator_moments", C.c_int, [C.c_void_p] * 6),
     ("rtx_error_angular_jump", C.c_int, [C.c_void_p] * 6),
     ("rtx_error_spatial_jump", C.c_int, [C.c_void_p] * 8),
