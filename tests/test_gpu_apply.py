@@ -208,6 +208,43 @@ def test_apply_c3_rows_and_linearity(cuda):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n,nref_ang,nref_spt", [(5, 4, 3), (4, 2, 2)], ids=["p4", "p3"])
+def test_uniform_apply_bit_reproducible(cuda, n, nref_ang, nref_spt):
+    """The uniform kernel adds every output in a fixed order (face fluxes
+    staged, no atomics): repeated FP64 and mixed applies are bit-identical, and
+    so are the FP64 applies of every valid part count (the per-DOF arithmetic
+    does not depend on how a column is split)."""
+    import torch
+    from rtx.operator import TransportOperator, OP_SYSTEM, OP_RESTRICT, PREC_MIXED
+    from rtx import RtxError
+    mesh = problems.uniform_mesh([1.0, 1.0], [n, n, n], nref_ang, nref_spt)
+    wl = problems.absorber_c2()
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    assert op.plan_info()["uniform"]
+    x = _dev(np.random.default_rng(9).uniform(-1.0, 1.0, op.ndof))
+    ops = OP_SYSTEM | OP_RESTRICT
+    ref = None
+    ran = 0
+    for H in (1, 2, 4):
+        try:
+            op.set_uniform_parts(H)
+        except RtxError:
+            continue
+        ys = [op.apply(x, ops=ops).clone() for _ in range(3)]
+        for y in ys[1:]:
+            assert torch.equal(y, ys[0]), H
+        if ref is None:
+            ref = ys[0]
+        assert torch.equal(ys[0], ref), H
+        ym = [op.apply(x.float(), ops=ops, precision=PREC_MIXED).clone() for _ in range(3)]
+        for y in ym[1:]:
+            assert torch.equal(y, ym[0]), H
+        ran += 1
+    assert ran >= 2
+    op.set_uniform_parts(0)
+
+
+@pytest.mark.gpu
 def test_uniform_parts_and_autotune(cuda):
     """Every valid part count of the uniform kernel, and the shapes the
     autotuner picks for the uniform kernel and the fused order bins, give the
