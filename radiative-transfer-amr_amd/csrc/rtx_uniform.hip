@@ -108,8 +108,10 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     const int Rl = wave * 16 + l16;            // part-local output angle of this lane
     const bool rok = (wave < NTILE) && (Rl < NOA);
     const int R = kk_lo * NTH + Rl;
-    // FP64 runs the scattering MFMA before the vector phases (phase 2); its
-    // B operands (G) are loaded here with the column in every precision
+    // FP64 runs the scattering MFMA before the vector phases (phase 2) and
+    // loads its B operands (G) at the end of phase 1, so that they do not
+    // hold registers while the column is prefetched (72 VGPRs, 7 waves per
+    // SIMD); FP32 and BF16 load them here with the column
     constexpr bool MFMA_FIRST = PREC == RTX_PREC_F64;
     auto load_b = [&]() {
         if constexpr (PREC != RTX_PREC_MIXED) {
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
             }
         }
     };
-    load_b();
+    if constexpr (!MFMA_FIRST) load_b();
     A xv[XU];
 #pragma unroll
     for (int u = 0; u < XU; ++u) {
@@ -234,6 +236,7 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
             if (e < nl * TRL) tr[e] = zero[u] ? A(0) : v[u];
         }
     }
+    if constexpr (MFMA_FIRST) load_b();
     __syncthreads();
 
     // ---- scattering for the part's output angles on MFMA, into registers,
@@ -388,7 +391,9 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
             for (int a = 0; a < N; ++a) es[a] = A(Es[a * N + m0]);
             const A *trl = tr + li * TRL;
             A acc = A(0);
-#pragma unroll
+            // one trace row at a time: keeps the registers of this phase
+            // below those of the MFMA phase
+#pragma unroll 1
             for (int m1 = 0; m1 < N; ++m1) {
                 A inner = A(0);
 #pragma unroll
