@@ -37,7 +37,7 @@ __device__ inline double theta_f(int F, double c, double s) {
 
 template <typename T>
 __global__ __launch_bounds__(TT) void k_transport(DevMesh m, uint32_t ops, int accumulate,
-                                                  const int32_t *__restrict__ cells,
+                                                  const int32_t *__restrict__ cells, int ncells,
                                                   const T *__restrict__ x, T *__restrict__ y) {
     // x and y of the cell (dynamic: twice the plan's largest cell), the
     // small operands of the cell and of one upwind link
@@ -46,7 +46,14 @@ __global__ __launch_bounds__(TT) void k_transport(DevMesh m, uint32_t ops, int a
     __shared__ double wxs[NM], wys[NM], wts[NM], cth[NM], sth[NM];
     __shared__ double kap[NM * NM];
     __shared__ double tr[NM * NM], inn[NM * NM], Ets[NM * NM], Ess[NM * NM];
-    const int k = cells ? cells[blockIdx.x] : blockIdx.x;
+    // workgroups go to the 8 XCDs round-robin; XCD j takes the j-th
+    // contiguous eighth of the cell list (cells in column order), so the
+    // neighbour columns whose traces a cell reads are mostly handled, and
+    // cached, on the same XCD
+    const int per = (ncells + 7) / 8;
+    const int ci = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (ci >= ncells) return;
+    const int k = cells ? cells[ci] : ci;
     const int lane = threadIdx.x;
     const int c = m.cell_col[k];
     const int nx = m.col_nx[c], ny = m.col_ny[c], nth = m.cell_nth[k];
@@ -377,8 +384,8 @@ static int launch_apply(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hip
         return RTX_OK;
     }
     if (tp && ncells > 0)
-        k_transport<T><<<ncells, TT, size_t(2) * p->max_cell_ndof * sizeof(double), st>>>(
-            p->m, ops, acc ? 1 : 0, cells, x, y);
+        k_transport<T><<<8 * ((ncells + 7) / 8), TT, size_t(2) * p->max_cell_ndof * sizeof(double),
+                         st>>>(p->m, ops, acc ? 1 : 0, cells, ncells, x, y);
     if (sc && nitems > 0)
         k_scatter_mma<T><<<nitems, SCAT_THREADS, 0, st>>>(p->m, ops, (acc || tp) ? 1 : 0, items,
                                                           x, y);
