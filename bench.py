@@ -136,6 +136,8 @@ def cpu_baseline(mesh, wl, ncols=128, budget_s=4.0):
     t_asm = time.perf_counter() - t0
     x = np.random.default_rng(0).uniform(-1, 1, A.shape[1])
     per1, reps1 = _time_reps(lambda: A @ x, budget_s)
+    if not os.path.exists(build_c.OUT):
+        build_c.build()
     lib = build_c.load()
     nt = _cpu_threads()
     if not np.array_equal(build_c.csr_spmv(lib, A, x, nt), A @ x):
