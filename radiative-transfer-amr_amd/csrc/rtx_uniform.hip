@@ -118,10 +118,21 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
             const EB *Gp = (PREC == RTX_PREC_F64) ? (const EB *)(m.G + m.col_G_off[c])
                                                   : (const EB *)(m.G_f + m.col_G_off[c]);
             const bool scat = (ops & RTX_OP_SCAT) != 0;
+            if constexpr (NA % 4 == 0 && NOA % 16 == 0) {
+                // every lane of a scattering wave has an in-range operand and
+                // the other waves never read them: one wave-uniform branch,
+                // no per-lane masks or zero fills
+                if (scat && wave < NTILE) {
+                    const EB *Gl = Gp + size_t(lg) * NA + R;
 #pragma unroll
-            for (int u = 0; u < NKS; ++u) {
-                const int Ak = u * 4 + lg;
-                bpre[u] = (scat && rok && Ak < NA) ? Gp[size_t(Ak) * NA + R] : EB(0);
+                    for (int u = 0; u < NKS; ++u) bpre[u] = Gl[size_t(u) * 4 * NA];
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < NKS; ++u) {
+                    const int Ak = u * 4 + lg;
+                    bpre[u] = (scat && rok && Ak < NA) ? Gp[size_t(Ak) * NA + R] : EB(0);
+                }
             }
         }
     };
