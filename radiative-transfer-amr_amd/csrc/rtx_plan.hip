@@ -429,6 +429,11 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
             p->n_sc_all = int(sc_all.size() / 2);
             RTX_HIP(hipGetLastError());
         }
+        // every bin starts on the generic kernels (one wavefront per cell, a
+        // tiled scattering GEMM): no atomics, so results are bit-reproducible,
+        // and on large hp meshes they are faster; rtx_autotune moves a bin to
+        // the fused per-column kernel where that measures faster
+        for (int b = 0; b < RTX_NBINS; ++b) p->fbin[b].generic = true;
         const int rc = rebuild_generic(p);
         if (rc) return rc;
     }
