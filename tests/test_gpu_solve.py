@@ -109,7 +109,8 @@ def test_refinement_mixed_reaches_fp64_residual(cuda):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant", ["plain", "flexible", "mixed", "mixed_f64_basis"])
+@pytest.mark.parametrize("variant", ["plain", "flexible", "mixed", "mixed_f64_basis", "hp_plain",
+                                     "hp_mixed"])
 def test_graph_replay_matches_eager(cuda, variant):
     """Chunks of GMRES iterations replayed as HIP graphs reproduce the eager
     run exactly (same kernels, same order): iterate, residual history and
@@ -120,7 +121,15 @@ def test_graph_replay_matches_eager(cuda, variant):
     from rtx.operator import TransportOperator
     from rtx.krylov import BlockJacobi, InteriorSystem, gmres, refine
     from rtx._lib import PREC_F64, PREC_F32, PREC_MIXED
-    mesh, wl = _c1_small()
+    if variant.startswith("hp"):
+        # hp mesh on the default (generic, atomic-free) launch shapes
+        from test_gpu_apply import small_random_hp
+        mesh, wl = small_random_hp(1), problems.two_scatterers_c3()
+        variant = variant[3:]
+        tol, max_it = 1e-6, 4000
+    else:
+        mesh, wl = _c1_small()
+        tol, max_it = 1e-10, 500
     op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
     pc = BlockJacobi(op, keep_f32=True)
     rhs = op.mask(_dev(np.random.default_rng(4).uniform(-1, 1, op.ndof)))
@@ -129,14 +138,14 @@ def test_graph_replay_matches_eager(cuda, variant):
         if variant.startswith("mixed"):
             hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
             lo = InteriorSystem(op, PREC_MIXED, pc, PREC_F32)
-            x, info = refine(hi, lo, rhs, None, rtol=1e-10, inner_rtol=1e-2, restart=7,
-                             max_inner=500, chunk=3, graphs=graphs,
+            x, info = refine(hi, lo, rhs, None, rtol=tol, inner_rtol=1e-2, restart=7,
+                             max_inner=max_it, chunk=3, graphs=graphs,
                              basis="f64" if variant == "mixed_f64_basis" else "f32")
             assert info["converged_reason"] > 0
             runs.append((x.cpu().numpy(), info["residual_history"], info["iteration_number"]))
         else:
             system = InteriorSystem(op, PREC_F64, pc, PREC_F64)
-            x, info = gmres(system, rhs, None, rtol=1e-10, restart=7, max_it=500,
+            x, info = gmres(system, rhs, None, rtol=tol, restart=7, max_it=max_it,
                             flexible=variant == "flexible", chunk=3, graphs=graphs)
             assert info["converged_reason"] > 0
             if graphs:
