@@ -310,6 +310,75 @@ def test_E_theta_total_is_face_integral(n0, n1, dlv, F):
     assert abs(E.sum() - exact) < 1e-9 * max(1.0, abs(exact)), (E.sum(), exact)
 
 
+def _bary(nodes, pts):
+    """L[i, k] = l_i(pts[k]) by the barycentric formula (a route independent
+    of the oracle's product form); exact node hits give the unit vector."""
+    nodes = np.asarray(nodes, dtype=np.float64)
+    wb = np.array([1.0 / np.prod([nodes[j] - nodes[k] for k in range(len(nodes)) if k != j])
+                   for j in range(len(nodes))])
+    L = np.zeros((len(nodes), len(pts)))
+    for k, x in enumerate(pts):
+        d = x - nodes
+        hit = np.flatnonzero(d == 0.0)
+        if hit.size:
+            L[hit[0], k] = 1.0
+        else:
+            t = wb / d
+            L[:, k] = t / t.sum()
+    return L
+
+
+def _eth_unified(n0, a0, b0, n1, a1, b1, F):
+    """get_Eth.py:54-147 in one statement: with S the smaller of the own cell
+    K = [a0, b0] and the neighbour K' = [a1, b1] and m = max(n0, n1),
+    E[a, r] = (|S| / |K|) sum_p w_p Theta_F(theta_p) l1_a(xi'(theta_p))
+    l0_r(xi(theta_p)), the m-point Gauss rule placed on S."""
+    m = max(n0, n1)
+    lo, hi = (a1, b1) if (b1 - a1) < (b0 - a0) else (a0, b0)
+    xg, wg = np.polynomial.legendre.leggauss(m)
+    th = lo + (hi - lo) / 2.0 * (xg + 1.0)
+    x0 = np.polynomial.legendre.leggauss(n0)[0] if n0 > 1 else np.array([0.0])
+    x1 = np.polynomial.legendre.leggauss(n1)[0] if n1 > 1 else np.array([0.0])
+    # the cell's own nodes sit exactly on its reference rule: when S is that
+    # cell and the rule has its node count, evaluate at the reference nodes
+    xi0 = xg if (lo, hi) == (a0, b0) else 2.0 * (th - a0) / (b0 - a0) - 1.0
+    xi1 = xg if (lo, hi) == (a1, b1) else 2.0 * (th - a1) / (b1 - a1) - 1.0
+    L0, L1 = _bary(x0, xi0), _bary(x1, xi1)
+    E = (hi - lo) / (b0 - a0) * (L1 * (wg * np.cos(th - F * math.pi / 2.0))) @ L0.T
+    E[np.abs(E) < 1.0e-15] = 0.0
+    return E
+
+
+@pytest.mark.parametrize("F", [0, 1, 2, 3])
+@pytest.mark.parametrize("case", ["same", "own_child_lo", "own_child_hi", "nbr_child_lo",
+                                  "nbr_child_hi"])
+def test_E_theta_entries_are_one_quadrature(case, F):
+    """Every entry of every E^theta branch, mismatched orders included, is
+    the one quadrature of _eth_unified: the reference's four branches differ
+    only in which interval carries the max(n0, n1)-point rule and in the 1/2
+    of a finer neighbour.  Fails for a wrong rule size, a rule on the wrong
+    interval, a swapped map, a missing 1/2 or a transposed E."""
+    a0, b0 = 3 * math.pi / 4, math.pi          # level-3 cell
+    lo_c, mid, hi_c = a0, 7 * math.pi / 8, b0  # its children
+    for n0 in range(1, 10):
+        for n1 in range(1, 10):
+            if case == "same":
+                args = (n0, a0, b0, 3, n1, a0, b0, 3)
+            elif case == "own_child_lo":
+                args = (n0, lo_c, mid, 4, n1, a0, b0, 3)
+            elif case == "own_child_hi":
+                args = (n0, mid, hi_c, 4, n1, a0, b0, 3)
+            elif case == "nbr_child_lo":
+                args = (n0, a0, b0, 3, n1, lo_c, mid, 4)
+            else:
+                args = (n0, a0, b0, 3, n1, mid, hi_c, 4)
+            E = emat.E_theta(*args, F)
+            ref = _eth_unified(args[0], args[1], args[2], args[4], args[5], args[6], F)
+            assert E.shape == (n1, n0)
+            np.testing.assert_allclose(E, ref, rtol=0, atol=1e-14 * max(1.0, np.abs(ref).max()),
+                                       err_msg=str((case, n0, n1, F)))
+
+
 # ------------------------------------------------------------ matrices
 
 def tiny_hp_mesh():
