@@ -15,8 +15,10 @@ relative residual of 1e-8 with (a) plain FP64 GMRES + block-Jacobi and
 learned coarse preconditioner on solver-generated pairs of the random-field
 workload (64 x 64 x 16 cells, p = 4, 8,192,000 DOF) and times plain FP64,
 mixed and learned+mixed solves of an unseen field there; times the error
-indicator kernels on the same mesh; and times the oracle's assembled CSR
-SpMV on a bounded sample on the host CPU.
+indicator kernels on the same mesh; times the oracle's assembled CSR
+SpMV on a bounded sample on the host CPU; and solves a smaller mesh with
+C2's coefficients (16 x 16 x 8 cells, 256,000 DOF) to 1e-8 on the host CPU
+(scipy GMRES with the oracle's block-Jacobi) and on the GPU.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve]
                        [--no-cpu] [--no-learned] [--no-indicators]
@@ -159,6 +161,97 @@ def cpu_baseline(mesh, wl, ncols=128, budget_s=4.0):
                               "reps": reps1}}
 
 
+def cpu_solve_baseline(nref_ang=3, nref_spt=4):
+    """Time-to-1e-8 on the host CPU beside the GPU, on a workload small
+    enough for the CPU: C2's coefficients on 16 x 16 x 8 cells, p = 4
+    (256,000 DOF).  CPU: the oracle's assembled interior system A_II and
+    right-hand side f_I - A_IB b_B (``src/rt/solve.py:49-100``), scipy's
+    GMRES(50) (left-preconditioned, as PETSc's default) with the oracle's
+    per-cell block-Jacobi inverses, the CSR product and the block apply on
+    every host thread through the oracle's C library; initial guess f_I as
+    the reference's, then restarted until the true relative residual is
+    1e-8.  GPU: the same system by ``_timed_solve`` (FP64 and mixed).
+    ``seconds`` = block-Jacobi setup + solve on both sides; the CPU's
+    assembly time is reported apart (the GPU's plan build likewise)."""
+    import numpy as np
+    import scipy.sparse.linalg as spla
+    import torch
+    from oracle import assemble as asm, system as osys, build_c
+    from rtx import problems
+    from rtx.operator import TransportOperator
+    mesh = problems.uniform_mesh([1.0, 1.0], [5, 5, 5], nref_ang=nref_ang, nref_spt=nref_spt)
+    wl = problems.absorber_c2()
+    t0 = time.perf_counter()
+    V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi)
+    A = asm.system_matrix(V)
+    f = asm.forcing(mesh, wl.f)
+    b = asm.bcs(mesh, wl.bcs)
+    mask = asm.intr_mask(mesh)
+    A_II, rhs, I, _ = osys.interior_system(A, f, b, mask)
+    A_II.indices = A_II.indices.astype(np.int32)
+    t_asm = time.perf_counter() - t0
+    if not os.path.exists(build_c.OUT):
+        build_c.build()
+    lib = build_c.load()
+    nt = _cpu_threads()
+    indptr = np.ascontiguousarray(A_II.indptr, dtype=np.int64)
+
+    def matvec(v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.empty(A_II.shape[0])
+        lib.oracle_csr_spmv(A_II.shape[0], indptr.ctypes.data, A_II.indices.ctypes.data,
+                            A_II.data.ctypes.data, v.ctypes.data, out.ctypes.data, nt)
+        return out
+
+    t0 = time.perf_counter()
+    invs = osys.block_jacobi_inverses(A, mesh, mask)
+    pos = np.full(mask.size, -1, dtype=np.int64)
+    pos[I] = np.arange(I.size)
+    idx = np.ascontiguousarray(pos[np.stack(osys.bj_block_dofs(mesh))])
+    inv = np.ascontiguousarray(np.stack(invs))
+    del invs
+    t_bj = time.perf_counter() - t0
+    Aop = spla.LinearOperator(A_II.shape, matvec=matvec, dtype=np.float64)
+    M = spla.LinearOperator(A_II.shape, matvec=lambda r: build_c.bj_apply(lib, idx, inv, r, nt),
+                            dtype=np.float64)
+    its = [0]
+
+    def count(_):
+        its[0] += 1
+
+    nb = np.linalg.norm(rhs)
+    x = np.ascontiguousarray(f[I])
+    rtol, res, rounds = 1e-8, np.inf, 0
+    t0 = time.perf_counter()
+    while rounds < 5:
+        x, _ = spla.gmres(Aop, rhs, x0=x, rtol=rtol, atol=0.0, restart=50, maxiter=5000,
+                          M=M, callback=count, callback_type="pr_norm")
+        rounds += 1
+        res = float(np.linalg.norm(rhs - matvec(x)) / nb)
+        if res <= 1e-8:
+            break
+        rtol *= 0.1
+    t_solve = time.perf_counter() - t0
+    out = {"workload": "uniform absorber 16x16x8 cells, p=4 ({} DOF), C2's coefficients"
+                       .format(mask.size),
+           "cpu": {"seconds": t_bj + t_solve, "setup_seconds": t_bj, "solve_seconds": t_solve,
+                   "assembly_seconds": t_asm, "iterations": its[0], "rel_residual": res,
+                   "converged": res <= 1e-8, "cores": nt,
+                   "method": "scipy GMRES(50) + the oracle's block-Jacobi, CSR product and "
+                             "block apply in C (OpenMP, {} threads)".format(nt)}}
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    grhs = _rhs(op, wl)
+    for prec in ("fp64", "mixed"):
+        xg, out[prec] = _timed_solve(op, grhs, prec)
+        xg = xg.cpu().numpy()[I]
+        out[prec]["rel_l2_vs_cpu"] = float(np.linalg.norm(xg - x) / np.linalg.norm(x))
+        out["speedup_{}_vs_cpu".format(prec)] = out["cpu"]["seconds"] / out[prec]["seconds"]
+        out["speedup_{}_vs_cpu_warm".format(prec)] = t_solve / out[prec]["solve_seconds_warm"]
+    del op
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -246,6 +339,8 @@ def main():
                           for ks in args.learned_kappa_scales.split(",") if ks.strip()]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, wl)
+        if not args.no_solve:
+            out["cpu_baseline"]["solve"] = cpu_solve_baseline()
     print(json.dumps(out), flush=True)
 
 

@@ -3,7 +3,8 @@
 Compiles ``oracle/csr_spmv.c`` with gcc and OpenMP into
 ``oracle/liboracle_spmv.so`` (git-ignored, like every built library).  Test
 infrastructure only: the library is the multi-core CPU baseline that
-``bench.py`` times beside the GPU apply.
+``bench.py`` times beside the GPU apply (CSR product) and the GPU solve (CSR
+product and block-Jacobi apply inside scipy's GMRES).
 """
 import os
 import subprocess
@@ -35,6 +36,9 @@ def load():
     lib.oracle_csr_spmv.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_int]
     lib.oracle_csr_spmv.restype = None
+    lib.oracle_bj_apply.argtypes = [C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_int]
+    lib.oracle_bj_apply.restype = None
     return lib
 
 
@@ -49,6 +53,24 @@ def csr_spmv(lib, A, x, nthreads):
     lib.oracle_csr_spmv(A.shape[0], indptr.ctypes.data, indices.ctypes.data, data.ctypes.data,
                         x.ctypes.data, y.ctypes.data, int(nthreads))
     return y
+
+
+def bj_apply(lib, idx, inv, r, nthreads, z=None):
+    """z = blockdiag(inv) r by the C library.  idx: int64 (nblocks, bs)
+    positions in r (-1: a boundary DOF, left out); inv: float64 (nblocks,
+    bs, bs).  Entries of z that no block names keep their value (zero when
+    z is not given)."""
+    import numpy as np
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    inv = np.ascontiguousarray(inv, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    if z is None:
+        z = np.zeros_like(r)
+    nb, bs = idx.shape
+    assert inv.shape == (nb, bs, bs) and idx.max(initial=-1) < r.size and z.size == r.size
+    lib.oracle_bj_apply(nb, bs, idx.ctypes.data, inv.ctypes.data, r.ctypes.data,
+                        z.ctypes.data, int(nthreads))
+    return z
 
 
 if __name__ == "__main__":

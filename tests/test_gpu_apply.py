@@ -350,3 +350,23 @@ def test_apply_periodic_matches_oracle(cuda, pbcs, path):
     mask = asm.intr_mask(mesh)
     y = op.apply(xd, ops=OP_SYSTEM | OP_RESTRICT).cpu().numpy()
     assert _rel(y, mask * (A @ (mask * x))) < TOL64
+
+
+@pytest.mark.gpu
+def test_copy_f64_measurement_helper(cuda):
+    """rtx_copy_f64 (the counter-calibration copy) copies exactly, for sizes
+    that are not a multiple of its workgroup, and does nothing for n = 0."""
+    import ctypes as C
+    import torch
+    from rtx import _lib
+    lib = _lib.load()
+    for n in (1, 255, 257, 1 << 20):
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.full((n + 1,), -7.0, dtype=torch.float64, device="cuda")
+        _lib.check(lib.rtx_copy_f64(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), n,
+                                    None), "rtx_copy_f64")
+        torch.cuda.synchronize()
+        assert torch.equal(dst[:n], src) and float(dst[n]) == -7.0, n
+    _lib.check(lib.rtx_copy_f64(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), 0, None),
+               "rtx_copy_f64")
+    assert lib.rtx_copy_f64(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), -1, None) != 0

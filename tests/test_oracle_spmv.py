@@ -41,3 +41,32 @@ def test_assembled_system_matches_scipy(lib):
     ref = A @ x
     assert np.array_equal(y, ref)
     assert lib.oracle_spmv_threads() >= 1
+
+
+def test_bj_apply_matches_block_jacobi_oracle(lib):
+    """The C block-Jacobi apply of the CPU solve against the oracle's
+    per-block numpy product, on interior positions of an hp mesh's system
+    (blocks of different sizes, grouped by size)."""
+    from oracle import system as osys
+    from test_gpu_apply import hp_mesh
+    mesh = hp_mesh()
+    wl = problems.two_scatterers()
+    V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi)
+    A = asm.system_matrix(V)
+    mask = asm.intr_mask(mesh)
+    invs = osys.block_jacobi_inverses(A, mesh, mask)
+    blocks = osys.bj_block_dofs(mesh)
+    r = np.random.default_rng(2).uniform(-1, 1, mask.size)
+    ref = osys.block_jacobi_apply(invs, mesh, r)
+    I = np.flatnonzero(mask)
+    pos = np.full(mask.size, -1, dtype=np.int64)
+    pos[I] = np.arange(I.size)
+    z = np.zeros(I.size)
+    sizes = sorted({len(b) for b in blocks})
+    assert len(sizes) > 1 and (~mask).any()
+    for bs in sizes:
+        sel = [k for k, b in enumerate(blocks) if len(b) == bs]
+        idx = np.stack([pos[blocks[k]] for k in sel])
+        inv = np.stack([invs[k] for k in sel])
+        build_c.bj_apply(lib, idx, inv, r[I], 3, z)
+    assert np.linalg.norm(z - ref[I]) / np.linalg.norm(ref[I]) < 1e-14
