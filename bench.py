@@ -255,8 +255,8 @@ def cpu_solve_baseline(nref_ang=3, nref_spt=4):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(--gpus {} requested)".format(args.gpus), file=sys.stderr)
+        print("bench.py runs on one GPU only: the operator and solver have no multi-GPU "
+              "path (--gpus {} requested)".format(args.gpus), file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
