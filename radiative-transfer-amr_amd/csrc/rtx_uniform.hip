@@ -23,9 +23,12 @@ typedef short u_sht4 __attribute__((ext_vector_type(4)));
 constexpr int UNM = RTX_NMAX;
 constexpr int UXCD = 8;
 
-__device__ inline bool u_on_bdry(int bd, int i, int j, int n) {
-    return ((bd & 1) && i == n - 1) || ((bd & 2) && j == n - 1) || ((bd & 4) && i == 0) ||
-           ((bd & 8) && j == 0);
+// Faces of the cell that spatial node (i, j) lies on, as cell_bdry bits:
+// 1 right (i = n - 1), 2 top (j = n - 1), 4 left (i = 0), 8 bottom (j = 0).
+// A node is a masked boundary node of a cell with boundary bits bd exactly
+// when bd & u_node_faces(i, j, n) != 0.
+__device__ inline int u_node_faces(int i, int j, int n) {
+    return (i == n - 1 ? 1 : 0) | (j == n - 1 ? 2 : 0) | (i == 0 ? 4 : 0) | (j == 0 ? 8 : 0);
 }
 
 __device__ inline uint16_t u_bf16(float f) {
@@ -56,7 +59,16 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     constexpr int MAXL = 4 * OC;               // links of the part (<= 2 per cell per
                                                // face pair; 4 leaves room for hp-free meshes)
     constexpr int TRL = N * NTH;               // trace values per link
-    constexpr int XU = (NCD + UTHREADS - 1) / UTHREADS;
+    // DOF passes t = t0 + u XS over a column (cell-major, CD DOFs per cell):
+    // when a cell fits the workgroup, a pass covers CPP whole cells and a
+    // thread keeps one in-cell position in every pass, so its (node, angle)
+    // indices are computed once; threads past CPP CD idle (t0 = NCD)
+    constexpr int CPP = CD <= UTHREADS ? UTHREADS / CD : 0;
+    constexpr int XS = CPP ? CPP * CD : UTHREADS;
+    constexpr int XU = (NCD + XS - 1) / XS;
+    // the same for trace and flux slots (TRL values per link, TRL <= 81)
+    constexpr int LPP = UTHREADS / TRL;
+    constexpr int TS = LPP * TRL;
     // LDS rows per spatial node, angle-contiguous, odd strides so 16 lanes on
     // 16 consecutive nodes hit distinct banks
     constexpr int NAP = NA | 1, NOAP = NOA | 1;
@@ -72,6 +84,7 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     __shared__ A wx_[N], WD[N * N];
     __shared__ A dth_[NC];
     __shared__ int cbd[NC], cqd[NC];
+    __shared__ int nfc[NSP];                   // u_node_faces of each spatial node
     __shared__ int l_src[MAXL], l_meta[MAXL], l_es[MAXL], l_et[MAXL];
     __shared__ int cfx[OC], cfy[OC];           // per part cell: 4 link + F of its x- / y-face
                                                // link, -1 without one
@@ -137,10 +150,11 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
         }
     };
     if constexpr (!MFMA_FIRST) load_b();
+    const int t0 = tid < XS ? tid : NCD;
     A xv[XU];
 #pragma unroll
     for (int u = 0; u < XU; ++u) {
-        const int t = tid + u * UTHREADS;
+        const int t = t0 + u * XS;
         xv[u] = (t < NCD) ? A(x[cdof + t]) : A(0);
     }
     double r_c = 0, r_s = 0, r_w = 0, r_kap = 0, r_sig = 0, r_wd = 0, r_g = 0;
@@ -182,6 +196,7 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     }
     if (tid < N * N) WD[tid] = A(r_wd);
     if (tid < N) wx_[tid] = A(m.lgl_w[N * UNM + tid]);
+    if (tid < NSP) nfc[tid] = u_node_faces(tid / N, tid % N, N);
     if (tid < NC) {
         cbd[tid] = restrict_ ? r_bd : 0;
         cqd[tid] = r_q;
@@ -205,46 +220,61 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     }
 
     // ---- phase 1: column input (masked) and the part's neighbour traces
+    {
+        const int lt0 = t0 % CD, sn0 = lt0 / NTH, a0 = lt0 % NTH, nf0 = nfc[sn0 < NSP ? sn0 : 0];
 #pragma unroll
-    for (int u = 0; u < XU; ++u) {
-        const int t = tid + u * UTHREADS;
-        if (t < NCD) {
-            A v = xv[u];
-            const int kk = t / CD, lt = t % CD, sn = lt / NTH, a = lt % NTH;
-            if (restrict_ && cbd[kk] && u_on_bdry(cbd[kk], sn / N, sn % N, N)) v = A(0);
-            xs[sn * NAP + kk * NTH + a] = v;
+        for (int u = 0; u < XU; ++u) {
+            const int t = t0 + u * XS;
+            if (t < NCD) {
+                A v = xv[u];
+                int kk, sn, a, nf;
+                if constexpr (CPP > 0) {
+                    kk = t0 / CD + u * CPP;
+                    sn = sn0;
+                    a = a0;
+                    nf = nf0;
+                } else {
+                    const int lt = t % CD;
+                    kk = t / CD;
+                    sn = lt / NTH;
+                    a = lt % NTH;
+                    nf = nfc[sn];
+                }
+                if (cbd[kk] & nf) v = A(0);   // cbd is 0 without RESTRICT
+                xs[sn * NAP + kk * NTH + a] = v;
+            }
         }
     }
     if (bconv) {
-        constexpr int TU = (MAXL * TRL + UTHREADS - 1) / UTHREADS;
+        constexpr int TU = (MAXL * TRL + TS - 1) / TS;
+        const int e0 = tid < TS ? tid : MAXL * TRL;
+        const int le = e0 % TRL, m1 = le / NTH, a = le % NTH;
         int addr[TU];
         bool zero[TU];
         A v[TU];
 #pragma unroll
         for (int u = 0; u < TU; ++u) {
-            const int e = tid + u * UTHREADS;
-            const int li = e / TRL, le = e % TRL;
+            const int li = e0 / TRL + u * LPP;
             addr[u] = -1;
             zero[u] = false;
             if (li < nl) {
                 const int w1 = l_meta[li];
                 const int F = w1 & 3, bd1 = restrict_ ? ((w1 >> 14) & 15) : 0;
-                const int m1 = le / NTH, a = le % NTH;
                 int i1, j1;
                 if (F == 0) { i1 = 0; j1 = m1; }
                 else if (F == 2) { i1 = N - 1; j1 = m1; }
                 else if (F == 1) { i1 = m1; j1 = 0; }
                 else { i1 = m1; j1 = N - 1; }
                 addr[u] = l_src[li] + (i1 * N + j1) * NTH + a;
-                zero[u] = bd1 && u_on_bdry(bd1, i1, j1, N);
+                zero[u] = (bd1 & nfc[i1 * N + j1]) != 0;
             }
         }
 #pragma unroll
         for (int u = 0; u < TU; ++u) v[u] = (addr[u] >= 0) ? A(x[addr[u]]) : A(0);
 #pragma unroll
         for (int u = 0; u < TU; ++u) {
-            const int e = tid + u * UTHREADS;
-            if (e < nl * TRL) tr[e] = zero[u] ? A(0) : v[u];
+            const int li = e0 / TRL + u * LPP;
+            if (li < nl) tr[li * TRL + le] = zero[u] ? A(0) : v[u];
         }
     }
     if constexpr (MFMA_FIRST) load_b();
@@ -333,6 +363,20 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
 #pragma unroll
             for (int i = 0; i < N; ++i) xl[i] = xs[(i * N + q) * NAP + ca];
             const A cx = A(dy * 0.25) * dth * wx_[q] * wr * c_;
+            // outflow faces quad and quad + 1 of the line's cell: one x-face
+            // (F even) and one y-face (F odd), added in that face order, each
+            // term's products in the order of the per-node formula
+            // A((F odd ? dx : dy) / 4) dth (F odd ? w_p : w_q) w_r Theta_F x.
+            // The x-face adds a per-line constant at one end node; the
+            // y-face adds to every node when q is its end (yon).
+            const bool obc = bconv && quad >= 0;
+            const bool yfirst = (quad & 1) != 0;
+            const int Fx = yfirst ? ((quad + 1) & 3) : quad, Fy = yfirst ? quad : ((quad + 1) & 3);
+            const A xtf = Fx == 0 ? c_ : -c_, ytf = Fy == 1 ? s_ : -s_;
+            const A xb = A(dy * 0.25) * dth * wx_[q] * wr * xtf;
+            const A yb = A(dx * 0.25) * dth;
+            const int xtg = Fx == 0 ? N - 1 : 0;
+            const bool yon = q == (Fy == 1 ? N - 1 : 0);
 #pragma unroll
             for (int pp = 0; pp < N; ++pp) {
                 A acc = A(0);
@@ -345,18 +389,11 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
                     for (int i = 0; i < N; ++i) sx += WD[pp * N + i] * xl[i];
                     acc -= cx * sx;
                 }
-                if (bconv && quad >= 0) {
-#pragma unroll
-                    for (int hh = 0; hh < 2; ++hh) {
-                        const int F = (quad + hh) & 3;
-                        const bool on = (F == 0 && pp == N - 1) || (F == 2 && pp == 0) ||
-                                        (F == 1 && q == N - 1) || (F == 3 && q == 0);
-                        if (on) {
-                            const A tf = F == 0 ? c_ : (F == 1 ? s_ : (F == 2 ? -c_ : -s_));
-                            acc += A(((F & 1) ? dx : dy) * 0.25) * dth *
-                                   ((F & 1) ? wx_[pp] : wx_[q]) * wr * tf * xn;
-                        }
-                    }
+                if (obc) {
+                    const A yt = yb * wx_[pp] * wr * ytf * xn;
+                    if (yfirst && yon) acc += yt;
+                    if (pp == xtg) acc += xb * xn;
+                    if (!yfirst && yon) acc += yt;
                 }
                 ys[(pp * N + q) * NOAP + oa] = acc;
             }
@@ -388,11 +425,11 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     // other mesh here) in a fixed order: bit-reproducible without atomics.
     // Phase 4 does not read the slots, so no barrier separates them.
     if (bconv) {
-        for (int e = tid; e < nl * N * NTH; e += UTHREADS) {
-            const int li = e / (N * NTH), le = e % (N * NTH);
+        const int f0 = tid < TS ? tid : MAXL * TRL;
+        const int le = f0 % TRL, m0 = le / NTH, r = le % NTH;
+        for (int li = f0 / TRL; li < nl; li += LPP) {
             const int w1 = l_meta[li];
             const int F = w1 & 3, kk = w1 >> 18;
-            const int m0 = le / NTH, r = le % NTH;
             const double *Es = m.Es_pool + l_es[li];
             const double *Et = m.Eth_pool + l_et[li];
             A et[NTH], es[N];
@@ -411,7 +448,7 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
                 for (int a = 0; a < NTH; ++a) inner += et[a] * trl[m1 * NTH + a];
                 acc += es[m1] * inner;
             }
-            fl[e] = A(((F & 1) ? dx : dy) * 0.25) * dth_[kk] * acc;
+            fl[li * TRL + le] = A(((F & 1) ? dx : dy) * 0.25) * dth_[kk] * acc;
         }
     }
 
@@ -429,16 +466,18 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     }
     __syncthreads();
 
-    // ---- phase 5: write back the part's DOFs
-    for (int e = tid; e < NOWN; e += UTHREADS) {
-        const int t = own0 + e;
-        const int kk = t / CD, lt = t % CD, sn = lt / NTH, a = lt % NTH;
-        const bool masked = restrict_ && cbd[kk] && u_on_bdry(cbd[kk], sn / N, sn % N, N);
-        A v = ys[sn * NOAP + (kk - kk_lo) * NTH + a];
+    // ---- phase 5: write back the part's DOFs (own DOF e = kl CD + lt of
+    // part cell kl), by whole-cell passes as in phase 1 when a cell fits the
+    // workgroup
+    auto emit = [&](int kl, int lt, int sn) {
+        const int kk = kk_lo + kl, a = lt % NTH;
+        const int t = own0 + kl * CD + lt;
+        const bool masked = (cbd[kk] & nfc[sn]) != 0;
+        A v = ys[sn * NOAP + kl * NTH + a];
         if (bconv) {
             // inflow face fluxes: x-face, then y-face
             const int i = sn / N, j = sn % N;
-            const int fx = cfx[kk - kk_lo], fy = cfy[kk - kk_lo];
+            const int fx = cfx[kl], fy = cfy[kl];
             if (fx >= 0 && i == ((fx & 3) == 0 ? N - 1 : 0)) v += fl[((fx >> 2) * N + j) * NTH + a];
             if (fy >= 0 && j == ((fy & 3) == 1 ? N - 1 : 0)) v += fl[((fy >> 2) * N + i) * NTH + a];
         }
@@ -448,6 +487,14 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
         } else {
             y[cdof + t] = T(v);
         }
+    };
+    if constexpr (CPP > 0) {
+        if (tid < XS) {
+            const int lt = tid % CD, sn = lt / NTH;
+            for (int kl = tid / CD; kl < OC; kl += CPP) emit(kl, lt, sn);
+        }
+    } else {
+        for (int e = tid; e < NOWN; e += UTHREADS) emit(e / CD, e % CD, (e % CD) / NTH);
     }
 }
 
