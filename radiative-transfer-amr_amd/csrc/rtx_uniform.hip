@@ -37,6 +37,20 @@ __device__ inline uint16_t u_bf16(float f) {
     return uint16_t(u >> 16);
 }
 
+// LDS row stride (in elements) of an [node][angle] tile with na angles per
+// node row.  With an odd angle count per cell (nth), a stride congruent to
+// nth mod 16 puts the rows of consecutive nodes of one cell at consecutive
+// addresses modulo 16, so lanes walking a cell's (node, angle) DOFs in order
+// hit distinct banks; the stride stays odd, so lanes on 16 consecutive nodes
+// at one angle (the MFMA operand reads) do too.  Even nth: the next odd
+// count.
+constexpr int u_pad(int na, int nth) {
+    if (nth % 2 == 0) return na | 1;
+    int v = na;
+    while (v % 16 != nth % 16) ++v;
+    return v;
+}
+
 // Threads per workgroup: one wavefront per 16-wide output-angle tile of the
 // part, at least four.
 template <int NTH, int NC, int H>
@@ -69,9 +83,8 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     // the same for trace and flux slots (TRL values per link, TRL <= 81)
     constexpr int LPP = UTHREADS / TRL;
     constexpr int TS = LPP * TRL;
-    // LDS rows per spatial node, angle-contiguous, odd strides so 16 lanes on
-    // 16 consecutive nodes hit distinct banks
-    constexpr int NAP = NA | 1, NOAP = NOA | 1;
+    // LDS rows per spatial node, angle-contiguous, strides from u_pad
+    constexpr int NAP = u_pad(NA, NTH), NOAP = u_pad(NOA, NTH);
     static_assert(NTILE <= UTHREADS / 64, "one output-angle tile per wavefront");
     static_assert(NC % H == 0, "parts must split the cells evenly");
     extern __shared__ __align__(16) unsigned char u_lds[];
@@ -355,7 +368,9 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
         constexpr int NLINE = OC * N * NTH;
         const A cm = A(dx * dy * 0.125);
         for (int e = tid; e < NLINE; e += UTHREADS) {
-            const int kl = e / (N * NTH), q = (e / NTH) % N, r = e % NTH;
+            // line (q, part angle oa): consecutive lanes read and write
+            // consecutive angles of one node row
+            const int q = e / NOA, oa0 = e % NOA, kl = oa0 / NTH, r = oa0 % NTH;
             const int kk = kk_lo + kl, ca = kk * NTH + r, oa = kl * NTH + r;
             const A dth = dth_[kk], wr = cw[ca], c_ = cth[ca], s_ = sth[ca];
             const int quad = cqd[kk];
@@ -401,7 +416,7 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
         if (ops & RTX_OP_ICONV) {
             __syncthreads();
             for (int e = tid; e < NLINE; e += UTHREADS) {
-                const int kl = e / (N * NTH), pp = (e / NTH) % N, r = e % NTH;
+                const int pp = e / NOA, oa0 = e % NOA, kl = oa0 / NTH, r = oa0 % NTH;
                 const int kk = kk_lo + kl, ca = kk * NTH + r, oa = kl * NTH + r;
                 A xl[N];
 #pragma unroll
@@ -501,7 +516,7 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
 template <typename T, typename A, int PREC, int N, int NTH, int NC, int H>
 int launch_one(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t st, int acc) {
     constexpr int NSP = N * N, NA = NC * NTH;
-    constexpr int NAP = NA | 1, NOAP = ((NC / H) * NTH) | 1;
+    constexpr int NAP = u_pad(NA, NTH), NOAP = u_pad((NC / H) * NTH, NTH);
     constexpr int MAXL = 4 * (NC / H), TRL = N * NTH;
     RTX_CHECK(p->max_part_links[H] <= MAXL, "uniform apply: %d links in a column part (max %d)",
               p->max_part_links[H], MAXL);
