@@ -99,7 +99,9 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     __shared__ int cbd[NC], cqd[NC];
     __shared__ int nfc[NSP];                   // u_node_faces of each spatial node
     __shared__ int l_src[MAXL], l_meta[MAXL], l_es[MAXL], l_et[MAXL];
-    __shared__ int cfx[OC], cfy[OC];           // per part cell: 4 link + F of its x- / y-face
+    // per part cell, its inflow x- / y-face link's flux slots: (first slot
+    // << 4) | (1 + the node row i / column j on that face), 0 without one
+    __shared__ int cfx[OC], cfy[OC];
                                                // link, -1 without one
 
     // workgroups go to the XCDs round-robin (b mod 8); XCD j takes the
@@ -222,14 +224,15 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
         l_et[tid] = r_et;
     }
     if (tid < OC) {
-        cfx[tid] = -1;
-        cfy[tid] = -1;
+        cfx[tid] = 0;
+        cfy[tid] = 0;
     }
     __syncthreads();
     if (bconv && tid < nl) {
         const int kl = (r_meta >> 18) - kk_lo, F = r_meta & 3;
-        if (F & 1) cfy[kl] = 4 * tid + F;
-        else cfx[kl] = 4 * tid + F;
+        const int slots = (tid * TRL) << 4;
+        if (F & 1) cfy[kl] = slots | (1 + (F == 1 ? N - 1 : 0));
+        else cfx[kl] = slots | (1 + (F == 0 ? N - 1 : 0));
     }
 
     // ---- phase 1: column input (masked) and the part's neighbour traces
@@ -493,8 +496,8 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
             // inflow face fluxes: x-face, then y-face
             const int i = sn / N, j = sn % N;
             const int fx = cfx[kl], fy = cfy[kl];
-            if (fx >= 0 && i == ((fx & 3) == 0 ? N - 1 : 0)) v += fl[((fx >> 2) * N + j) * NTH + a];
-            if (fy >= 0 && j == ((fy & 3) == 1 ? N - 1 : 0)) v += fl[((fy >> 2) * N + i) * NTH + a];
+            if (i + 1 == (fx & 15)) v += fl[(fx >> 4) + j * NTH + a];
+            if (j + 1 == (fy & 15)) v += fl[(fy >> 4) + i * NTH + a];
         }
         if (masked) v = A(0);
         if (accumulate) {
