@@ -51,6 +51,18 @@ constexpr int u_pad(int na, int nth) {
     return v;
 }
 
+// BF16 copy of the column for the MIXED scattering MFMA: rows padded to
+// whole 16-node tiles, columns to whole 16-angle k-steps, both with zeros;
+// row stride na16 + 4 (a multiple of 4 elements, so one 8-byte read per
+// operand; 2 dwords mod 32 banks apart per row, so 16 rows hit distinct
+// banks).  Byte offset of the copy: the FP32 column and output tiles,
+// rounded up to 16 bytes.
+constexpr int u_bf_stride(int na) { return (na + 15) / 16 * 16 + 4; }
+constexpr size_t u_bf_off(size_t tiles_bytes) { return (tiles_bytes + 15) / 16 * 16; }
+constexpr size_t u_bf_bytes(int nsp, int na) {
+    return size_t((nsp + 15) / 16 * 16) * u_bf_stride(na) * 2;
+}
+
 // Threads per workgroup: one wavefront per 16-wide output-angle tile of the
 // part, at least four.
 template <int NTH, int NC, int H>
@@ -90,7 +102,13 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     extern __shared__ __align__(16) unsigned char u_lds[];
     A *xs = reinterpret_cast<A *>(u_lds);      // [NSP][NAP]: column x, angle A of node s
     A *ys = xs + NSP * NAP;                    // [NSP][NOAP]: part outputs
-    A *tr = ys + NSP * NOAP;                   // links * TRL: neighbour traces
+    // MIXED: [MT 16][NAPB] BF16 copy of the column (MFMA A operands)
+    constexpr bool XB = PREC == RTX_PREC_MIXED;
+    constexpr int NAPB = u_bf_stride(NA);
+    constexpr size_t XB_OFF = u_bf_off(size_t(NSP * NAP + NSP * NOAP) * sizeof(A));
+    uint16_t *xb = reinterpret_cast<uint16_t *>(u_lds + XB_OFF);
+    A *tr = XB ? reinterpret_cast<A *>(u_lds + XB_OFF + u_bf_bytes(NSP, NA))
+               : ys + NSP * NOAP;              // links * TRL: neighbour traces
     A *fl = tr + nlink_lds * TRL;              // links * TRL: face fluxes (phase 3)
     __shared__ A cth[NA], sth[NA], cw[NA];
     __shared__ A kap[NSP], sgw[NSP];
@@ -258,8 +276,18 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
                 }
                 if (cbd[kk] & nf) v = A(0);   // cbd is 0 without RESTRICT
                 xs[sn * NAP + kk * NTH + a] = v;
+                if constexpr (XB) xb[sn * NAPB + kk * NTH + a] = u_bf16(float(v));
             }
         }
+    }
+    if constexpr (XB) {
+        // zero the BF16 tile's padding: angles [NA, NKB 16) of the NSP
+        // nodes, then every angle of the rows [NSP, MT 16)
+        constexpr int PC = NKB * 16 - NA, PR = (MT * 16 - NSP) * NKB * 16;
+        if constexpr (PC > 0)
+            for (int e = tid; e < NSP * PC; e += UTHREADS) xb[(e / PC) * NAPB + NA + e % PC] = 0;
+        for (int e = tid; e < PR; e += UTHREADS)
+            xb[(NSP + e / (NKB * 16)) * NAPB + e % (NKB * 16)] = 0;
     }
     if (bconv) {
         constexpr int TU = (MAXL * TRL + TS - 1) / TS;
@@ -349,13 +377,8 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
 #pragma unroll
                 for (int t = 0; t < MT; ++t) {
                     const int s = t * 16 + l16;
-                    u_sht4 av;
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int Ak = ks * 16 + 4 * lg + u;
-                        av[u] = (Ak < NA && s < NSP) ? short(u_bf16(float(xs[s * NAP + Ak])))
-                                                     : short(0);
-                    }
+                    // four consecutive angles of node s, zero past NA or NSP
+                    const u_sht4 av = *reinterpret_cast<const u_sht4 *>(xb + s * NAPB + ks * 16 + 4 * lg);
                     sacc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[ks], sacc[t], 0, 0, 0);
                 }
             }
@@ -525,7 +548,10 @@ int launch_one(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t 
               p->max_part_links[H], MAXL);
     // column and output rows, then trace and flux slots for the largest link
     // count of any part
-    const size_t lds = size_t(NSP * NAP + NSP * NOAP + 2 * p->max_part_links[H] * TRL) * sizeof(A);
+    constexpr bool XB = PREC == RTX_PREC_MIXED;
+    const size_t tiles = XB ? u_bf_off(size_t(NSP * NAP + NSP * NOAP) * sizeof(A)) + u_bf_bytes(NSP, NA)
+                            : size_t(NSP * NAP + NSP * NOAP) * sizeof(A);
+    const size_t lds = tiles + size_t(2 * p->max_part_links[H] * TRL) * sizeof(A);
     auto kern = k_uniform<T, A, PREC, N, NTH, NC, H>;
     RTX_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 int(lds)));
