@@ -127,3 +127,35 @@ def test_lds_fits_three_workgroups_for_the_c2_bin(n, nth, nc, h):
         # the C2 bin has at most 2 links per cell: 3 workgroups per CU
         lds2 = (nsp * u_pad(na, nth) * 2 + 2 * 2 * oc * n * nth) * 8 + 4400
         assert 3 * lds2 <= 160 * 1024
+
+
+@pytest.mark.parametrize("n,nth,nc", BINS)
+def test_bf16_tile_padding_and_operands(n, nth, nc):
+    """The mixed path's BF16 column tile: phase 1 and the padding loops
+    write every element the MFMA operand reads exactly once, the padding is
+    exactly the region past NA angles or NSP nodes, and every operand read
+    (four consecutive angles) is 8-byte aligned inside the tile."""
+    nsp, na = n * n, nc * nth
+    mt, nkb = (nsp + 15) // 16, (na + 15) // 16
+    napb = (na + 15) // 16 * 16 + 4
+    assert napb % 4 == 0 and napb >= nkb * 16
+    written = {}
+    for sn in range(nsp):
+        for col in range(na):
+            written[(sn, col)] = written.get((sn, col), 0) + 1
+    pc, pr = nkb * 16 - na, (mt * 16 - nsp) * nkb * 16
+    for e in range(nsp * pc):
+        key = (e // pc, na + e % pc)
+        written[key] = written.get(key, 0) + 1
+    for e in range(pr):
+        key = (nsp + e // (nkb * 16), e % (nkb * 16))
+        written[key] = written.get(key, 0) + 1
+    assert all(v == 1 for v in written.values())
+    assert set(written) == {(r, c) for r in range(mt * 16) for c in range(nkb * 16)}
+    for t in range(mt):
+        for ks in range(nkb):
+            for lane in range(64):
+                s, lg = t * 16 + lane % 16, lane // 16
+                first = s * napb + ks * 16 + 4 * lg
+                assert (first * 2) % 8 == 0
+                assert all((s, ks * 16 + 4 * lg + u) in written for u in range(4))
