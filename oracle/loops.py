@@ -14,6 +14,9 @@ same drop thresholds, emitting COO triplets:
 * ``preconditioner`` - ``src/rt/preconditioner_matrix.py:116-267`` (M + (-S))
 * ``forcing`` / ``bcs`` / ``intr_mask`` - ``forcing_vector.py:103-156``,
   ``boundary_conditions_vector.py:101-207``, ``get_masks.py:22-129``
+* ``dirac_bcs``      - the Dirac-delta branch of
+                       ``boundary_conditions_vector.py:142-196`` (per-column
+                       write index, cells skipped without advancing it)
 
 Only usable on meshes of a few hundred DOFs.  ``assemble.py`` is checked
 against these on such meshes.
@@ -21,7 +24,7 @@ against these on such meshes.
 import numpy as np
 import scipy.sparse as sp
 
-from .quad import lgl, lg, dlag, push
+from .quad import lgl, lg, dlag, push, pull, lag
 from .emat import E_space, E_theta, theta_F
 from .meshq import leaf_order, dof_offsets, nhbr_cells
 
@@ -301,6 +304,75 @@ def bcs(mesh, bcs_fn):
                 for j in range(ny):
                     for a in range(nth):
                         out[o + nth * ny * i + nth * j + a] = bcs_fn(xf[i], yf[j], thf[a])
+    return out
+
+
+def dirac_bcs(mesh, bcs_fn, dirac):
+    """``boundary_conditions_vector.py:101-207`` as written: per column a zero
+    vector filled at a running index ``idx`` (``:127-196``), the delta branch
+    taken when ``any(dirac)`` (``:142``), each cell written only when one of
+    its three "in cell" flags holds (``:164``), scalar bcs calls; the column
+    vectors are placed at the columns' first DOFs (``:55-84``)."""
+    offs, n = dof_offsets(mesh)
+    out = np.zeros(n)
+    for ck, kks in leaf_order(mesh):
+        col = mesh.cols[ck]
+        x0, y0, x1, y1 = col.pos
+        nx, ny = col.ndofs
+        xxb, yyb = lgl(nx)[0], lgl(ny)[0]
+        xxf, yyf = push(x0, x1, xxb), push(y0, y1, yyb)
+        col_ndof = sum(nx * ny * col.cells[kk].ndofs[0] for kk in kks)
+        col_vec = np.zeros(col_ndof)
+        idx = 0
+        for kk in kks:
+            cell = col.cells[kk]
+            th0, th1 = cell.pos
+            nth = cell.ndofs[0]
+            thb = lg(nth)[0]
+            thf = push(th0, th1, thb)
+            if any(dirac):
+                in_cell = [True, True, True]
+                if dirac[0] is not None:
+                    if dirac[0] < x0 or x1 < dirac[0]:
+                        in_cell[0] = False
+                    else:
+                        xs_b = pull(x0, x1, dirac[0])
+                if dirac[1] is not None:
+                    if dirac[1] < y0 or y1 < dirac[1]:
+                        in_cell[1] = False
+                    else:
+                        ys_b = pull(y0, y1, dirac[1])
+                if dirac[2] is not None:
+                    if dirac[2] < th0 or th1 < dirac[2]:
+                        in_cell[2] = False
+                    else:
+                        ths_b = pull(th0, th1, dirac[2])
+                if any(in_cell):
+                    for ii in range(nx):
+                        if dirac[0] is not None and in_cell[0]:
+                            phi_i, x_i = max(0.0, lag(xxb, ii, xs_b)), dirac[0]
+                        else:
+                            phi_i, x_i = 1.0, xxf[ii]
+                        for jj in range(ny):
+                            if dirac[1] is not None and in_cell[1]:
+                                psi_j, y_j = max(0.0, lag(yyb, jj, ys_b)), dirac[1]
+                            else:
+                                psi_j, y_j = 1.0, yyf[jj]
+                            for aa in range(nth):
+                                if dirac[2] is not None and in_cell[2]:
+                                    xsi_a, th_a = max(0.0, lag(thb, aa, ths_b)), dirac[2]
+                                else:
+                                    xsi_a, th_a = 1.0, thf[aa]
+                                col_vec[idx] = bcs_fn(x_i, y_j, th_a) * phi_i * psi_j * xsi_a
+                                idx += 1
+            else:
+                for ii in range(nx):
+                    for jj in range(ny):
+                        for aa in range(nth):
+                            col_vec[idx] = bcs_fn(xxf[ii], yyf[jj], thf[aa])
+                            idx += 1
+        o = offs[(ck, kks[0])]
+        out[o:o + col_ndof] = col_vec
     return out
 
 

@@ -1,5 +1,6 @@
 """rt.Problem: the reference's operator/solve interface on the device path."""
 import time
+import warnings
 from typing import Callable
 
 import numpy as np
@@ -16,6 +17,24 @@ from rtx import krylov
 _PRECISIONS = {"fp64": PREC_F64, "fp32": PREC_F32, "mixed": PREC_MIXED}
 
 
+class KaczmarzSubstituted(UserWarning):
+    """``pc_type="kaczmarz"`` was run as per-cell block-Jacobi."""
+
+
+def mesh_signature(mesh) -> tuple:
+    """Leaf columns and cells of ``mesh``: (key, position, orders) of each,
+    in key order, with the domain size and periodicity."""
+    cols = []
+    for ck in sorted(mesh.cols):
+        col = mesh.cols[ck]
+        if not col.is_lf:
+            continue
+        cells = tuple((kk, cell.ndofs[0], cell.pos[0], cell.pos[1])
+                      for kk, cell in sorted(col.cells.items()) if cell.is_lf)
+        cols.append((ck, col.ndofs[0], col.ndofs[1], tuple(col.pos), cells))
+    return (tuple(mesh.Ls), tuple(mesh.pbcs), tuple(cols))
+
+
 class Problem:
     def __init__(self, kappa: Callable, sigma: Callable, Phi: Callable, f: Callable,
                  bcs_dirac: list):
@@ -28,12 +47,17 @@ class Problem:
 
     # ------------------------------------------------------------ operator
     def operator(self, mesh, device=None) -> TransportOperator:
-        """Matrix-free device operator for ``mesh`` (cached for the last mesh
-        object; a refined mesh is a new object and gets a new plan)."""
-        if self._cache is not None and self._cache[0] is mesh:
-            return self._cache[1]
+        """Matrix-free device operator for ``mesh``.  The last plan is reused
+        while the mesh is the same object with the same leaf columns and
+        cells (keys, positions and orders); a mesh refined in place
+        (``ref_mesh``, ``ref_col``, ``ref_cell``) or with changed orders gets
+        a new plan, as the reference rebuilds every matrix per call."""
+        sig = mesh_signature(mesh)
+        if self._cache is not None and self._cache[0] is mesh and self._cache[1] == sig:
+            return self._cache[2]
+        self._cache = None
         op = TransportOperator(mesh, self.kappa, self.sigma, self.Phi, device=device)
-        self._cache = (mesh, op)
+        self._cache = (mesh, sig, op)
         return op
 
     # ------------------------------------------------------------- vectors
@@ -49,10 +73,11 @@ class Problem:
         """Nodal boundary data at every DOF
         (``src/rt/boundary_conditions_vector.py:101-207``)."""
         bcs, dirac = self.bcs_dirac
-        op = self.operator(mesh)
-        if dirac is None or not any(d is not None for d in dirac):
-            return op.nodal(bcs)
-        return _dirac_bcs_vector(mesh, bcs, dirac)
+        # the reference's trigger is ``any(dirac)`` (boundary_conditions_vector.py:142):
+        # a delta at coordinate 0.0 alone takes the nodal branch
+        if dirac is None or not any(dirac):
+            return self.operator(mesh).nodal(bcs)
+        return dirac_bcs_vector(mesh, bcs, dirac)
 
     # ------------------------------------------------------------ matrices
     def mass_matrix(self, mesh, **kwargs):
@@ -112,6 +137,12 @@ class Problem:
         pc_used = {"bjacobi": "bjacobi", "kaczmarz": "bjacobi", "none": "none"}.get(pc_type)
         if pc_used is None:
             raise ValueError("unsupported pc_type {!r}".format(pc_type))
+        if pc_type != pc_used:
+            # PETSc's Kaczmarz sweep is sequential over rows; the device path
+            # runs its per-cell block-Jacobi in its place
+            warnings.warn("pc_type {!r} runs as per-cell block-Jacobi on the device "
+                          "(convergence_info['pc_type_used'])".format(pc_type),
+                          KaczmarzSubstituted, stacklevel=2)
         prec = _PRECISIONS[kw["precision"]]
         t_setup = time.perf_counter()
         pc = (krylov.BlockJacobi(op, keep_f64=prec == PREC_F64, keep_f32=prec == PREC_F32,
@@ -120,6 +151,7 @@ class Problem:
         t_pc = time.perf_counter()
         # the refinement's FP64 system only forms residuals: no preconditioner
         hi = krylov.InteriorSystem(op, PREC_F64, pc if prec == PREC_F64 else None, PREC_F64)
+        pc_prec = PREC_F64
         if prec == PREC_F64:
             x, info = krylov.gmres(hi, rhs, x0, rtol=kw["rtol"], atol=kw["atol"],
                                    restart=kw["restart"], max_it=kw["max_it"],
@@ -129,7 +161,7 @@ class Problem:
             lo = krylov.InteriorSystem(op, prec, pc, pc_prec)
             x, info = krylov.refine(hi, lo, rhs, x0, rtol=max(kw["rtol"], 1e-14),
                                     inner_rtol=kw["inner_rtol"], restart=kw["restart"],
-                                    max_inner=kw["max_it"])
+                                    max_inner=kw["max_it"], max_total=kw["max_it"])
         u = (x + b_bdry).cpu().numpy()
         t_end = time.perf_counter()
         uh = Projection(mesh)
@@ -144,6 +176,7 @@ class Problem:
             "rtol": kw["rtol"], "atol": kw["atol"],
             "divtol": float(np.sqrt(consts.INF)), "max_it": kw["max_it"],
             "pc_type_used": pc_used, "precision": kw["precision"],
+            "attainable_floor": bool(info.get("attainable_floor", False)),
             "seconds_total": t_end - t_start, "seconds_pc_setup": t_pc - t_setup,
             "seconds_krylov": info["seconds"],
         }
@@ -151,7 +184,7 @@ class Problem:
             "ndof": int(op.ndof), "ndof_interior": int(op.mask(torch.ones(op.ndof, dtype=torch.float64, device=dev)).sum().item()),
             "block_size": 1,
             "extreme_singular_values": info.get("extreme_singular_values", [float("nan")] * 2),
-            "pc_bytes": int(pc.bytes_per_apply(PREC_F64 if prec == PREC_F64 else PREC_F32))
+            "pc_bytes": int(pc.bytes_per_apply(pc_prec if prec != PREC_F64 else PREC_F64))
             if pc is not None else 0,
         }
         if kw["verbose"]:
@@ -162,53 +195,62 @@ class Problem:
         return [uh, convergence_info, matrix_info]
 
 
-def _dirac_bcs_vector(mesh, bcs, dirac):
-    """Dirac-delta boundary data (``boundary_conditions_vector.py:142-196``),
-    including its indexing behaviour: a cell none of whose coordinates hosts
-    a delta leaves its entries unwritten and does not advance the index."""
+def _dirac_factors(nodes_ref, nodes_phys, lo, hi, delta):
+    """Per-node factor and coordinate along one direction of a cell: with a
+    delta inside [lo, hi], max(0, l_i(xi_delta)) and the delta's coordinate;
+    otherwise 1 and the node's own coordinate.  Returns (inside, factor,
+    coordinate); a direction without a delta counts as inside, as in the
+    reference."""
+    n = len(nodes_ref)
+    if delta is None or delta < lo or hi < delta:
+        return delta is None, np.ones(n), np.asarray(nodes_phys, dtype=float)
+    xi = float(pull_back(lo, hi, delta))
+    phi = np.array([max(0.0, lag_eval(nodes_ref, i, xi)) for i in range(n)])
+    return True, phi, np.full(n, float(delta))
+
+
+def dirac_bcs_vector(mesh, bcs, dirac):
+    """Boundary data with Dirac deltas (``boundary_conditions_vector.py:142-196``).
+
+    For each direction whose delta coordinate lies in the cell's interval the
+    nodal value is replaced by bcs at the delta's coordinate times
+    max(0, l_i(xi_delta)); other directions keep their nodes.  A cell in which
+    every given delta coordinate lies outside the cell (and all three are
+    given) is skipped without advancing the column's write position, as in
+    the reference, so the column's later cells land earlier and its last
+    entries stay zero.  Vectorised per cell: one bcs call on the cell's
+    broadcast (x, y, theta) node arrays."""
     ndof = mesh.get_ndof()
     out = np.zeros(ndof)
-    idx_g = 0
+    col_start = 0
     for ck in sorted(mesh.cols.keys()):
         col = mesh.cols[ck]
+        if not col.is_lf:
+            continue
         x0, y0, x1, y1 = col.pos
         nx, ny = col.ndofs
         xb, _, yb, _, _, _ = quad_xyth(nnodes_x=nx, nnodes_y=ny)
-        xf, yf = push_forward(x0, x1, xb), push_forward(y0, y1, yb)
-        col_start = idx_g
-        idx = 0
+        in_x, phi, xs = _dirac_factors(xb, push_forward(x0, x1, xb), x0, x1, dirac[0])
+        in_y, psi, ys = _dirac_factors(yb, push_forward(y0, y1, yb), y0, y1, dirac[1])
+        w = col_start
+        col_ndof = 0
         for kk in sorted(col.cells.keys()):
             cell = col.cells[kk]
+            if not cell.is_lf:
+                continue
             th0, th1 = cell.pos
             nth = cell.ndofs[0]
+            col_ndof += nx * ny * nth
             thb = quad_xyth(nnodes_th=nth)[4]
-            thf = push_forward(th0, th1, thb)
-            in_cell = [True, True, True]
-            pb = [None, None, None]
-            for d, (lo, hi) in enumerate(((x0, x1), (y0, y1), (th0, th1))):
-                if dirac[d] is not None:
-                    if dirac[d] < lo or hi < dirac[d]:
-                        in_cell[d] = False
-                    else:
-                        pb[d] = pull_back(lo, hi, dirac[d])
-            if any(in_cell):
-                for i in range(nx):
-                    if dirac[0] is not None and in_cell[0]:
-                        phi, xi = max(0.0, lag_eval(xb, i, pb[0])), dirac[0]
-                    else:
-                        phi, xi = 1.0, xf[i]
-                    for j in range(ny):
-                        if dirac[1] is not None and in_cell[1]:
-                            psi, yj = max(0.0, lag_eval(yb, j, pb[1])), dirac[1]
-                        else:
-                            psi, yj = 1.0, yf[j]
-                        for a in range(nth):
-                            if dirac[2] is not None and in_cell[2]:
-                                xsi, ta = max(0.0, lag_eval(thb, a, pb[2])), dirac[2]
-                            else:
-                                xsi, ta = 1.0, thf[a]
-                            out[col_start + idx] = bcs(xi, yj, ta) * phi * psi * xsi
-                            idx += 1
-        nth_tot = sum(col.cells[k].ndofs[0] for k in col.cells)
-        idx_g = col_start + nx * ny * nth_tot
+            in_t, xsi, ts = _dirac_factors(thb, push_forward(th0, th1, thb), th0, th1, dirac[2])
+            if not (in_x or in_y or in_t):
+                continue
+            X, Y, T = np.meshgrid(xs, ys, ts, indexing="ij")
+            vals = np.broadcast_to(np.asarray(bcs(X, Y, T), dtype=float), X.shape)
+            # products in the reference's order: ((bcs phi) psi) xsi
+            blk = (((vals * phi[:, None, None]) * psi[None, :, None])
+                   * xsi[None, None, :]).ravel()
+            out[w:w + blk.size] = blk
+            w += blk.size
+        col_start += col_ndof
     return out

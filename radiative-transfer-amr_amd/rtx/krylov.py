@@ -6,10 +6,17 @@ choices, not accidents:
 
 * right preconditioning, so the GMRES residual is the true residual of the
   interior system (PETSc's default is left preconditioning);
-* the Krylov basis, the Arnoldi orthogonalisation (classical Gram-Schmidt
-  with one conditional re-orthogonalisation), the solution update and the
-  true residual are FP64 on the device; only the small Hessenberg least
-  squares problem (at most (m + 1) x m) is solved on the host;
+* the Krylov basis, the Arnoldi orthogonalisation (classical Gram-Schmidt,
+  one pass as PETSc's default, optionally re-orthogonalised), the Givens QR
+  of the Hessenberg matrix with its back substitution, the solution update
+  and the true residual all stay on the device (FP64, or an FP32 basis for
+  the inner solves of mixed refinement); the host only reads the residual
+  estimates once per chunk of iterations;
+* convergence is checked on the FP64 true residual at every restart; when
+  the recurrence estimate has met the target but the true residual sits
+  at its attainable floor (it did not halve over the last cycle and is
+  below 1e-10 |b|), the solve stops as converged, as PETSc's
+  estimate-based test does with the reference's rtol = atol = 1e-15;
 * the block-Jacobi blocks are the angular cells (``BlockJacobi``);
 * ``refine`` wraps an inexact inner GMRES that applies the operator and the
   preconditioner in FP32 / BF16 inside an FP64 residual loop.
@@ -27,6 +34,11 @@ import torch
 from . import _lib
 from ._lib import check, PREC_F64, PREC_F32, PREC_MIXED, PREC_FP8, OP_SYSTEM, OP_RESTRICT
 from .operator import stream_handle
+
+
+# A true residual below ATTAINABLE |b| that stops decreasing counts as the
+# rounding floor of the FP64 apply (about eps |A| |x|).
+ATTAINABLE = 1e-10
 
 
 def _vp(t):
@@ -289,6 +301,8 @@ class GMRES:
         target = max(rtol * bnorm, atol)
         history, true_hist = [], []
         its, reason, k_last, stop = 0, 0, 0, None
+        est_met = False          # the last cycle's estimate reached the target
+        floor = False
         t0 = time.perf_counter()
         while True:
             system.residual(b, x, r)
@@ -300,6 +314,12 @@ class GMRES:
                 history.append(beta)
             if beta <= target:
                 reason = 2 if beta <= rtol * bnorm else 3     # KSP_CONVERGED_RTOL / ATOL
+                break
+            if (est_met and len(true_hist) >= 2 and beta > 0.5 * true_hist[-2]
+                    and beta <= ATTAINABLE * bnorm):
+                # rounding floor: the recurrence has converged, the true
+                # residual cannot follow (PETSc stops on the estimate here)
+                reason, floor = 2, True
                 break
             if its >= max_it:
                 reason = -3                                   # KSP_DIVERGED_ITS
@@ -332,6 +352,7 @@ class GMRES:
                         callback(its + i + 1, float(e[i]))
                     if e[i] <= target:
                         stop = i + 1
+                        est_met = True
                         break
                 if stop is not None:
                     its += stop
@@ -340,6 +361,8 @@ class GMRES:
                 its += j1 - j
                 j = k = j1
             k_last = k
+            if stop is None:
+                est_met = False
             if k > 0:
                 check(lib.rtx_gmres_solve(k, m, _vp(self.state), _vp(self.ydev), st),
                       "rtx_gmres_solve")
@@ -377,7 +400,7 @@ class GMRES:
                            bnorm=bnorm,
                            rel_residual=true_hist[-1] / bnorm if bnorm > 0 else 0.0,
                            seconds=elapsed, restart=m, extreme_singular_values=sv,
-                           graphs=len(self.chunks.graphs))
+                           graphs=len(self.chunks.graphs), attainable_floor=floor)
         return x, info
 
 
@@ -402,7 +425,7 @@ def hessenberg_singular_values(H, k):
 
 def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=50,
            max_outer=50, max_inner=200, flexible=False, after_outer=None, chunk=10,
-           graphs=True, inner=None, basis="f32"):
+           graphs=True, inner=None, basis="f32", max_total=None):
     """FP64 iterative refinement: r = b - A x in FP64 (``system_hi``), inner
     correction solve A d = r by GMRES whose operator and preconditioner run in
     reduced precision (``system_lo``) on a ``basis`` "f32" (default) or "f64"
@@ -411,7 +434,8 @@ def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=
     ``flexible`` selects flexible inner GMRES (for a nonlinear
     preconditioner); ``after_outer(outer, |r|)`` sees every FP64 residual;
     ``inner`` may pass a GMRES object on ``system_lo`` to reuse (with its
-    graphs) across calls."""
+    graphs) across calls; ``max_total`` caps the inner iterations summed
+    over all outer steps (reason -3 when it is spent, as PETSc's max_it)."""
     lib = system_hi.lib
     n = b.numel()
     x = torch.zeros_like(b) if x is None else x
@@ -423,7 +447,7 @@ def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=
     bnorm = _norm(lib, b, work, scal)
     hist, inner_its = [], []
     t0 = time.perf_counter()
-    reason = -3
+    reason, floor = -3, False
     if inner is None:
         inner = GMRES(system_lo, n, b.device, restart=restart, flexible=flexible, chunk=chunk,
                       graphs=graphs, basis=basis)
@@ -437,15 +461,26 @@ def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=
         if rn <= rtol * bnorm:
             reason = 2
             break
+        if (len(hist) >= 3 and rn > 0.5 * hist[-2] and hist[-2] > 0.5 * hist[-3]
+                and rn <= ATTAINABLE * bnorm):
+            # two outer steps without halving the FP64 residual: its rounding floor
+            reason, floor = 2, True
+            break
         if outer == max_outer or not np.isfinite(rn):
             reason = -3 if np.isfinite(rn) else -9
             break
+        budget = max_inner
+        if max_total is not None:
+            budget = min(max_inner, max_total - sum(inner_its))
+            if budget <= 0:
+                reason = -3
+                break
         d.zero_()
-        d, info = inner.solve(r, d, rtol=inner_rtol, max_it=max_inner)
+        d, info = inner.solve(r, d, rtol=inner_rtol, max_it=budget)
         inner_its.append(info["iteration_number"])
         check(lib.rtx_axpby(PREC_F64, n, 1.0, _vp(d), 1.0, _vp(x), stream_handle()), "axpby")
     torch.cuda.synchronize()
     return x, dict(converged_reason=reason, residual_history=hist, inner_iterations=inner_its,
                    iteration_number=int(sum(inner_its)), res_f=hist[-1], bnorm=bnorm,
                    rel_residual=hist[-1] / bnorm if bnorm > 0 else 0.0,
-                   seconds=time.perf_counter() - t0)
+                   seconds=time.perf_counter() - t0, attainable_floor=floor)
