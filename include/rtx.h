@@ -259,6 +259,16 @@ int rtx_arnoldi_step_f32(const float *d_V, int64_t ld, int nv, int64_t n, float 
 int rtx_gmres_arnoldi(int basis_precision, const void *d_V, int64_t ld, int64_t n, void *d_w,
                       double *d_hcol, double *d_work, int refine, int j, int m,
                       double *d_state, void *stream);
+/* rtx_gmres_arnoldi without the workspace argument: the same kernels and
+   results, with the reduction partials, norms and re-orthogonalisation flag
+   in a static device array of the library (2.1 MB per device, held until
+   the process ends).  No counterpart in the reference: it is for callers
+   that cannot allocate device memory (code under HIP graph capture, serving
+   loops).  Limit: m < 255 (RTX_ERR_ARG beyond).  Allocates nothing; calls
+   that overlap on one device share the array and race. */
+int rtx_gmres_arnoldi_static(int basis_precision, const void *d_V, int64_t ld, int64_t n,
+                             void *d_w, double *d_hcol, int refine, int j, int m,
+                             double *d_state, void *stream);
 /* Least-squares state of one restart cycle of m steps kept on the device
    (Givens QR of the Hessenberg matrix; PETSc keeps it on the host, here it
    stays on the GPU so whole chunks of iterations can run as one HIP graph).

@@ -7,6 +7,8 @@
 // normalisation) is a fixed sequence of fused FP64 kernels over the tall
 // basis V; reductions are two-phase and deterministic (per-block partial
 // sums, then one reduction per output).
+#include <mutex>
+
 #include "rtx_internal.h"
 
 namespace rtx {
@@ -471,6 +473,56 @@ extern "C" int rtx_gmres_arnoldi(int basis_precision, const void *d_V, int64_t l
     return arnoldi_step<float>(static_cast<const float *>(d_V), ld, j + 1, n,
                                static_cast<float *>(d_w), d_hcol, d_work, refine, stream, j, m,
                                d_state);
+}
+
+/* ---- rtx_gmres_arnoldi without a workspace argument.  Its reduction
+   partials, norms and re-orthogonalisation flag live in a file-scope device
+   array instead, so a caller that cannot allocate (code being captured into
+   a HIP graph, a serving loop) passes only its inputs and outputs.  The
+   array has the workspace layout of rtx_arnoldi_workspace (every step
+   writes each partial before it reads it, so nothing needs clearing
+   between calls).  One copy per device, held until the process ends;
+   calls that overlap on one device share it and race. */
+constexpr int64_t STATIC_WORK_DOUBLES = int64_t(KMAXV + 2) * KNB + 16;   // 2.1 MB
+static __device__ __align__(256) double scratch[STATIC_WORK_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex scratch_mutex;
+static double *scratch_ptr[MAX_DEVICES];     // device address of `scratch`, per device
+
+// Device address of `scratch` on the current device (looked up once per
+// device with that device current); null with the error set on failure.
+static double *static_workspace() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+        set_error("rtx_gmres_arnoldi_static: no usable current device (%d)", dev);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error("rtx_gmres_arnoldi_static: hipGetSymbolAddress failed on device %d: %s",
+                      dev, hipGetErrorString(e));
+            return nullptr;
+        }
+        scratch_ptr[dev] = static_cast<double *>(ptr);
+    }
+    return scratch_ptr[dev];
+}
+
+extern "C" int rtx_gmres_arnoldi_static(int basis_precision, const void *d_V, int64_t ld,
+                                        int64_t n, void *d_w, double *d_hcol, int refine,
+                                        int j, int m, double *d_state, void *stream) {
+    RTX_CHECK(m >= 1 && m < KMAXV - 1 && j >= 0 && j < m,
+              "rtx_gmres_arnoldi_static: cycle length m=%d (step j=%d) outside the static "
+              "workspace's limit m < %d",
+              m, j, KMAXV - 1);
+    double *work = static_workspace();
+    if (!work) return RTX_ERR_HIP;
+    return rtx_gmres_arnoldi(basis_precision, d_V, ld, n, d_w, d_hcol, work, refine, j, m,
+                             d_state, stream);
 }
 
 /* The same step on an FP32 basis (dot products, norms and the Hessenberg

@@ -102,6 +102,9 @@ SIGNATURES = [
     ("rtx_gmres_arnoldi", C.c_int, [C.c_int, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                     C.c_void_p, C.c_void_p]),
+    ("rtx_gmres_arnoldi_static", C.c_int, [C.c_int, C.c_void_p, C.c_int64, C.c_int64,
+                                           C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                           C.c_void_p, C.c_void_p]),
     ("rtx_gmres_solve", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     # learned coarse correction
     ("rtx_coarse_setup", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
