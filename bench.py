@@ -23,6 +23,7 @@ C2's coefficients (16 x 16 x 8 cells, 256,000 DOF) to 1e-8 on the host CPU
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve]
                        [--no-cpu] [--no-learned] [--no-indicators]
                        [--learned-fields N] [--learned-kappa-scales S,...] [--calib]
+                       [--profile-apply {c2-f64,c2-mixed,c3-f64,c3-mixed}]
 """
 import argparse
 import json
@@ -54,7 +55,83 @@ def parse():
     ap.add_argument("--calib", action="store_true",
                     help="run one plain 1 GiB device copy after the timed region "
                          "(counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--path", type=int, default=-1,
+                    help="apply path for --profile-apply (-1 automatic, 0 generic, "
+                         "1 fused parts, 3 hp column kernel)")
+    ap.add_argument("--profile-apply", choices=sorted(PROFILE_WORKLOADS),
+                    help="only time the interior operator apply of one workload and "
+                         "precision (for rocprofv3 counter runs; prints its own JSON line)")
     return ap.parse_args()
+
+
+# --profile-apply workloads: (mesh, coefficients, precision)
+PROFILE_WORKLOADS = {
+    "c2-f64": ("c2", "f64"), "c2-mixed": ("c2", "mixed"),
+    "c3-f64": ("c3", "f64"), "c3-mixed": ("c3", "mixed"),
+}
+
+
+def profile_apply(args):
+    """Interior operator applies only (``--profile-apply``): C2 (uniform
+    absorber, 32 x 32 x 16 cells, p = 4) or C3 (batched uneven-order hp
+    mesh of 8,263,762 DOF with two strong scatterers), FP64 or mixed (FP32
+    convection, BF16 scattering); ``warmup`` untimed and ``steps`` timed
+    launches on random input, HIP events on the launch stream; with
+    ``--calib`` the two 1 GiB calibration copies follow."""
+    import torch
+    from rtx import problems
+    from rtx.operator import TransportOperator, OP_SYSTEM, OP_RESTRICT, PREC_F64, PREC_MIXED
+    mesh_name, prec_name = PROFILE_WORKLOADS[args.profile_apply]
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    if mesh_name == "c2":
+        mesh, wl = workload()
+    else:
+        mesh, wl = problems.random_hp_mesh(0), problems.two_scatterers_c3()
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi, device=dev)
+    op.set_path(args.path)
+    prec = PREC_F64 if prec_name == "f64" else PREC_MIXED
+    dt = torch.float64 if prec == PREC_F64 else torch.float32
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    x = (torch.rand(op.ndof, dtype=torch.float64, device=dev, generator=gen) * 2 - 1).to(dt)
+    op.mask(x)
+    y = torch.empty_like(x)
+    ops = OP_SYSTEM | OP_RESTRICT
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        op.apply(x, y, ops=ops, precision=prec)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        op.apply(x, y, ops=ops, precision=prec)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    us = ev0.elapsed_time(ev1) / max(args.steps, 1) * 1e3
+    if args.calib:
+        _calibration_copies(op, dev)
+    bytes_, flops = algorithmic(op)
+    print(json.dumps({"profile_apply": args.profile_apply, "path": args.path, "ndof": op.ndof,
+                      "us_per_apply": us, "gdof_per_s": op.ndof / us * 1e-3,
+                      "algorithmic_bytes_f64": bytes_, "algorithmic_flops": flops,
+                      "steps": args.steps, "warmup": args.warmup}), flush=True)
+
+
+def _calibration_copies(op, dev):
+    """Two copies of 1 GiB for the traffic counters: the runtime's blit, then
+    rtx_copy_f64 (plain per-lane FP64 loads and stores, the access type of
+    the operator kernels)."""
+    import ctypes as C
+    import torch
+    from rtx import _lib
+    from rtx.operator import stream_handle
+    big = torch.empty(1 << 27, dtype=torch.float64, device=dev)   # 1 GiB
+    big2 = torch.empty_like(big)
+    big2.copy_(big)
+    _lib.check(op.lib.rtx_copy_f64(C.c_void_p(big.data_ptr()), C.c_void_p(big2.data_ptr()),
+                                   big.numel(), stream_handle()), "rtx_copy_f64")
+    torch.cuda.synchronize()
 
 
 def workload():
@@ -258,6 +335,9 @@ def main():
         print("bench.py runs on one GPU only: the operator and solver have no multi-GPU "
               "path (--gpus {} requested)".format(args.gpus), file=sys.stderr)
         sys.exit(2)
+    if args.profile_apply:
+        profile_apply(args)
+        return
     import numpy as np
     import torch
     from rtx.operator import TransportOperator, OP_SYSTEM, OP_RESTRICT, PREC_F64, PREC_F32, PREC_MIXED
@@ -295,18 +375,7 @@ def main():
     # the binding resource: the larger of bytes / HBM peak and flops / FP64 peak
     fp64_bound = flops / (FP64_PEAK_TFLOPS * 1e12) > bytes_ / (HBM_PEAK_GBS * 1e9)
     if args.calib:
-        # two copies of 1 GiB for the traffic counters: the runtime's blit,
-        # then rtx_copy_f64 (plain per-lane FP64 loads and stores, the access
-        # type of the operator kernels)
-        import ctypes as C
-        from rtx import _lib
-        from rtx.operator import stream_handle
-        big = torch.empty(1 << 27, dtype=torch.float64, device=dev)   # 1 GiB
-        big2 = torch.empty_like(big)
-        big2.copy_(big)
-        _lib.check(op.lib.rtx_copy_f64(C.c_void_p(big.data_ptr()), C.c_void_p(big2.data_ptr()),
-                                       big.numel(), stream_handle()), "rtx_copy_f64")
-        torch.cuda.synchronize()
+        _calibration_copies(op, dev)
 
     out = {
         "metric": "time-to-1e-8 residual & DOF/s per apply: learned+mixed vs plain FP64, 1 GPU",

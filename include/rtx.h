@@ -145,11 +145,13 @@ int rtx_plan_set_coeffs(rtx_plan *plan, int64_t nsp, const double *kappa,
 int rtx_apply(rtx_plan *plan, uint32_t op_mask, int precision, const void *d_x,
               void *d_y, void *stream);
 
-/* Apply path: -1 automatic (default: the order-binned kernel when the whole
-   mesh is one compiled bin, else the fused per-column kernel whenever every
-   column fits it, else the generic per-cell/per-column kernels), 0 generic,
-   1 fused, 2 order-binned (rtx_apply then fails if the mesh does not fit the
-   forced path). */
+/* Apply path: -1 automatic (the uniform-bin kernel when the whole mesh is
+   one compiled bin, else the hp column kernel on the columns it takes and the
+   generic kernels on the rest, unless rtx_autotune found the order bins or
+   the generic kernels faster), 0 generic per-cell / per-column kernels,
+   1 fused per-part kernel, 2 uniform-bin kernel, 3 hp column kernel (one
+   workgroup per column; quadrature weights not supported) - rtx_apply fails
+   if the mesh does not fit a forced path. */
 int rtx_set_path(rtx_plan *plan, int mode);
 
 /* Autotune the apply's launch shapes on this plan: for the uniform-bin kernel
@@ -169,9 +171,11 @@ int rtx_set_uniform_parts(rtx_plan *plan, int parts);
    on the fused kernel (0); rebuilds the generic lists (synchronises the
    device).  No counterpart in the reference. */
 int rtx_set_bin_generic(rtx_plan *plan, int bin, int generic);
-/* Launch-shape summary (22 values): uniform bin usable, its parts per column;
+/* Launch-shape summary (24 values): uniform bin usable, its parts per column;
    parts, columns and generic flag of the six fused order bins (part count by
-   LDS class); columns and cells on the generic kernels.  Returns the count. */
+   LDS class); columns and cells on the generic kernels; whether the hp
+   column kernel is the default path, and the columns it takes.  Returns the
+   count. */
 int rtx_plan_info(const rtx_plan *plan, int64_t *out, int n);
 
 /* In-place mask of a full-length vector: keep = 0 zeroes the inflow-boundary

@@ -406,10 +406,11 @@ static bool any_fused(const rtx_plan *p, bool all_bins) {
 
 using namespace rtx;
 
-/* Select the apply path: -1 automatic (fused when every column fits), 0 the
-   generic per-cell + per-column kernels, 1 the fused per-column kernel. */
+/* Select the apply path: -1 automatic, 0 the generic per-cell + per-column
+   kernels, 1 the fused per-part kernel, 2 the uniform-bin kernel, 3 the hp
+   column kernel. */
 extern "C" int rtx_set_path(rtx_plan *p, int mode) {
-    RTX_CHECK(p && mode >= -1 && mode <= 2, "rtx_set_path: bad argument");
+    RTX_CHECK(p && mode >= -1 && mode <= 3, "rtx_set_path: bad argument");
     p->fused_mode = mode;
     return RTX_OK;
 }
@@ -434,6 +435,22 @@ extern "C" int rtx_apply(rtx_plan *p, uint32_t op_mask, int precision, const voi
     }
     RTX_CHECK(p->fused_mode != 2, "rtx_apply: the uniform-bin path was requested but the mesh "
                                   "is not a compiled uniform bin");
+    if (p->fused_mode == 3 || (p->fused_mode == -1 && p->hp_ok && p->hp_default)) {
+        // hp column kernel (mass, scattering, both convections, restriction)
+        RTX_CHECK(p->hp_ok, "rtx_apply: the hp column path was requested but no column fits it");
+        RTX_CHECK(!(op_mask & RTX_OP_QUADW) || p->fused_mode != 3,
+                  "rtx_apply: the hp column path does not apply quadrature weights");
+        if (!(op_mask & RTX_OP_QUADW)) {
+            int rc = launch_hpcol(p, op_mask, precision, d_x, d_y, st, acc);
+            if (rc || p->hp_over_cols.empty()) return rc;
+            if (precision == RTX_PREC_F64)
+                return launch_apply<double>(p, op_mask, d_x, d_y, st, p->hp_gen_cells,
+                                            p->hp_n_gen_cells, p->hp_gen_items,
+                                            p->hp_n_gen_items);
+            return launch_apply<float>(p, op_mask, d_x, d_y, st, p->hp_gen_cells,
+                                       p->hp_n_gen_cells, p->hp_gen_items, p->hp_n_gen_items);
+        }
+    }
     const bool forced = p->fused_mode == 1;
     RTX_CHECK(!forced || p->h_over_cols.empty(),
               "rtx_apply: the fused per-column path was requested but %d columns do not fit "
@@ -530,6 +547,9 @@ extern "C" int rtx_autotune(rtx_plan *p, int precision, const void *d_x, void *d
         }
         p->uni_H = bestH;
     }
+    // the order bins and generic kernels are tuned with the hp column kernel
+    // off; it is compared against their best afterwards
+    p->hp_default = false;
     bool changed = false;
     for (int bi = 0; bi < RTX_NBINS && !p->uni_ok; ++bi) {   // a uniform mesh uses no bin
         FusedBin &fb = p->fbin[bi];
@@ -581,6 +601,20 @@ extern "C" int rtx_autotune(rtx_plan *p, int precision, const void *d_x, void *d
         }
         p->fused_mode = mode;
     }
+    // the hp column kernel against the bins and generic kernels chosen above
+    if (!p->uni_ok && p->hp_ok) {
+        auto whole = [&]() { return rtx_apply(p, ops, precision, d_x, d_y, stream); };
+        const int mode = p->fused_mode;
+        p->fused_mode = -1;
+        p->hp_default = true;
+        const float t_hp = time_launches(whole, reps, st, &rc);
+        if (rc) return rc;
+        p->hp_default = false;
+        const float t_rest = time_launches(whole, reps, st, &rc);
+        if (rc) return rc;
+        p->hp_default = t_hp <= t_rest;
+        p->fused_mode = mode;
+    }
     return RTX_OK;
 }
 
@@ -605,10 +639,11 @@ extern "C" int rtx_set_uniform_parts(rtx_plan *p, int H) {
 
 /* Launch-shape summary: out[0] uniform bin usable, [1] its parts per column,
    [2 + 3 b .. 4 + 3 b] parts, columns and generic flag of fused bin b
-   (b < 6), [20], [21] columns and cells on the generic kernels.  Returns the
-   count (22). */
+   (b < 6), [20], [21] columns and cells on the generic kernels, [22] the hp
+   column kernel is the default path, [23] columns it takes.  Returns the
+   count (24). */
 extern "C" int rtx_plan_info(const rtx_plan *p, int64_t *out, int n) {
-    constexpr int NV = 4 + 3 * RTX_NBINS;
+    constexpr int NV = 6 + 3 * RTX_NBINS;
     if (!p) return 0;
     int64_t v[NV] = {};
     v[0] = p->uni_ok ? 1 : 0;
@@ -618,8 +653,12 @@ extern "C" int rtx_plan_info(const rtx_plan *p, int64_t *out, int n) {
         v[3 + 3 * b] = p->fbin[b].n;
         v[4 + 3 * b] = p->fbin[b].generic ? 1 : 0;
     }
-    v[NV - 2] = p->n_gen_cols;
-    v[NV - 1] = p->n_gen_cells;
+    v[20] = p->n_gen_cols;
+    v[21] = p->n_gen_cells;
+    v[22] = p->hp_default ? 1 : 0;
+    int hp_cols = 0;
+    for (int k = 0; k < HP_NCLS; ++k) hp_cols += p->hp_cls[k].n;
+    v[23] = hp_cols;
     for (int i = 0; i < NV && i < n && out; ++i) out[i] = v[i];
     return NV;
 }

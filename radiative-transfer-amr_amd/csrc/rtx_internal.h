@@ -79,6 +79,24 @@ int fused_column_ok(int ncells, int nsp, int nang, int part_links, size_t bytes6
 size_t fused_bytes(int64_t col_dofs, int64_t nang, int64_t part_dofs, int64_t part_trace,
                    size_t a);
 
+// Column kernel of uneven-order hp meshes (one workgroup per column):
+// column-local offsets of each link's angle-contracted trace block and
+// the column totals.
+constexpr int HP_MAX_CELLS = 64;    // cells per column on this path
+constexpr int HP_MAX_LINKS = 384;   // upwind links per column on this path
+constexpr int HP_NCLS = 2;          // LDS classes (one launch each)
+struct HpTables {
+    const int32_t *link_inn = nullptr, *col_inn = nullptr;
+};
+struct HpClass {
+    int n = 0;                      // columns
+    const int32_t *cols = nullptr;  // device list (key order)
+    size_t bytes[3] = {0, 0, 0};    // dynamic LDS per workgroup: FP64, FP32, mixed
+};
+int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d);
+int launch_hpcol(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
+                 int accumulate);
+
 constexpr int RTX_NBINS = 6;
 // Tiled scattering work items: 16 spatial nodes x SC_RG output angles
 constexpr int SC_RG = 128;
@@ -136,6 +154,16 @@ struct rtx_plan {
     const int32_t *sc_all = nullptr, *sc_gen = nullptr;
     int n_sc_all = 0, n_sc_gen = 0;
     void *gen_alloc = nullptr;   // device lists of the generic set
+    // column kernel of hp meshes: per-link tables, LDS classes, and the
+    // columns it cannot take (run on the generic kernels, lists in hp_alloc)
+    bool hp_ok = false;
+    bool hp_default = false;     // mode -1 uses the column kernel
+    rtx::HpTables hp;
+    rtx::HpClass hp_cls[rtx::HP_NCLS];
+    std::vector<int32_t> hp_over_cols;
+    const int32_t *hp_gen_cells = nullptr, *hp_gen_items = nullptr;
+    int hp_n_gen_cells = 0, hp_n_gen_items = 0;
+    void *hp_alloc = nullptr;
     // host copies: per-column cell start (ncol + 1), spatial nodes, angles;
     // fused-bin columns and the columns no bin takes
     std::vector<int32_t> h_col_cell_start, h_col_nsp, h_col_nang;

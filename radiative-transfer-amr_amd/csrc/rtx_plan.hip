@@ -437,6 +437,11 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
         const int rc = rebuild_generic(p);
         if (rc) return rc;
     }
+    // the hp column kernel's tables and LDS classes
+    {
+        const int rc = hpcol_build(p, d);
+        if (rc) return rc;
+    }
     // sizes of the column G blocks are checked in set_coeffs
     *out = p;
     return RTX_OK;
@@ -528,6 +533,7 @@ extern "C" void rtx_plan_destroy(rtx_plan *p) {
     if (p->coarse) (void)hipFree(p->coarse);
     if (p->bins) (void)hipFree(p->bins);
     if (p->gen_alloc) (void)hipFree(p->gen_alloc);
+    if (p->hp_alloc) (void)hipFree(p->hp_alloc);
     (void)hipSetDevice(cur);
     delete p;
 }

@@ -127,7 +127,8 @@ class TransportOperator:
         return y
 
     def set_path(self, mode):
-        """-1 automatic, 0 generic per-cell/per-column kernels, 1 fused per-column kernel."""
+        """-1 automatic, 0 generic per-cell/per-column kernels, 1 fused per-part
+        kernel, 2 uniform-bin kernel, 3 hp column kernel."""
         check(self.lib.rtx_set_path(self.plan, int(mode)), "rtx_set_path")
 
     def autotune(self, precision=PREC_F64, reps=10, stream=None):
@@ -146,14 +147,15 @@ class TransportOperator:
         """Launch shapes of the apply: uniform bin (usable, parts per column),
         the fused order bins (parts, columns; by LDS class) and the column and cell
         counts left to the generic kernels."""
-        out = (C.c_int64 * 22)()
-        n = self.lib.rtx_plan_info(self.plan, out, 22)
+        out = (C.c_int64 * 24)()
+        n = self.lib.rtx_plan_info(self.plan, out, 24)
         v = list(out)[:n]
-        bins = [tuple(v[2 + 3 * b: 5 + 3 * b]) for b in range((n - 4) // 3)]
+        bins = [tuple(v[2 + 3 * b: 5 + 3 * b]) for b in range(6)]
         return {"uniform": bool(v[0]), "uniform_parts": v[1],
                 "fused_bins": [(H, nc) for H, nc, g in bins if nc and not g],
                 "bins_on_generic": [nc for H, nc, g in bins if nc and g],
-                "generic_cols": v[n - 2], "generic_cells": v[n - 1]}
+                "generic_cols": v[20], "generic_cells": v[21],
+                "column_kernel": bool(v[22]), "column_kernel_cols": v[23]}
 
     def set_bin_generic(self, b, generic=True):
         """Run fused order bin ``b`` on the generic kernels (or back)."""

@@ -36,7 +36,7 @@ def _dev(x):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("path", [0, 1, 2], ids=["generic", "fused", "uniform"])
+@pytest.mark.parametrize("path", [0, 1, 2, 3], ids=["generic", "fused", "uniform", "column"])
 @pytest.mark.parametrize("case", ["hp", "c1"])
 def test_apply_parts_match_oracle(cuda, case, path):
     from rtx.operator import (TransportOperator, OP_MASS, OP_SCAT, OP_ICONV, OP_BCONV,
@@ -71,7 +71,7 @@ TOLMIX = 5e-3     # relative 2-norm, BF16 scattering operands with FP32 accumula
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("path", [0, 1, 2], ids=["generic", "fused", "uniform"])
+@pytest.mark.parametrize("path", [0, 1, 2, 3], ids=["generic", "fused", "uniform", "column"])
 def test_apply_accumulate_and_reduced_precision(cuda, path):
     import torch
     from rtx.operator import (TransportOperator, OP_SYSTEM, OP_ACCUM, OP_SCAT, OP_RESTRICT,
@@ -140,7 +140,7 @@ def small_random_hp(seed):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("path", [0, 1], ids=["generic", "fused"])
+@pytest.mark.parametrize("path", [0, 1, 3], ids=["generic", "fused", "column"])
 @pytest.mark.parametrize("seed", [1, 2])
 def test_apply_random_hp_matches_oracle(cuda, seed, path):
     """Uneven orders on both sides of every face and level jumps in space and
@@ -167,7 +167,8 @@ def test_apply_random_hp_matches_oracle(cuda, seed, path):
 
 
 @pytest.mark.gpu
-def test_apply_c3_rows_and_linearity(cuda):
+@pytest.mark.parametrize("path", [-1, 3], ids=["auto", "column"])
+def test_apply_c3_rows_and_linearity(cuda, path):
     """Full-size batched uneven-order hp mesh (16 x 16 x 8 base, random
     refinement, node counts 3..9: 8,263,762 DOF): rows of sampled columns
     (the largest column among them) against the oracle, linearity over the
@@ -177,6 +178,8 @@ def test_apply_c3_rows_and_linearity(cuda):
     mesh = problems.random_hp_mesh(0)
     wl = problems.two_scatterers_c3()
     op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    op.set_path(path)
+    print("c3 path", path, flush=True)
     assert op.ndof == 8_263_762
     rng = np.random.default_rng(0)
     x = rng.uniform(-1, 1, op.ndof)
@@ -246,20 +249,46 @@ def test_uniform_apply_bit_reproducible(cuda, n, nref_ang, nref_spt):
 
 @pytest.mark.gpu
 def test_hp_default_apply_bit_reproducible(cuda):
-    """On an hp mesh the default launch shapes (generic kernels: one
-    wavefront per cell, tiled scattering GEMM, no atomics) give bit-identical
-    repeated applies, in FP64 and mixed precision."""
+    """On an hp mesh the default launch shapes (the hp column kernel: one
+    workgroup per column, every output written by one lane; the generic
+    kernels for columns it does not take) give bit-identical repeated
+    applies, in FP64 and mixed precision."""
     import torch
     from rtx.operator import TransportOperator, OP_SYSTEM, OP_RESTRICT, PREC_MIXED
     mesh = small_random_hp(2)
     wl = problems.two_scatterers_c3()
     op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
-    assert op.plan_info()["fused_bins"] == []
+    info = op.plan_info()
+    assert info["fused_bins"] == [] and info["column_kernel"]
+    assert info["column_kernel_cols"] == op.t.ncol
     x = _dev(np.random.default_rng(11).uniform(-1.0, 1.0, op.ndof))
     for xx, prec in ((x, 0), (x.float(), PREC_MIXED)):
         ys = [op.apply(xx, ops=OP_SYSTEM | OP_RESTRICT, precision=prec).clone() for _ in range(3)]
         for y in ys[1:]:
             assert torch.equal(y, ys[0]), prec
+
+
+@pytest.mark.gpu
+def test_column_kernel_bit_reproducible(cuda):
+    """The hp column kernel adds every output in a fixed order (one owner per
+    output, no atomics): repeated FP64, FP32 and mixed applies are
+    bit-identical, and accumulate mode adds to y exactly."""
+    import torch
+    from rtx.operator import (TransportOperator, OP_SYSTEM, OP_RESTRICT, OP_ACCUM, PREC_F32,
+                              PREC_MIXED)
+    mesh = small_random_hp(2)
+    wl = problems.two_scatterers_c3()
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    op.set_path(3)
+    x = _dev(np.random.default_rng(12).uniform(-1.0, 1.0, op.ndof))
+    for xx, prec in ((x, 0), (x.float(), PREC_F32), (x.float(), PREC_MIXED)):
+        ys = [op.apply(xx, ops=OP_SYSTEM | OP_RESTRICT, precision=prec).clone() for _ in range(3)]
+        for y in ys[1:]:
+            assert torch.equal(y, ys[0]), prec
+    y0 = torch.rand_like(x)
+    y1 = y0.clone()
+    op.apply(x, y1, ops=OP_SYSTEM | OP_ACCUM)
+    assert torch.allclose(y1 - y0, op.apply(x, ops=OP_SYSTEM), rtol=1e-13, atol=1e-13)
 
 
 @pytest.mark.gpu
@@ -310,7 +339,7 @@ def test_uniform_parts_and_autotune(cuda):
     x = np.random.default_rng(4).uniform(-1.0, 1.0, op.ndof)
     mask = asm.intr_mask(mesh)
     ref = mask * (A @ (mask * x))
-    for path in (-1, 0, 1):   # tuned, all generic, all fused
+    for path in (-1, 0, 1, 3):   # tuned, all generic, all fused, column kernel
         op.set_path(path)
         y = op.apply(_dev(x), ops=OP_SYSTEM | OP_RESTRICT).cpu().numpy()
         assert _rel(y, ref) < TOL64, (path, _rel(y, ref))
@@ -326,7 +355,7 @@ def test_uniform_parts_and_autotune(cuda):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("pbcs", [(True, False), (True, True)], ids=["periodic-x", "periodic-xy"])
-@pytest.mark.parametrize("path", [-1, 0], ids=["auto", "generic"])
+@pytest.mark.parametrize("path", [-1, 0, 3], ids=["auto", "generic", "column"])
 def test_apply_periodic_matches_oracle(cuda, pbcs, path):
     """Periodic meshes (``Mesh(pbcs=...)``, class_Mesh.py:46-53): columns on
     opposite walls are face neighbours and no wall in a periodic direction
@@ -334,7 +363,7 @@ def test_apply_periodic_matches_oracle(cuda, pbcs, path):
     from rtx.operator import (TransportOperator, OP_MASS, OP_SCAT, OP_ICONV, OP_BCONV,
                               OP_SYSTEM, OP_RESTRICT)
     mesh = problems.uniform_mesh([1.0, 1.0], [4, 4, 4], 2, 2, pbcs=pbcs)
-    if path == 0:
+    if path in (0, 3):
         keys = sorted(mesh.cols)
         mesh.ref_col(keys[0], "spt", "p")
     wl = problems.manufactured()

@@ -1,0 +1,275 @@
+"""CPU model of the hp column kernel (``csrc/rtx_hpcol.hip``), test-only.
+
+Restates the kernel's data flow with its own index arithmetic: the
+node-major LDS copy of a column with the padded row stride, the per-link
+angle-contracted traces at the column-local offsets the host builds, the
+angle -> cell table, the matrix-core operand reads (K steps of 4 for FP64 /
+FP32, 16 for BF16), the accumulator rows, the face-node mapping of each
+upwind link and the output addresses.  Every index is checked against its
+buffer (LDS tile, link block, E pools, G block, output vector), and the
+model's result is compared with the oracle's assembled operator, so the
+indexing is known to be in bounds and the formulas right before the kernel
+runs on a GPU.
+"""
+import numpy as np
+import pytest
+
+from oracle import assemble as asm
+from rtx import problems
+
+NM = 10
+HP_MAX_CELLS, HP_MAX_LINKS, HP_MT = 64, 384, 7
+
+
+def hp_stride(na, kstep, c, M):
+    n = (na + kstep - 1) // kstep * kstep
+    return n + ((c - n) % M + M) % M
+
+
+def hp_layout(nsp, na, ninn, ea, mixed):
+    XS = hp_stride(na, 4, 2, 4) if ea == 8 else hp_stride(na, 16 if mixed else 4, 4, 8)
+    XSB = hp_stride(na, 16, 8, 16) if mixed else 0
+    o = nsp * XS * ea
+    o = (o + 15) & ~15
+    o += nsp * XSB * 2
+    o = (o + 15) & ~15
+    o += ninn * ea + 3 * na * ea + na * 4
+    return XS, XSB, (o + 15) & ~15
+
+
+def test_strides_spread_operand_reads_over_banks():
+    """The matrix-core operand reads of a half-wave (16 rows of the node-major
+    copy, 2 adjacent 8-byte k, 4 adjacent 4-byte k, or 2 BF16 quads) fall
+    on 32 distinct 8-byte bank pairs / 64 distinct 4-byte banks."""
+    for na in range(1, 300):
+        XS = hp_stride(na, 4, 2, 4)
+        assert XS >= (na + 3) // 4 * 4
+        pairs = {(l16 * XS + lg) % 32 for l16 in range(16) for lg in range(2)}
+        assert len(pairs) == 32, na
+        XS = hp_stride(na, 4, 4, 8)
+        banks = {(l16 * XS + lg) % 64 for l16 in range(16) for lg in range(4)}
+        assert len(banks) == 64, na
+        XSB = hp_stride(na, 16, 8, 16)
+        assert XSB >= (na + 15) // 16 * 16
+        words = {(l16 * XSB // 2 + 2 * lg + w) % 64 for l16 in range(16) for lg in range(2)
+                 for w in range(2)}
+        assert len(words) == 64, na
+
+
+def host_tables(T):
+    """link_inn (column-local offset of each link's n1 x nth block) and
+    col_inn, as hpcol_build computes them."""
+    link_inn = np.zeros(max(T.nlink, 1), np.int64)
+    col_inn = np.zeros(T.ncol, np.int64)
+    for c in range(T.ncol):
+        k0, k1 = T.col_cell_start[c], T.col_cell_start[c + 1]
+        o = 0
+        for l in range(T.link_start[k0], T.link_start[k1]):
+            s, F = T.link_src[l], T.link_face[l]
+            c1 = T.cell_col[s]
+            n1 = T.col_nx[c1] if F & 1 else T.col_ny[c1]
+            link_inn[l] = o
+            o += n1 * T.cell_nth[T.link_target[l]]
+        col_inn[c] = o
+    return link_inn, col_inn
+
+
+def _on_bdry(bd, i, j, nx, ny):
+    return bool(((bd & 1) and i == nx - 1) or ((bd & 2) and j == ny - 1) or
+                ((bd & 4) and i == 0) or ((bd & 8) and j == 0))
+
+
+def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024):
+    link_inn, col_inn = host_tables(T)
+    y = np.full(T.ndof, np.nan)
+    for c in range(T.ncol):
+        k0, k1 = int(T.col_cell_start[c]), int(T.col_cell_start[c + 1])
+        ncells = k1 - k0
+        nx, ny = int(T.col_nx[c]), int(T.col_ny[c])
+        nsp, na = nx * ny, int(T.col_nang[c])
+        ncd = nsp * na
+        cdof = int(T.cell_dof_off[k0])
+        L0, nl = int(T.link_start[k0]), int(T.link_start[k1] - T.link_start[k0])
+        ninn = int(col_inn[c])
+        assert ncells <= HP_MAX_CELLS and nl <= HP_MAX_LINKS and nsp <= HP_MT * 16
+        for ea, mixed in ((8, False), (4, False), (4, True)):
+            assert hp_layout(nsp, na, ninn, ea, mixed)[2] <= lds_cap
+        XS, _, _ = hp_layout(nsp, na, ninn, 8, False)
+        XSB = hp_layout(nsp, na, ninn, 4, True)[1]
+        x0, y0, x1, y1 = T.col_geom[c]
+        dx, dy = x1 - x0, y1 - y0
+        c_dof = [int(T.cell_dof_off[k]) - cdof for k in range(k0, k1)] + [ncd]
+        c_nth = [int(T.cell_nth[k]) for k in range(k0, k1)]
+        c_bd = [int(T.cell_bdry[k]) if restrict else 0 for k in range(k0, k1)]
+        c_ao = [int(T.cell_ang_off[k]) for k in range(k0, k1)] + [na]
+        c_l0 = [int(T.link_start[k]) - L0 for k in range(k0, k1)] + [nl]
+        # phase 1: node-major copy (xs[s * XS + A]), padding zero
+        xs = np.full(nsp * XS, np.nan)
+        xs_b = np.full(nsp * XSB, np.nan)
+        acell = []
+        for k in range(ncells):
+            acell += [k | (a << 8) for a in range(c_nth[k])]
+        assert len(acell) == na
+        for t in range(ncd):
+            # the kernel's lookup: DOF t lies in the cell of angle t // nsp
+            kk = acell[int((t + 0.5) / nsp)] & 255
+            assert kk == int(np.searchsorted(c_dof[:ncells], t, side="right") - 1)
+            assert c_dof[kk] == nsp * c_ao[kk]
+            nt, lt = c_nth[kk], t - c_dof[kk]
+            s = int((lt + 0.5) / nt)
+            a = lt - s * nt
+            assert s == lt // nt and 0 <= a < nt
+            v = x[cdof + t]
+            i = int((s + 0.5) / ny)
+            if c_bd[kk] and _on_bdry(c_bd[kk], i, s - i * ny, nx, ny):
+                v = 0.0
+            col = c_ao[kk] + a
+            assert 0 <= s * XS + col < nsp * XS and col < na
+            xs[s * XS + col] = v
+            xs_b[s * XSB + col] = v
+        for s in range(nsp):
+            xs[s * XS + na:(s + 1) * XS] = 0.0
+            xs_b[s * XSB + na:(s + 1) * XSB] = 0.0
+        assert not np.isnan(xs).any() and not np.isnan(xs_b).any()
+        # phase 1: angle-contracted traces
+        inn = np.full(max(ninn, 1), np.nan)
+        l_inn = [int(link_inn[L0 + li]) for li in range(nl)] + [ninn]
+        for e in range(ninn):
+            li = int(np.searchsorted(l_inn[:nl], e, side="right") - 1)
+            l = L0 + li
+            w1 = int(T.link_meta[4 * l + 1])
+            F, nx1, ny1 = w1 & 3, (w1 >> 2) & 15, (w1 >> 6) & 15
+            nt1, bd1 = (w1 >> 10) & 15, ((w1 >> 14) & 15) if restrict else 0
+            kk = w1 >> 18
+            assert kk == T.link_target[l] - k0 and F == T.link_face[l]
+            nt = c_nth[kk]
+            le = e - l_inn[li]
+            m1 = int((le + 0.5) / nt)
+            r = le - m1 * nt
+            n1 = nx1 if F & 1 else ny1
+            assert 0 <= m1 < n1 and 0 <= r < nt
+            i1, j1 = {0: (0, m1), 2: (nx1 - 1, m1), 1: (m1, 0), 3: (m1, ny1 - 1)}[F]
+            acc = 0.0
+            if not (bd1 and _on_bdry(bd1, i1, j1, nx1, ny1)):
+                base = int(T.link_meta[4 * l]) + (i1 * ny1 + j1) * nt1
+                et = int(T.link_Eth_off[l]) + r
+                for a in range(nt1):
+                    assert base + a < T.ndof and et + a * nt < T.Eth_pool.size
+                    acc += T.Eth_pool[et + a * nt] * x[base + a]
+            inn[e] = acc
+        # phase 2: matrix-core operands (FP64 K step 4, BF16 K step 16)
+        g0 = int(G_off[c])
+        assert g0 + na * na <= G.size
+        Gt = G[g0:g0 + na * na].reshape(na, na)
+        for kb in range(0, na, 4):
+            for lg in range(4):
+                Ak = kb + lg
+                assert Ak < XS                      # padding column, zero beyond na
+        for kb in range(0, na, 16):
+            assert kb + 16 <= XSB
+        X = xs.reshape(nsp, XS)[:, :na]
+        scat = X @ Gt                               # [s, R]
+        for a in range(na):
+            kk = int(np.searchsorted(c_ao[:ncells], a, side="right") - 1)
+            assert acell[a] == kk | ((a - c_ao[kk]) << 8)
+        wx = T.lgl_w[nx * NM:nx * NM + nx]
+        wy = T.lgl_w[ny * NM:ny * NM + ny]
+        WDx = np.array([[wx[i] * T.lgl_D[(nx * NM + p) * NM + i] for i in range(nx)] for p in range(nx)])
+        WDy = np.array([[wy[j] * T.lgl_D[(ny * NM + q) * NM + j] for j in range(ny)] for q in range(ny)])
+        sp0 = int(T.col_sp_off[c])
+        ga0 = int(T.col_ang_start[c])
+        for R in range(na):
+            ac = acell[R]
+            kk, r = ac & 255, ac >> 8
+            nt, dof, bd = c_nth[kk], c_dof[kk], c_bd[kk]
+            quad = int(T.cell_quad[k0 + kk])
+            dth = T.cell_geom[k0 + kk, 1] - T.cell_geom[k0 + kk, 0]
+            ct, sn, w_r = T.ang_trig[3 * (ga0 + R):3 * (ga0 + R) + 3]
+            vol = dx * dy * 0.125 * dth * w_r
+            cx, cy = dy * 0.25 * dth * w_r * ct, dx * 0.25 * dth * w_r * sn
+            for s in range(nsp):
+                p = int((s + 0.5) / ny)
+                q = s - p * ny
+                xv = xs[s * XS + R]
+                val = vol * wx[p] * wy[q] * kappa[sp0 + s] * xv
+                sx = sum(WDx[p, i] * xs[(i * ny + q) * XS + R] for i in range(nx))
+                sy = sum(WDy[q, j] * xs[(p * ny + j) * XS + R] for j in range(ny))
+                val -= cx * wy[q] * sx + cy * wx[p] * sy
+                if quad >= 0:
+                    for hh in range(2):
+                        F = (quad + hh) & 3
+                        on = (F == 0 and p == nx - 1) or (F == 2 and p == 0) or \
+                             (F == 1 and q == ny - 1) or (F == 3 and q == 0)
+                        if on:
+                            tf = [ct, sn, -ct, -sn][F]
+                            val += ((dx if F & 1 else dy) * 0.25) * dth * \
+                                (wx[p] if F & 1 else wy[q]) * w_r * tf * xv
+                for li in range(c_l0[kk], c_l0[kk + 1]):
+                    l = L0 + li
+                    F = int(T.link_face[l])
+                    s1 = int(T.link_src[l])
+                    c1 = T.cell_col[s1]
+                    n1 = int(T.col_nx[c1] if F & 1 else T.col_ny[c1])
+                    m0 = {0: q if p == nx - 1 else -1, 2: q if p == 0 else -1,
+                          1: p if q == ny - 1 else -1, 3: p if q == 0 else -1}[F]
+                    if m0 < 0:
+                        continue
+                    n0 = nx if F & 1 else ny
+                    es = int(T.link_Es_off[l]) + m0
+                    f = 0.0
+                    for m1 in range(n1):
+                        ii = l_inn[li] + m1 * nt + r
+                        assert es + m1 * n0 < T.Es_pool.size and ii < ninn
+                        f += T.Es_pool[es + m1 * n0] * inn[ii]
+                    val += ((dx if F & 1 else dy) * 0.25) * dth * f
+                sgw = dx * dy * sigma[sp0 + s] * wx[p] * wy[q]
+                val -= sgw * scat[s, R]
+                masked = bool(bd) and _on_bdry(bd, p, q, nx, ny)
+                addr = cdof + dof + s * nt + r
+                assert 0 <= addr < T.ndof and np.isnan(y[addr])
+                y[addr] = 0.0 if masked else val
+    assert not np.isnan(y).any()                     # every output written once
+    return y
+
+
+def _small_random_hp(seed):
+    return problems.random_hp_mesh(seed, base_spt=1, base_ang=2, extra_spt=1, extra_ang=1,
+                                   p_col=0.5, p_cell=0.5)
+
+
+@pytest.mark.parametrize("case", ["random_hp_1", "random_hp_2", "periodic_x"])
+def test_hpcol_model_matches_oracle(case):
+    from rtx.plan import MeshTables
+    if case == "periodic_x":
+        mesh = problems.uniform_mesh([1.0, 1.0], [3, 4, 3], 2, 2, pbcs=(True, False))
+        mesh.ref_col(sorted(mesh.cols)[0], kind="spt", form="h")
+    else:
+        mesh = _small_random_hp(int(case[-1]))
+    wl = problems.two_scatterers()
+    T = MeshTables(mesh)
+    G, G_off = T.G_pool(wl.Phi)
+    kappa, sigma = T.eval_spatial(wl.kappa), T.eval_spatial(wl.sigma)
+    x = np.random.default_rng(3).uniform(-1, 1, T.ndof)
+    y = hpcol_model(T, kappa, sigma, G, G_off, x)
+    V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi, drop=False)
+    A = (V["BC"] - V["IC"] + V["M"] + V["S"]).tocsr()
+    mask = asm.intr_mask(mesh)
+    ref = mask * (A @ (mask * x))
+    err = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+    assert err < 1e-12, err
+
+
+def test_hpcol_host_tables_fit_c3_and_c4():
+    """Every column of the C3 and C4 meshes fits the kernel's static tables
+    and the device's LDS in all three precisions (no generic fallback)."""
+    from rtx.plan import MeshTables
+    for p_col in (0.3, 0.4):
+        T = MeshTables(problems.random_hp_mesh(0, p_col=p_col))
+        _, col_inn = host_tables(T)
+        for c in range(T.ncol):
+            k0, k1 = T.col_cell_start[c], T.col_cell_start[c + 1]
+            nsp, na = int(T.col_nx[c] * T.col_ny[c]), int(T.col_nang[c])
+            assert k1 - k0 <= HP_MAX_CELLS
+            assert T.link_start[k1] - T.link_start[k0] <= HP_MAX_LINKS
+            for ea, mixed in ((8, False), (4, False), (4, True)):
+                assert hp_layout(nsp, na, int(col_inn[c]), ea, mixed)[2] <= 150 * 1024
