@@ -251,8 +251,11 @@ int rtx_arnoldi_workspace(int max_vecs, int64_t *n_doubles);
 int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n, double *d_w,
                      double *d_hcol, double *d_work, int refine, void *stream);
 /* The same step on an FP32 basis d_V / d_w (the inner solves of mixed
-   iterative refinement); dot products, norms and d_hcol in FP64.  No
-   counterpart in the reference (PETSc's basis is FP64). */
+   iterative refinement); dot products, norms and d_hcol in FP64.  With
+   refine 0 the norm of the projected vector is taken from the dot products
+   (|w'|^2 = |w|^2 - sum h_j^2; a relative norm below 1e-6 counts as a
+   breakdown: the new column is zero and h_{nv} = 0) and d_w is left
+   unprojected.  No counterpart in the reference (PETSc's basis is FP64). */
 int rtx_arnoldi_step_f32(const float *d_V, int64_t ld, int nv, int64_t n, float *d_w,
                          double *d_hcol, double *d_work, int refine, void *stream);
 /* Arnoldi step j of a GMRES cycle of m steps (nv = j + 1 basis vectors,
@@ -273,6 +276,15 @@ int rtx_gmres_arnoldi(int basis_precision, const void *d_V, int64_t ld, int64_t 
 int rtx_gmres_arnoldi_static(int basis_precision, const void *d_V, int64_t ld, int64_t n,
                              void *d_w, double *d_hcol, int refine, int j, int m,
                              double *d_state, void *stream);
+/* HIP graphs of a stream's work (the GMRES iterations of a chunk are
+   captured once and replayed).  No counterpart in the reference.
+   capture_begin: start capturing the calls this thread enqueues on `stream`
+   (a created stream, not the legacy default one); capture_end: stop and
+   instantiate into *graph_exec; launch: replay on `stream`; destroy: free. */
+int rtx_graph_capture_begin(void *stream);
+int rtx_graph_capture_end(void *stream, void **graph_exec);
+int rtx_graph_launch(void *graph_exec, void *stream);
+int rtx_graph_destroy(void *graph_exec);
 /* Least-squares state of one restart cycle of m steps kept on the device
    (Givens QR of the Hessenberg matrix; PETSc keeps it on the host, here it
    stays on the GPU so whole chunks of iterations can run as one HIP graph).

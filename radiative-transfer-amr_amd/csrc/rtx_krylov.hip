@@ -8,6 +8,7 @@
 // basis V; reductions are two-phase and deterministic (per-block partial
 // sums, then one reduction per output).
 #include <mutex>
+#include <type_traits>
 
 #include "rtx_internal.h"
 
@@ -18,6 +19,8 @@ constexpr int KCH = 8;           // basis vectors per mdot block (gridDim.y chun
 constexpr int KNB = 1024;        // row blocks of the tall-skinny kernels
 constexpr int KMAXV = 256;       // largest basis handled by one Arnoldi step
 constexpr double REORTH = 0.5;   // re-orthogonalise when |w'|^2 < 0.5 |w|^2
+// state block after the partials (doubles): [0] |w|^2, [1] |w'|^2, [4] flag (int)
+constexpr int WORK_TAIL = 16;
 
 template <int N>
 __device__ inline void block_sum(double (&v)[N], double *sh) {
@@ -175,6 +178,86 @@ __global__ __launch_bounds__(KT) void k_update(const T *__restrict__ V, int64_t 
     }
     block_sum<1>(ss, sh);
     if (threadIdx.x == 0) partial[b] = sh[0];
+}
+
+// One Arnoldi update for an FP32 basis with the norm taken from the dot
+// products: hd[0..nv-1] = V^T w, hd[nv] = |w|^2 (FP64 sums of FP32 data),
+// |w'|^2 = |w|^2 - sum h_j^2 (every block computes it, in the same order),
+// v_out = (w - V h) / |w'| written directly (w itself is left as it was);
+// a relative norm below 1e-6 counts as a breakdown (v_out = 0, h = 0).
+// Block 0 writes hcol[0..nv] and, when state is given, folds the column
+// into the Givens factorisation (k_normalize_givens's recurrence).
+template <typename T, bool VEC>
+__global__ __launch_bounds__(KT) void k_update_pyth(const T *__restrict__ V, int64_t ld, int nv,
+                                                    const double *__restrict__ hd,
+                                                    const T *__restrict__ w,
+                                                    T *__restrict__ v_out, int64_t n,
+                                                    double *__restrict__ hcol, int j, int m,
+                                                    double *state) {
+    __shared__ double hs[KMAXV + 1];
+    __shared__ double beta_sh;
+    for (int q = threadIdx.x; q <= nv; q += KT) hs[q] = hd[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double hh = 0.0;
+        for (int q = 0; q < nv; ++q) hh += hs[q] * hs[q];
+        const double r2 = hs[nv] - hh;
+        beta_sh = (r2 > 1e-12 * hs[nv]) ? sqrt(r2) : 0.0;
+    }
+    __syncthreads();
+    const double beta = beta_sh;
+    const double sc = beta > 0.0 ? 1.0 / beta : 0.0;
+    const int b = blockIdx.x, nb = gridDim.x;
+    int64_t i0, i1;
+    row_range<VEC>(n, b, nb, i0, i1);
+    const int64_t nq = VEC ? (i1 - i0) / 4 : 0;
+    for (int64_t q = threadIdx.x; q < nq; q += KT) {
+        const int64_t i = i0 + 4 * q;
+        double acc[4];
+        ld4(w + i, acc);
+        for (int jj = 0; jj < nv; ++jj) {
+            double vv[4];
+            ld4(V + int64_t(jj) * ld + i, vv);
+            const double hj = hs[jj];
+            acc[0] -= vv[0] * hj; acc[1] -= vv[1] * hj;
+            acc[2] -= vv[2] * hj; acc[3] -= vv[3] * hj;
+        }
+        T out[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) out[u] = T(acc[u] * sc);
+        st4(v_out + i, out);
+    }
+    for (int64_t i = i0 + 4 * nq + threadIdx.x; i < i1; i += KT) {
+        double acc = double(w[i]);
+        for (int jj = 0; jj < nv; ++jj) acc -= double(V[int64_t(jj) * ld + i]) * hs[jj];
+        v_out[i] = T(acc * sc);
+    }
+    if (b != 0 || threadIdx.x != 0) return;
+    for (int q = 0; q < nv; ++q) hcol[q] = hs[q];
+    hcol[nv] = beta;
+    if (!state) return;
+    double *cs = state, *sn = state + m, *g = state + 2 * m;
+    double *Rj = state + 3 * m + 1 + int64_t(j) * m;
+    double *Hj = state + 3 * m + 1 + int64_t(m) * m + int64_t(j) * (m + 1);
+    double *est = state + 3 * m + 1 + int64_t(m) * m + int64_t(m + 1) * m;
+    for (int t = 0; t <= j; ++t) Hj[t] = hs[t];
+    Hj[j + 1] = beta;
+    double carry = hs[0];
+    for (int t = 0; t < j; ++t) {
+        const double bb = hs[t + 1];
+        Rj[t] = cs[t] * carry + sn[t] * bb;
+        carry = -sn[t] * carry + cs[t] * bb;
+    }
+    const double den = hypot(carry, beta);
+    const double c = den != 0.0 ? carry / den : 1.0;
+    const double sv = den != 0.0 ? beta / den : 0.0;
+    cs[j] = c;
+    sn[j] = sv;
+    Rj[j] = den;
+    const double gj = g[j];
+    g[j + 1] = -sv * gj;
+    g[j] = c * gj;
+    est[j] = fabs(g[j + 1]);
 }
 
 // state layout (doubles): [0] |w|^2 before projection, [1] |w'|^2 after;
@@ -354,7 +437,7 @@ extern "C" int rtx_arnoldi_workspace(int max_vecs, int64_t *n_doubles) {
               "rtx_arnoldi_workspace: max_vecs must be in [1, %d)", KMAXV);
     // dot partials ((KMAXV + 1) rows, the last one also holding the second-pass
     // coefficients), norm partials, and a small state block (norms + flag)
-    *n_doubles = int64_t(KMAXV + 2) * KNB + 16;
+    *n_doubles = int64_t(KMAXV + 2) * KNB + WORK_TAIL;
     return RTX_OK;
 }
 
@@ -406,6 +489,23 @@ static int arnoldi_step(const T *d_V, int64_t ld, int nv, int64_t n, T *d_w, dou
     const int nchunk = (nv + KCH - 1) / KCH;
     T *V_out = const_cast<T *>(d_V) + int64_t(nv) * ld;
     const bool vec = (ld % 4 == 0) && aligned16(d_V) && aligned16(d_w);
+    if (refine == 0 && std::is_same<T, float>::value) {
+        // FP32 basis (the inner solves of mixed refinement): classical
+        // Gram-Schmidt, one pass, with the norm from the dot products, in
+        // three launches: dots, their reduction, the update writing the
+        // normalised vector (and the Givens update)
+        double *hd = part2;                     // nv + 1 reduced values
+        launch_mdot<T>(vec, dim3(KNB, nchunk), st, d_V, ld, nv, d_w, n, part, 1, nullptr);
+        k_reduce<<<nv + 1, KT, 0, st>>>(part, KNB, hd, 0, nullptr);
+        if (vec)
+            k_update_pyth<T, true><<<KNB, KT, 0, st>>>(d_V, ld, nv, hd, d_w, V_out, n, d_hcol,
+                                                     gj, gm, d_state);
+        else
+            k_update_pyth<T, false><<<KNB, KT, 0, st>>>(d_V, ld, nv, hd, d_w, V_out, n, d_hcol,
+                                                      gj, gm, d_state);
+        RTX_HIP(hipGetLastError());
+        return RTX_OK;
+    }
     auto normalize = [&]() {
         if (d_state)
             k_normalize_givens<T><<<2048, KT, 0, st>>>(d_w, V_out, n, state + 1, d_hcol, nv, gj,
@@ -483,7 +583,7 @@ extern "C" int rtx_gmres_arnoldi(int basis_precision, const void *d_V, int64_t l
    writes each partial before it reads it, so nothing needs clearing
    between calls).  One copy per device, held until the process ends;
    calls that overlap on one device share it and race. */
-constexpr int64_t STATIC_WORK_DOUBLES = int64_t(KMAXV + 2) * KNB + 16;   // 2.1 MB
+constexpr int64_t STATIC_WORK_DOUBLES = int64_t(KMAXV + 2) * KNB + WORK_TAIL;   // 2.1 MB
 static __device__ __align__(256) double scratch[STATIC_WORK_DOUBLES];
 
 constexpr int MAX_DEVICES = 64;

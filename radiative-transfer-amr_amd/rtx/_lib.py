@@ -106,6 +106,10 @@ SIGNATURES = [
                                            C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                            C.c_void_p, C.c_void_p]),
     ("rtx_gmres_solve", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtx_graph_capture_begin", C.c_int, [C.c_void_p]),
+    ("rtx_graph_capture_end", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("rtx_graph_launch", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("rtx_graph_destroy", C.c_int, [C.c_void_p]),
     # learned coarse correction
     ("rtx_coarse_setup", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p]),

@@ -193,12 +193,17 @@ def _norm(lib, v, work, out):
 class _ChunkGraphs:
     """HIP graphs of chunks of GMRES iterations, keyed by (j0, j1).  A chunk
     runs eagerly the first time it is needed (that also loads every kernel
-    it uses) and is captured the next time; replays reuse the same device
-    buffers, so a graph stays valid for the whole solve."""
+    it uses and makes every lazily allocated buffer) and is captured the
+    next time through the library's own capture calls (rtx_graph_*), on a
+    side stream ordered after the current one; replays run on the current
+    stream and reuse the same device buffers, so a graph stays valid for the
+    whole solve."""
 
-    def __init__(self):
+    def __init__(self, lib):
+        self.lib = lib
         self.seen = set()
         self.graphs = {}
+        self.stream = None
 
     def run(self, key, body, enabled):
         if not enabled:
@@ -210,11 +215,30 @@ class _ChunkGraphs:
                 self.seen.add(key)
                 body()
                 return
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                body()
-            self.graphs[key] = g
-        g.replay()
+            cur = torch.cuda.current_stream()
+            if self.stream is None:
+                self.stream = torch.cuda.Stream(device=cur.device)
+            self.stream.wait_stream(cur)
+            h = C.c_void_p()
+            with torch.cuda.stream(self.stream):
+                st = stream_handle(self.stream)
+                check(self.lib.rtx_graph_capture_begin(st), "rtx_graph_capture_begin")
+                try:
+                    body()
+                finally:
+                    rc = self.lib.rtx_graph_capture_end(st, C.byref(h))
+                check(rc, "rtx_graph_capture_end")
+            cur.wait_stream(self.stream)
+            g = self.graphs[key] = h
+        check(self.lib.rtx_graph_launch(g, stream_handle()), "rtx_graph_launch")
+
+    def __del__(self):
+        try:
+            for g in self.graphs.values():
+                self.lib.rtx_graph_destroy(g)
+            self.graphs = {}
+        except Exception:
+            pass
 
 
 class GMRES:
@@ -253,7 +277,8 @@ class GMRES:
         self.r = torch.empty(n, dtype=torch.float64, device=dev)
         ws = C.c_int64()
         check(self.lib.rtx_arnoldi_workspace(m + 1, C.byref(ws)), "workspace")
-        self.work = torch.empty(int(ws.value), dtype=torch.float64, device=dev)
+        # zero-filled: the single-launch reductions keep their counters here
+        self.work = torch.zeros(int(ws.value), dtype=torch.float64, device=dev)
         self.hcol = torch.zeros(m + 2, dtype=torch.float64, device=dev)
         check(self.lib.rtx_gmres_state_size(m, C.byref(ws)), "gmres state size")
         self.state = torch.zeros(int(ws.value), dtype=torch.float64, device=dev)
@@ -262,7 +287,7 @@ class GMRES:
         self.est_h = torch.zeros(m, dtype=torch.float64).pin_memory()
         self.ydev = torch.empty(m, dtype=torch.float64, device=dev)
         self.scal = torch.empty(2, dtype=torch.float64, device=dev)
-        self.chunks = _ChunkGraphs()
+        self.chunks = _ChunkGraphs(self.lib)
         self._on_direction = None
 
     def _steps(self, j0, j1):
@@ -442,7 +467,7 @@ def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=
     r = torch.empty_like(b)
     work_n = C.c_int64()
     check(lib.rtx_arnoldi_workspace(2, C.byref(work_n)), "workspace")
-    work = torch.empty(int(work_n.value), dtype=torch.float64, device=b.device)
+    work = torch.zeros(int(work_n.value), dtype=torch.float64, device=b.device)
     scal = torch.empty(2, dtype=torch.float64, device=b.device)
     bnorm = _norm(lib, b, work, scal)
     hist, inner_its = [], []

@@ -221,7 +221,7 @@ def test_arnoldi_fp32_basis(cuda):
     W = rng.normal(size=(k, n))
     ws = C.c_int64()
     _lib.check(lib.rtx_arnoldi_workspace(k + 1, C.byref(ws)), "workspace")
-    work = torch.empty(int(ws.value), dtype=torch.float64, device="cuda")
+    work = torch.zeros(int(ws.value), dtype=torch.float64, device="cuda")
     out = {}
     for dt, step in ((torch.float32, lib.rtx_arnoldi_step_f32),
                      (torch.float64, lib.rtx_arnoldi_step)):
