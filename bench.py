@@ -15,13 +15,15 @@ relative residual of 1e-8 with (a) plain FP64 GMRES + block-Jacobi and
 learned coarse preconditioner on solver-generated pairs of the random-field
 workload (64 x 64 x 16 cells, p = 4, 8,192,000 DOF) and times plain FP64,
 mixed and learned+mixed solves of an unseen field there; times the error
-indicator kernels on the same mesh; times the oracle's assembled CSR
+indicator kernels on the same mesh; times the same three solves on the
+batched uneven-order hp mesh with two strong scatterers (8,263,762 DOF),
+the coarse model trained on the two-scatterer family; times the oracle's assembled CSR
 SpMV on a bounded sample on the host CPU; and solves a smaller mesh with
 C2's coefficients (16 x 16 x 8 cells, 256,000 DOF) to 1e-8 on the host CPU
 (scipy GMRES with the oracle's block-Jacobi) and on the GPU.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve]
-                       [--no-cpu] [--no-learned] [--no-indicators]
+                       [--no-cpu] [--no-learned] [--no-indicators] [--no-hp] [--hp-fields N]
                        [--learned-fields N] [--learned-kappa-scales S,...] [--calib]
                        [--profile-apply {c2-f64,c2-mixed,c3-f64,c3-mixed}]
 """
@@ -48,6 +50,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-indicators", action="store_true")
+    ap.add_argument("--no-hp", action="store_true",
+                    help="skip the hp-mesh (C3) time-to-1e-8 section")
+    ap.add_argument("--hp-fields", type=int, default=17,
+                    help="training fields of the learned correction on the hp mesh")
     ap.add_argument("--learned-fields", type=int, default=17)
     ap.add_argument("--learned-kappa-scales", default="1,30",
                     help="comma-separated kappa scales of the random-field workload "
@@ -400,9 +406,14 @@ def main():
         out["solve"] = solve_section(op, mesh, wl)
     if not args.no_indicators:
         out["indicators"] = indicators_section(op, mesh, wl, cpu=not args.no_cpu)
-    if not args.no_learned:
+    if not args.no_learned or not args.no_hp:
         del op
         torch.cuda.empty_cache()
+    if not args.no_hp:
+        out["hp"] = hp_section(fields=args.hp_fields,
+                               log=lambda m: print(m, file=sys.stderr, flush=True))
+        torch.cuda.empty_cache()
+    if not args.no_learned:
         out["learned"] = [learned_section(fields=args.learned_fields, kappa_scale=float(ks),
                                           log=lambda m: print(m, file=sys.stderr, flush=True))
                           for ks in args.learned_kappa_scales.split(",") if ks.strip()]
@@ -564,30 +575,16 @@ def solve_section(op, mesh, wl):
     return res
 
 
-def learned_section(fields=17, eval_seed=10_000, kappa_scale=1.0, log=None):
-    """The learned coarse preconditioner on the random-field workload: 64 x 64
-    columns x 16 angular cells, p = 4 (8,192,000 DOF), log kappa ~ GRF
-    (length 0.15, mean log kappa_scale, variance 1), sigma / kappa ~
-    U(0.9, 0.999), HG g = 0.5, unit source, vacuum inflow.  Pairs from
-    ``fields`` seeded fields (solver-generated), a 4-layer width-128 MLP
-    trained on the device, then fp64 / mixed / learned+mixed solves of an
-    unseen field (seed ``eval_seed``) to an FP64 residual of 1e-8."""
+def _learned_solves(op, mesh, field, eval_wl, fields, log=None, ncoarse=64, epochs=100):
+    """Train the coarse model on solver pairs of ``fields`` seeded members of
+    ``field`` on ``op``'s mesh, then time FP64, mixed and learned+mixed
+    solves of ``eval_wl`` (coefficients set on ``op``) to an FP64 residual of
+    1e-8.  Returns the timing record."""
     import torch
-    from rtx import problems, learned
-    from rtx.operator import TransportOperator
+    from rtx import learned
     from rtx._lib import PREC_MIXED
+    grid = learned.CoarseGrid(op, ncoarse)
     t0 = time.perf_counter()
-    mesh = problems.uniform_mesh([1.0, 1.0], [5, 5, 5], nref_ang=4, nref_spt=6)
-
-    def field(s):
-        return problems.grf_scatterer(s, n=64, kappa_scale=kappa_scale)
-
-    wl0 = field(0)
-    op = TransportOperator(mesh, wl0.kappa, wl0.sigma, wl0.Phi)
-    grid = learned.CoarseGrid(op, 64)
-    t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    # 19 restart pairs + 100 direction pairs per field: about 2,000 pairs
     X, Y = learned.generate_pairs(op, grid, range(fields), cycles=20, restart=10, skip=1,
                                   field=field, direction_stride=2, log=log)
     torch.cuda.synchronize()
@@ -596,26 +593,22 @@ def learned_section(fields=17, eval_seed=10_000, kappa_scale=1.0, log=None):
     mlp = learned.MLP([5 * nc, 128, 128, 128, nc], op.device, seed=0)
     mlp.n_moments = 3 * nc
     t0 = time.perf_counter()
-    xs, ys, hist = learned.train(mlp, X, Y, epochs=100, batch=64, lr=1e-3, log=log)
+    xs, ys, hist = learned.train(mlp, X, Y, epochs=epochs, batch=64, lr=1e-3, log=log)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     npairs = int(X.shape[0])
     del X, Y
-    wl = field(eval_seed)
-    op.set_coeffs(wl.kappa, wl.sigma)
-    rhs = _rhs(op, wl)
-    res = {"workload": "grf scatterer 64x64x16 cells, p=4 (8,192,000 DOF), kappa_scale {}, "
-                       "seed {}, sigma/kappa {:.4f}".format(kappa_scale, eval_seed, wl.ratio),
-           "pairs": npairs, "fields": fields, "setup_seconds": t_setup,
-           "pair_generation_seconds": t_pairs, "training_seconds": t_train,
-           "validation_rel_error": hist[-1][1]}
+    op.set_coeffs(eval_wl.kappa, eval_wl.sigma)
+    rhs = _rhs(op, eval_wl)
+    res = {"pairs": npairs, "fields": fields, "pair_generation_seconds": t_pairs,
+           "training_seconds": t_train, "validation_rel_error": hist[-1][1]}
     x64, res["fp64"] = _timed_solve(op, rhs, "fp64")
     xm, res["mixed"] = _timed_solve(op, rhs, "mixed")
 
     def make_pc(bj):
         model = learned.CoarseModel(mlp, 3 * nc, xs, ys)
         model.validation_error = hist[-1][1]
-        model.set_coefficients(grid.coefficient_channels(wl.kappa, wl.sigma))
+        model.set_coefficients(grid.coefficient_channels(eval_wl.kappa, eval_wl.sigma))
         return learned.TwoLevelPC(op, bj, grid, model, apply_precision=PREC_MIXED)
 
     xl, res["learned_mixed"] = _timed_solve(op, rhs, "mixed", make_pc=make_pc)
@@ -623,6 +616,77 @@ def learned_section(fields=17, eval_seed=10_000, kappa_scale=1.0, log=None):
         res[k]["rel_l2_vs_fp64"] = float((x - x64).norm() / x64.norm())
     _speedups(res, "learned_mixed", "fp64", "seconds")
     _speedups(res, "mixed", "fp64", "seconds")
+    return res
+
+
+def hp_section(fields=17, log=None):
+    """The north star's target workload: the batched uneven-order hp mesh
+    (16 x 16 x 8 base, seeded random h-refinement, node counts 3..9:
+    8,263,762 DOF) with the two strong scatterers (kappa = 1 + 9 max(g1, g2),
+    sigma / kappa = 0.1 + 0.89 max(g1, g2), HG g = 0.9, unit inflow on the
+    left wall).  Time to an FP64 relative residual of 1e-8: plain FP64
+    GMRES(50) + block-Jacobi, mixed refinement, and learned+mixed with the
+    coarse model trained on ``fields`` seeded members of the two-scatterer
+    family (blob centres, radius, strength and ratios at random).  Also the
+    default (hp column kernel) and generic FP64 / mixed apply times."""
+    import torch
+    from rtx import problems
+    from rtx.operator import TransportOperator, OP_SYSTEM, OP_RESTRICT, PREC_F64, PREC_MIXED
+    t0 = time.perf_counter()
+    mesh = problems.random_hp_mesh(0)
+    wl = problems.two_scatterers_c3()
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    t_setup = time.perf_counter() - t0
+    res = {"workload": "two strong scatterers, hp mesh 16x16x8 base + random h-refinement, "
+                       "node counts 3..9 ({} DOF)".format(op.ndof),
+           "ndof": op.ndof, "setup_seconds": t_setup, "apply_us": {}}
+    stream = torch.cuda.current_stream()
+    x = torch.rand(op.ndof, dtype=torch.float64, device=op.device) * 2 - 1
+    op.mask(x)
+    for path, pname in ((-1, "default"), (0, "generic")):
+        op.set_path(path)
+        for prec, xx in ((PREC_F64, x), (PREC_MIXED, x.float())):
+            y = torch.empty_like(xx)
+            for _ in range(3):
+                op.apply(xx, y, ops=OP_SYSTEM | OP_RESTRICT, precision=prec)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(20):
+                op.apply(xx, y, ops=OP_SYSTEM | OP_RESTRICT, precision=prec)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            res["apply_us"]["{}_{}".format(pname, "f64" if prec == PREC_F64 else "mixed")] = \
+                e0.elapsed_time(e1) / 20 * 1e3
+    op.set_path(-1)
+    res.update(_learned_solves(op, mesh, problems.two_scatterer_field, wl, fields, log=log))
+    del op
+    return res
+
+
+def learned_section(fields=17, eval_seed=10_000, kappa_scale=1.0, log=None):
+    """The learned coarse preconditioner on the random-field workload: 64 x 64
+    columns x 16 angular cells, p = 4 (8,192,000 DOF), log kappa ~ GRF
+    (length 0.15, mean log kappa_scale, variance 1), sigma / kappa ~
+    U(0.9, 0.999), HG g = 0.5, unit source, vacuum inflow.  Pairs from
+    ``fields`` seeded fields (solver-generated), a 4-layer width-128 MLP
+    trained on the device, then fp64 / mixed / learned+mixed solves of an
+    unseen field (seed ``eval_seed``) to an FP64 residual of 1e-8."""
+    from rtx import problems
+    from rtx.operator import TransportOperator
+    t0 = time.perf_counter()
+    mesh = problems.uniform_mesh([1.0, 1.0], [5, 5, 5], nref_ang=4, nref_spt=6)
+
+    def field(s):
+        return problems.grf_scatterer(s, n=64, kappa_scale=kappa_scale)
+
+    wl0 = field(0)
+    op = TransportOperator(mesh, wl0.kappa, wl0.sigma, wl0.Phi)
+    t_setup = time.perf_counter() - t0
+    wl = field(eval_seed)
+    res = {"workload": "grf scatterer 64x64x16 cells, p=4 (8,192,000 DOF), kappa_scale {}, "
+                       "seed {}, sigma/kappa {:.4f}".format(kappa_scale, eval_seed, wl.ratio),
+           "setup_seconds": t_setup}
+    res.update(_learned_solves(op, mesh, field, wl, fields, log=log))
     return res
 
 

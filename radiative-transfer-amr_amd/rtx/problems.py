@@ -16,6 +16,10 @@ configurations (the analogue of the reference's
   HG g = 0.8, unit isotropic inflow on the left wall, no forcing.
 * ``two_scatterers``: two Gaussian strong-scattering blobs in a weakly
   scattering background.
+* ``two_scatterers_c3``: the hp-mesh workload's fixed pair of blobs;
+  ``two_scatterer_field(seed)``: a seeded member of the same family (blob
+  centres, radius, strength and ratios drawn at random), the training
+  fields of its learned coarse correction.
 """
 import math
 
@@ -273,3 +277,38 @@ def two_scatterers_c3(g=0.9):
         return np.where(np.asarray(x) == 0.0, 1.0, 0.0) + 0.0 * (y + th)
 
     return Workload("two_scatterers_c3", kappa, sigma, Phi, f, bcs)
+
+
+def two_scatterer_field(seed, g=0.9):
+    """A seeded member of the two-strong-scatterer family: two Gaussian blobs
+    with centres uniform in [0.15, 0.85]^2, a common radius U(0.07, 0.13),
+    kappa = 1 + (k_b - 1) max(g1, g2) with k_b ~ U(5, 15), sigma / kappa =
+    r_0 + (r_b - r_0) max(g1, g2) with r_b ~ U(0.95, 0.995) and r_0 ~
+    U(0.05, 0.2); Henyey-Greenstein g; unit isotropic inflow on the left
+    wall, no source (the form of ``two_scatterers_c3``)."""
+    rng = np.random.default_rng(int(seed) + 7_777)
+    cen = rng.uniform(0.15, 0.85, size=(2, 2))
+    rad = float(rng.uniform(0.07, 0.13))
+    kb = float(rng.uniform(5.0, 15.0))
+    rb, r0 = float(rng.uniform(0.95, 0.995)), float(rng.uniform(0.05, 0.2))
+    Phi = henyey_greenstein(g)
+
+    def blob(x, y):
+        return np.maximum(np.exp(-((x - cen[0, 0]) ** 2 + (y - cen[0, 1]) ** 2) / rad ** 2),
+                          np.exp(-((x - cen[1, 0]) ** 2 + (y - cen[1, 1]) ** 2) / rad ** 2))
+
+    def kappa(x, y):
+        return 1.0 + (kb - 1.0) * blob(x, y)
+
+    def sigma(x, y):
+        return (r0 + (rb - r0) * blob(x, y)) * kappa(x, y)
+
+    def f(x, y, th):
+        return 0.0 * (x + y + th)
+
+    def bcs(x, y, th):
+        return np.where(np.asarray(x) == 0.0, 1.0, 0.0) + 0.0 * (y + th)
+
+    wl = Workload("two_scatterer_field", kappa, sigma, Phi, f, bcs)
+    wl.seed, wl.ratio = int(seed), rb
+    return wl
