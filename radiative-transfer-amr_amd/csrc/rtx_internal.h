@@ -83,7 +83,7 @@ size_t fused_bytes(int64_t col_dofs, int64_t nang, int64_t part_dofs, int64_t pa
 // column-local offsets of each link's angle-contracted trace block and
 // the column totals.
 constexpr int HP_MAX_CELLS = 64;    // cells per column on this path
-constexpr int HP_MAX_LINKS = 384;   // upwind links per column on this path
+constexpr int HP_MAX_LINKS = 192;   // upwind links per column on this path
 constexpr int HP_NCLS = 2;          // LDS classes (one launch each)
 struct HpTables {
     const int32_t *link_inn = nullptr, *col_inn = nullptr;
@@ -164,6 +164,8 @@ struct rtx_plan {
     const int32_t *hp_gen_cells = nullptr, *hp_gen_items = nullptr;
     int hp_n_gen_cells = 0, hp_n_gen_items = 0;
     void *hp_alloc = nullptr;
+    hipStream_t hp_side = nullptr;
+    hipEvent_t hp_fork = nullptr, hp_join = nullptr;
     // host copies: per-column cell start (ncol + 1), spatial nodes, angles;
     // fused-bin columns and the columns no bin takes
     std::vector<int32_t> h_col_cell_start, h_col_nsp, h_col_nang;

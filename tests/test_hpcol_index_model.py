@@ -18,7 +18,7 @@ from oracle import assemble as asm
 from rtx import problems
 
 NM = 10
-HP_MAX_CELLS, HP_MAX_LINKS, HP_MT = 64, 384, 7
+HP_MAX_CELLS, HP_MAX_LINKS, HP_MT = 64, 192, 7
 
 
 def hp_stride(na, kstep, c, M):
@@ -26,29 +26,34 @@ def hp_stride(na, kstep, c, M):
     return n + ((c - n) % M + M) % M
 
 
-def hp_layout(nsp, na, ninn, ea, mixed):
-    XS = hp_stride(na, 4, 2, 4) if ea == 8 else hp_stride(na, 16 if mixed else 4, 4, 8)
+def hp_layout(nsp, na, ninn, ntr, ea, mixed):
+    """(S, NA, XSB, total bytes): the angle-major copy has NA rows (angles,
+    padded to the K step) of S = nsp rounded up to odd entries, in a region
+    that first holds the ntr neighbour-trace values; mixed adds a node-major
+    BF16 copy of row stride XSB."""
+    kstep = 16 if mixed else 4
+    S, NA = nsp | 1, (na + kstep - 1) // kstep * kstep
     XSB = hp_stride(na, 16, 8, 16) if mixed else 0
-    o = nsp * XS * ea
+    o = max(NA * S, ntr) * ea
     o = (o + 15) & ~15
     o += nsp * XSB * 2
     o = (o + 15) & ~15
     o += ninn * ea + 3 * na * ea + na * 4
-    return XS, XSB, (o + 15) & ~15
+    return S, NA, XSB, (o + 15) & ~15
 
 
-def test_strides_spread_operand_reads_over_banks():
-    """The matrix-core operand reads of a half-wave (16 rows of the node-major
-    copy, 2 adjacent 8-byte k, 4 adjacent 4-byte k, or 2 BF16 quads) fall
-    on 32 distinct 8-byte bank pairs / 64 distinct 4-byte banks."""
+def admitted(T, c):
+    """The host's rule for the column kernel: square columns of 3..10 nodes,
+    at most 64 cells and 384 links."""
+    k0, k1 = T.col_cell_start[c], T.col_cell_start[c + 1]
+    return (T.col_nx[c] == T.col_ny[c] and T.col_nx[c] >= 3 and k1 - k0 <= HP_MAX_CELLS
+            and T.link_start[k1] - T.link_start[k0] <= HP_MAX_LINKS)
+
+
+def test_bf16_stride_spreads_operand_reads_over_banks():
+    """The BF16 matrix-core operand reads of a half-wave (16 rows of the
+    node-major copy, 2 quads of k) fall on 64 distinct 4-byte banks."""
     for na in range(1, 300):
-        XS = hp_stride(na, 4, 2, 4)
-        assert XS >= (na + 3) // 4 * 4
-        pairs = {(l16 * XS + lg) % 32 for l16 in range(16) for lg in range(2)}
-        assert len(pairs) == 32, na
-        XS = hp_stride(na, 4, 4, 8)
-        banks = {(l16 * XS + lg) % 64 for l16 in range(16) for lg in range(4)}
-        assert len(banks) == 64, na
         XSB = hp_stride(na, 16, 8, 16)
         assert XSB >= (na + 15) // 16 * 16
         words = {(l16 * XSB // 2 + 2 * lg + w) % 64 for l16 in range(16) for lg in range(2)
@@ -83,6 +88,7 @@ def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024)
     link_inn, col_inn = host_tables(T)
     y = np.full(T.ndof, np.nan)
     for c in range(T.ncol):
+        assert admitted(T, c)
         k0, k1 = int(T.col_cell_start[c]), int(T.col_cell_start[c + 1])
         ncells = k1 - k0
         nx, ny = int(T.col_nx[c]), int(T.col_ny[c])
@@ -92,10 +98,12 @@ def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024)
         L0, nl = int(T.link_start[k0]), int(T.link_start[k1] - T.link_start[k0])
         ninn = int(col_inn[c])
         assert ncells <= HP_MAX_CELLS and nl <= HP_MAX_LINKS and nsp <= HP_MT * 16
+        ntr = int(T.col_ntrace[c])
         for ea, mixed in ((8, False), (4, False), (4, True)):
-            assert hp_layout(nsp, na, ninn, ea, mixed)[2] <= lds_cap
-        XS, _, _ = hp_layout(nsp, na, ninn, 8, False)
-        XSB = hp_layout(nsp, na, ninn, 4, True)[1]
+            assert hp_layout(nsp, na, ninn, ntr, ea, mixed)[3] <= lds_cap
+        S, NA, _, _ = hp_layout(nsp, na, ninn, ntr, 8, False)
+        XSB = hp_layout(nsp, na, ninn, ntr, 4, True)[2]
+        region = max(NA * S, ntr)
         x0, y0, x1, y1 = T.col_geom[c]
         dx, dy = x1 - x0, y1 - y0
         c_dof = [int(T.cell_dof_off[k]) - cdof for k in range(k0, k1)] + [ncd]
@@ -103,8 +111,8 @@ def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024)
         c_bd = [int(T.cell_bdry[k]) if restrict else 0 for k in range(k0, k1)]
         c_ao = [int(T.cell_ang_off[k]) for k in range(k0, k1)] + [na]
         c_l0 = [int(T.link_start[k]) - L0 for k in range(k0, k1)] + [nl]
-        # phase 1: node-major copy (xs[s * XS + A]), padding zero
-        xs = np.full(nsp * XS, np.nan)
+        # phase 1: angle-major copy (xs[A * S + s]), padding rows zero
+        xs = np.full(NA * S, np.nan)
         xs_b = np.full(nsp * XSB, np.nan)
         acell = []
         for k in range(ncells):
@@ -124,38 +132,58 @@ def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024)
             if c_bd[kk] and _on_bdry(c_bd[kk], i, s - i * ny, nx, ny):
                 v = 0.0
             col = c_ao[kk] + a
-            assert 0 <= s * XS + col < nsp * XS and col < na
-            xs[s * XS + col] = v
+            assert 0 <= col * S + s < na * S and col < na
+            xs[col * S + s] = v
             xs_b[s * XSB + col] = v
+        xs[na * S:] = 0.0
         for s in range(nsp):
-            xs[s * XS + na:(s + 1) * XS] = 0.0
             xs_b[s * XSB + na:(s + 1) * XSB] = 0.0
-        assert not np.isnan(xs).any() and not np.isnan(xs_b).any()
-        # phase 1: angle-contracted traces
+        # entries S - 1 of each row are padding when nsp is even (never read)
+        used = np.zeros(NA * S, bool)
+        for A_ in range(NA):
+            used[A_ * S:A_ * S + nsp] = True
+        assert not np.isnan(xs[used]).any() and not np.isnan(xs_b).any()
+        # phase 1: neighbour traces gathered into the x region at the
+        # column-local trace offsets, then contracted with E^theta
+        l_tr = [int(T.link_meta[4 * (L0 + li) + 2]) for li in range(nl)] + [ntr]
+        tr = np.full(region, np.nan)
+        for e in range(ntr):
+            li = int(np.searchsorted(l_tr[:nl], e, side="right") - 1)
+            l = L0 + li
+            w1 = int(T.link_meta[4 * l + 1])
+            F, nx1, ny1 = w1 & 3, (w1 >> 2) & 15, (w1 >> 6) & 15
+            nt1, bd1 = (w1 >> 10) & 15, ((w1 >> 14) & 15) if restrict else 0
+            le = e - l_tr[li]
+            m1 = int((le + 0.5) / nt1)
+            a = le - m1 * nt1
+            n1 = nx1 if F & 1 else ny1
+            assert 0 <= m1 < n1 and 0 <= a < nt1 and e < region
+            i1, j1 = {0: (0, m1), 2: (nx1 - 1, m1), 1: (m1, 0), 3: (m1, ny1 - 1)}[F]
+            v = 0.0
+            if not (bd1 and _on_bdry(bd1, i1, j1, nx1, ny1)):
+                addr = int(T.link_meta[4 * l]) + (i1 * ny1 + j1) * nt1 + a
+                assert 0 <= addr < T.ndof
+                v = x[addr]
+            tr[e] = v
         inn = np.full(max(ninn, 1), np.nan)
         l_inn = [int(link_inn[L0 + li]) for li in range(nl)] + [ninn]
         for e in range(ninn):
             li = int(np.searchsorted(l_inn[:nl], e, side="right") - 1)
             l = L0 + li
             w1 = int(T.link_meta[4 * l + 1])
-            F, nx1, ny1 = w1 & 3, (w1 >> 2) & 15, (w1 >> 6) & 15
-            nt1, bd1 = (w1 >> 10) & 15, ((w1 >> 14) & 15) if restrict else 0
-            kk = w1 >> 18
+            F, nt1, kk = w1 & 3, (w1 >> 10) & 15, w1 >> 18
             assert kk == T.link_target[l] - k0 and F == T.link_face[l]
             nt = c_nth[kk]
             le = e - l_inn[li]
             m1 = int((le + 0.5) / nt)
             r = le - m1 * nt
-            n1 = nx1 if F & 1 else ny1
-            assert 0 <= m1 < n1 and 0 <= r < nt
-            i1, j1 = {0: (0, m1), 2: (nx1 - 1, m1), 1: (m1, 0), 3: (m1, ny1 - 1)}[F]
+            assert 0 <= r < nt
+            et = int(T.link_Eth_off[l]) + r
             acc = 0.0
-            if not (bd1 and _on_bdry(bd1, i1, j1, nx1, ny1)):
-                base = int(T.link_meta[4 * l]) + (i1 * ny1 + j1) * nt1
-                et = int(T.link_Eth_off[l]) + r
-                for a in range(nt1):
-                    assert base + a < T.ndof and et + a * nt < T.Eth_pool.size
-                    acc += T.Eth_pool[et + a * nt] * x[base + a]
+            for a in range(nt1):
+                ti = l_tr[li] + m1 * nt1 + a
+                assert et + a * nt < T.Eth_pool.size and ti < l_tr[li + 1]
+                acc += T.Eth_pool[et + a * nt] * tr[ti]
             inn[e] = acc
         # phase 2: matrix-core operands (FP64 K step 4, BF16 K step 16)
         g0 = int(G_off[c])
@@ -164,10 +192,10 @@ def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024)
         for kb in range(0, na, 4):
             for lg in range(4):
                 Ak = kb + lg
-                assert Ak < XS                      # padding column, zero beyond na
+                assert Ak < NA                      # padding row, zero beyond na
         for kb in range(0, na, 16):
             assert kb + 16 <= XSB
-        X = xs.reshape(nsp, XS)[:, :na]
+        X = xs.reshape(NA, S)[:na, :nsp].T
         scat = X @ Gt                               # [s, R]
         for a in range(na):
             kk = int(np.searchsorted(c_ao[:ncells], a, side="right") - 1)
@@ -190,10 +218,10 @@ def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024)
             for s in range(nsp):
                 p = int((s + 0.5) / ny)
                 q = s - p * ny
-                xv = xs[s * XS + R]
+                xv = xs[R * S + s]
                 val = vol * wx[p] * wy[q] * kappa[sp0 + s] * xv
-                sx = sum(WDx[p, i] * xs[(i * ny + q) * XS + R] for i in range(nx))
-                sy = sum(WDy[q, j] * xs[(p * ny + j) * XS + R] for j in range(ny))
+                sx = sum(WDx[p, i] * xs[R * S + i * ny + q] for i in range(nx))
+                sy = sum(WDy[q, j] * xs[R * S + p * ny + j] for j in range(ny))
                 val -= cx * wy[q] * sx + cy * wx[p] * sy
                 if quad >= 0:
                     for hh in range(2):
@@ -241,7 +269,7 @@ def _small_random_hp(seed):
 def test_hpcol_model_matches_oracle(case):
     from rtx.plan import MeshTables
     if case == "periodic_x":
-        mesh = problems.uniform_mesh([1.0, 1.0], [3, 4, 3], 2, 2, pbcs=(True, False))
+        mesh = problems.uniform_mesh([1.0, 1.0], [4, 4, 3], 2, 2, pbcs=(True, False))
         mesh.ref_col(sorted(mesh.cols)[0], kind="spt", form="h")
     else:
         mesh = _small_random_hp(int(case[-1]))
@@ -271,5 +299,7 @@ def test_hpcol_host_tables_fit_c3_and_c4():
             nsp, na = int(T.col_nx[c] * T.col_ny[c]), int(T.col_nang[c])
             assert k1 - k0 <= HP_MAX_CELLS
             assert T.link_start[k1] - T.link_start[k0] <= HP_MAX_LINKS
+            assert admitted(T, c)
             for ea, mixed in ((8, False), (4, False), (4, True)):
-                assert hp_layout(nsp, na, int(col_inn[c]), ea, mixed)[2] <= 150 * 1024
+                assert hp_layout(nsp, na, int(col_inn[c]), int(T.col_ntrace[c]), ea,
+                                 mixed)[3] <= 148 * 1024
