@@ -12,7 +12,9 @@
 //            operands), and, for every upwind link, the neighbour's trace
 //            contracted with E^theta over the neighbour's angles:
 //            inn[m1, r] = sum_a E^th[a, r] tr[m1, a]
-//            (boundary_convection_matrix.py:348-524, the angular half)
+//            (boundary_convection_matrix.py:348-524, the angular half), then
+//            with E^s over the neighbour's face nodes, summed over the links
+//            of each cell face: one value per (cell, face, face node, angle)
 //   phase 2  each wavefront takes (tile of 16 output angles R, group of
 //            node tiles) items: it accumulates the scattering contraction
 //            sum_A x[s, A] G^T[A, R] on the matrix cores
@@ -20,8 +22,7 @@
 //            adds for each of its (s, R) the mass (mass_matrix.py:140-173),
 //            interior convection (interior_convection_matrix.py:161-243), the
 //            outflow faces (boundary_convection_matrix.py:284-346) and the
-//            upwind inflow faces (the spatial half, E^s applied to inn), and
-//            stores y.
+//            inflow-face sums of the node's faces, and stores y.
 //
 // Phases 1 and 2 are compiled per node count N (3..10), so the line reads of
 // the derivative terms and the E^s rows have compile-time offsets.  Every
@@ -46,6 +47,7 @@ constexpr int HP_MT = (RTX_NMAX * RTX_NMAX + 15) / 16;   // node tiles of 16
 constexpr int HP_XCD = 8;
 constexpr int HP_TG = 2;          // node tiles per scattering pass (accumulators in registers)
 constexpr int HP_NMIN = 3;        // node counts compiled for the column kernel
+constexpr int HP_FV = 8;          // inflow-face sums per thread (a column holds <= HP_FV * HP_THREADS)
 
 typedef double hp_d4 __attribute__((ext_vector_type(4)));
 typedef float hp_f4 __attribute__((ext_vector_type(4)));
@@ -92,10 +94,11 @@ struct HpLayout {           // dynamic LDS of one column, in bytes from the base
 
 // Angle-major copy: NA rows (the column's angles, padded to the K step with
 // zero rows) of S = nsp rounded up to odd entries; the same region first
-// holds the ntr neighbour-trace values of the column's links.  A: LDS
-// element type.
+// holds the ntr neighbour-trace values of the column's links.  The inn
+// region holds nin = max(ninn, nffc) entries: the E^theta-contracted traces,
+// then the inflow-face sums written over them.  A: LDS element type.
 template <typename A, int PREC>
-__host__ __device__ inline HpLayout hp_layout(int nsp, int na, int ninn, int ntr) {
+__host__ __device__ inline HpLayout hp_layout(int nsp, int na, int nin, int ntr) {
     HpLayout L;
     const int ea = int(sizeof(A));
     const int kstep = (PREC == RTX_PREC_MIXED) ? 16 : 4;
@@ -110,7 +113,7 @@ __host__ __device__ inline HpLayout hp_layout(int nsp, int na, int ninn, int ntr
     o += nsp * L.XSB * 2;
     o = (o + 15) & ~15;
     L.inn = o;
-    o += ninn * ea;
+    o += nin * ea;
     L.cth = o;
     o += na * ea;
     L.sth = o;
@@ -128,6 +131,7 @@ template <typename A>
 struct HpShared {
     int c_nth[HP_MAX_CELLS], c_bd[HP_MAX_CELLS], c_lf[HP_MAX_CELLS];
     int c_quad[HP_MAX_CELLS], c_ao[HP_MAX_CELLS + 1], c_l0[HP_MAX_CELLS + 1];
+    int c_fo[HP_MAX_CELLS + 1];     // offsets of the cells' inflow-face sums
     A c_dth[HP_MAX_CELLS];
     int l_w[HP_MAX_LINKS], l_inn[HP_MAX_LINKS + 1], l_es[HP_MAX_LINKS];
     // per link: packed source shape word (link_meta[4 l + 1]), source cell's
@@ -142,7 +146,7 @@ template <typename T, typename A, int PREC, int N>
 __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int accumulate,
                                           HpShared<A> &sh, unsigned char *lds, int c, int k0,
                                           int ncells, int na, int cdof, int L0, int nl,
-                                          int ninn, int ntr, double dx, double dy,
+                                          int ninn, int nffc, int ntr, double dx, double dy,
                                           const T *__restrict__ x, T *__restrict__ y) {
     constexpr int NSP = N * N;
     constexpr int MT = (NSP + 15) / 16;
@@ -151,7 +155,7 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
     const bool restrict_ = (ops & RTX_OP_RESTRICT) != 0;
     const bool bconv = (ops & RTX_OP_BCONV) != 0;
     const int ncd = NSP * na;
-    const HpLayout Lo = hp_layout<A, PREC>(NSP, na, ninn, ntr);
+    const HpLayout Lo = hp_layout<A, PREC>(NSP, na, ninn > nffc ? ninn : nffc, ntr);
     const int S = Lo.S, XSB = Lo.XSB;
     A *xs = reinterpret_cast<A *>(lds + Lo.xs);
     uint16_t *xb = reinterpret_cast<uint16_t *>(lds + Lo.xb);
@@ -227,6 +231,58 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
             inn[e] = acc;
         }
         __syncthreads();
+        // inflow-face sums.  Entry (kk, h, m0, r): cell kk, the h-th of its
+        // faces that carry links (ascending face number F), own face node
+        // m0, angle r of the cell:
+        //   sum over kk's links on F, in link order, of
+        //   (face factor) * sum_m1 E^s[m0, m1] inn[link, m1, r]
+        // (the spatial half of boundary_convection_matrix.py:348-524).  They
+        // are held in registers over a barrier, then written over inn, which
+        // no one reads after them.
+        A fv[HP_FV];
+#pragma unroll
+        for (int u = 0; u < HP_FV; ++u) {
+            const int e = tid + u * HP_THREADS;
+            fv[u] = A(0);
+            if (e < nffc) {
+                const int kk = hp_find(sh.c_fo, 0, ncells, e);
+                const int nt = sh.c_nth[kk], lf = sh.c_lf[kk];
+                const int le = e - sh.c_fo[kk];
+                const int row = hp_div(le, 1.0f / float(nt)), r = le - row * nt;
+                const int hs = hp_div(row, 1.0f / float(N)), m0 = row - hs * N;
+                int F = 0;
+                for (int cnt = hs; F < 4; ++F)
+                    if ((lf >> F) & 1) {
+                        if (cnt == 0) break;
+                        --cnt;
+                    }
+                const A fxy = A(((F & 1) ? dx : dy) * 0.25) * sh.c_dth[kk];
+                A sum = A(0);
+                for (int li = sh.c_l0[kk]; li < sh.c_l0[kk + 1]; ++li) {
+                    const int w1 = sh.l_w[li];
+                    if ((w1 & 3) != F) continue;
+                    const int n1 = w1 >> 2;
+                    const double *Es = m.Es_pool + sh.l_es[li] + m0;
+                    const A *in_ = inn + sh.l_inn[li] + r;
+                    // the E^s row's (at most RTX_NMAX) loads in flight together
+                    A ev[HP_NM];
+#pragma unroll
+                    for (int m1 = 0; m1 < HP_NM; ++m1) ev[m1] = (m1 < n1) ? A(Es[m1 * N]) : A(0);
+                    A f = A(0);
+#pragma unroll
+                    for (int m1 = 0; m1 < HP_NM; ++m1)
+                        if (m1 < n1) f += ev[m1] * in_[m1 * nt];
+                    sum += fxy * f;
+                }
+                fv[u] = sum;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < HP_FV; ++u) {
+            const int e = tid + u * HP_THREADS;
+            if (e < nffc) inn[e] = fv[u];
+        }
     }
     for (int t0 = tid; t0 < ncd; t0 += XU * HP_THREADS) {
         if (t0 != tid) {
@@ -358,7 +414,6 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
         const int kk = ac & 255, r = ac >> 8;
         const int nt = sh.c_nth[kk], bd = sh.c_bd[kk], quad = sh.c_quad[kk];
         const int lf = bconv ? sh.c_lf[kk] : 0;
-        const int lk0 = sh.c_l0[kk], lk1 = sh.c_l0[kk + 1];
         const A dth = sh.c_dth[kk], ct = cth[R], sn = sth[R], w_r = wr[R];
         const A vol = A(dx * dy * 0.125) * dth * w_r;
         const A cx = A(dy * 0.25) * dth * w_r * ct, cy = A(dx * 0.25) * dth * w_r * sn;
@@ -404,17 +459,20 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
                     }
                 }
                 if (onf & lf) {
-                    // upwind couplings of this cell on the node's faces, in link order
-                    for (int li = lk0; li < lk1; ++li) {
-                        const int w1 = sh.l_w[li];
-                        const int F = w1 & 3, n1 = w1 >> 2;
-                        if (!(onf & (1 << F))) continue;
-                        const int m0 = (F & 1) ? p : q;
-                        const double *Es = m.Es_pool + sh.l_es[li] + m0;
-                        const A *in_ = inn + sh.l_inn[li] + r;
-                        A f = A(0);
-                        for (int m1 = 0; m1 < n1; ++m1) f += A(Es[m1 * N]) * in_[m1 * nt];
-                        val += ((F & 1) ? fy : fx) * f;
+                    // the cell's inflow-face sums on the node's faces, by face
+                    const int fo = sh.c_fo[kk];
+                    int slot = 0;
+                    // kept rolled: unrolled, the compiler's schedule of this
+                    // loop spills phase 2's registers
+#pragma unroll 1
+                    for (int F = 0; F < 4; ++F) {
+                        if ((lf >> F) & 1) {
+                            if ((onf >> F) & 1) {
+                                const int m0 = (F & 1) ? p : q;
+                                val += inn[fo + (slot * N + m0) * nt + r];
+                            }
+                            ++slot;
+                        }
                     }
                 }
                 if (ops & RTX_OP_SCAT) val -= sh.sgw[s] * A(acc[t][v]);
@@ -454,11 +512,11 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
     const int cdof = m.cell_dof_off[k0];
     const int L0 = m.link_start[k0], nl = m.link_start[k0 + ncells] - L0;
     const int ga0 = m.col_ang_start[c];
-    const int ninn = h.col_inn[c], ntr = m.col_ntrace[c];
+    const int ninn = h.col_inn[c], nffc = h.col_ffc[c], ntr = m.col_ntrace[c];
     const bool restrict_ = (ops & RTX_OP_RESTRICT) != 0;
     const double x0 = m.col_geom[4 * c + 0], y0 = m.col_geom[4 * c + 1];
     const double dx = m.col_geom[4 * c + 2] - x0, dy = m.col_geom[4 * c + 3] - y0;
-    const HpLayout Lo = hp_layout<A, PREC>(nsp, na, ninn, ntr);
+    const HpLayout Lo = hp_layout<A, PREC>(nsp, na, ninn > nffc ? ninn : nffc, ntr);
     A *cth = reinterpret_cast<A *>(hp_lds + Lo.cth);
     A *sth = reinterpret_cast<A *>(hp_lds + Lo.sth);
     A *wr = reinterpret_cast<A *>(hp_lds + Lo.wr);
@@ -474,6 +532,7 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
         sh.c_quad[tid] = m.cell_quad[k];
         sh.c_ao[tid] = ao;
         sh.c_l0[tid] = lb - L0;
+        sh.c_fo[tid] = h.cell_fo[k];
         sh.c_dth[tid] = A(m.cell_geom[2 * k + 1] - m.cell_geom[2 * k]);
         int lf = 0;
         for (int l = lb; l < le; ++l) lf |= 1 << m.link_face[l];
@@ -484,6 +543,7 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
         sh.c_ao[ncells] = na;
         sh.c_l0[ncells] = nl;
         sh.l_inn[nl] = ninn;
+        sh.c_fo[ncells] = nffc;
         sh.l_tr[nl] = ntr;
     }
     for (int a = tid; a < na; a += HP_THREADS) {
@@ -530,7 +590,7 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
 #define HP_CASE(NN)                                                                           \
     case NN:                                                                                  \
         hp_column<T, A, PREC, NN>(m, ops, accumulate, sh, hp_lds, c, k0, ncells, na, cdof, L0, \
-                                  nl, ninn, ntr, dx, dy, x, y);                               \
+                                  nl, ninn, nffc, ntr, dx, dy, x, y);                         \
         break;
         HP_CASE(3) HP_CASE(4) HP_CASE(5) HP_CASE(6) HP_CASE(7) HP_CASE(8) HP_CASE(9) HP_CASE(10)
 #undef HP_CASE
@@ -542,12 +602,12 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
 
 // Bytes of dynamic LDS a column needs (FP64 or FP32 storage, mixed adds the
 // BF16 copy).
-static size_t hp_bytes(int nsp, int na, int ninn, int ntr, int prec) {
+static size_t hp_bytes(int nsp, int na, int nin, int ntr, int prec) {
     if (prec == RTX_PREC_F64)
-        return size_t(hp_layout<double, RTX_PREC_F64>(nsp, na, ninn, ntr).total);
+        return size_t(hp_layout<double, RTX_PREC_F64>(nsp, na, nin, ntr).total);
     if (prec == RTX_PREC_F32)
-        return size_t(hp_layout<float, RTX_PREC_F32>(nsp, na, ninn, ntr).total);
-    return size_t(hp_layout<float, RTX_PREC_MIXED>(nsp, na, ninn, ntr).total);
+        return size_t(hp_layout<float, RTX_PREC_F32>(nsp, na, nin, ntr).total);
+    return size_t(hp_layout<float, RTX_PREC_MIXED>(nsp, na, nin, ntr).total);
 }
 
 // Static LDS of k_hpcol (the tables above), for the capacity check.
@@ -557,6 +617,7 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
     const int nc = d->ncol;
     p->hp_ok = false;
     std::vector<int32_t> link_inn(std::max(d->nlink, 1), 0), col_inn(nc, 0);
+    std::vector<int32_t> cell_fo(std::max(d->ncell, 1), 0), col_ffc(nc, 0);
     std::vector<char> fits(nc, 0);
     std::vector<size_t> need[3];
     for (int k = 0; k < 3; ++k) need[k].assign(nc, 0);
@@ -575,12 +636,23 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
             o += n1 * d->cell_nth[d->link_target[l]];
         }
         col_inn[c] = o;
+        // inflow-face sums: per cell, one block of nx x nth per face that
+        // carries links
+        int fo = 0;
+        for (int k = k0; k < k1; ++k) {
+            int lf = 0;
+            for (int l = d->link_start[k]; l < d->link_start[k + 1]; ++l) lf |= 1 << d->link_face[l];
+            cell_fo[k] = fo;
+            fo += __builtin_popcount(lf) * d->col_nx[c] * d->cell_nth[k];
+        }
+        col_ffc[c] = fo;
         const int nlc = d->link_start[k1] - d->link_start[k0];
         // square columns of HP_NMIN..RTX_NMAX nodes (the compiled node counts)
         bool ok = (k1 - k0) <= HP_MAX_CELLS && nlc <= HP_MAX_LINKS &&
-                  d->col_nx[c] == d->col_ny[c] && d->col_nx[c] >= HP_NMIN && na <= 4096;
+                  d->col_nx[c] == d->col_ny[c] && d->col_nx[c] >= HP_NMIN && na <= 4096 &&
+                  fo <= HP_FV * HP_THREADS;
         for (int pr = 0; pr < 3; ++pr) {
-            need[pr][c] = hp_bytes(nsp, na, o, d->col_ntrace[c], pr);
+            need[pr][c] = hp_bytes(nsp, na, std::max(o, fo), d->col_ntrace[c], pr);
             ok = ok && need[pr][c] <= cap;
         }
         fits[c] = ok ? 1 : 0;
@@ -609,7 +681,8 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
     for (int c : over)
         for (int k = d->col_cell_start[c]; k < d->col_cell_start[c + 1]; ++k) gcells.push_back(k);
     const std::vector<int32_t> gitems = scatter_items(p, over);
-    size_t total = link_inn.size() + col_inn.size() + gcells.size() + gitems.size();
+    size_t total = link_inn.size() + col_inn.size() + cell_fo.size() + col_ffc.size() +
+                   gcells.size() + gitems.size();
     for (int k = 0; k < HP_NCLS; ++k) total += lists[k].size();
     RTX_HIP(hipMalloc(&p->hp_alloc, total * sizeof(int32_t)));
     int32_t *base = static_cast<int32_t *>(p->hp_alloc);
@@ -623,6 +696,8 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
     };
     p->hp.link_inn = put(link_inn);
     p->hp.col_inn = put(col_inn);
+    p->hp.cell_fo = put(cell_fo);
+    p->hp.col_ffc = put(col_ffc);
     for (int k = 0; k < HP_NCLS; ++k) {
         p->hp_cls[k].cols = put(lists[k]);
         p->hp_cls[k].n = int(lists[k].size());

@@ -87,6 +87,9 @@ constexpr int HP_MAX_LINKS = 192;   // upwind links per column on this path
 constexpr int HP_NCLS = 2;          // LDS classes (one launch each)
 struct HpTables {
     const int32_t *link_inn = nullptr, *col_inn = nullptr;
+    // per cell: column-local offset of its inflow-face sums; per column:
+    // their count
+    const int32_t *cell_fo = nullptr, *col_ffc = nullptr;
 };
 struct HpClass {
     int n = 0;                      // columns

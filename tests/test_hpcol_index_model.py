@@ -18,7 +18,7 @@ from oracle import assemble as asm
 from rtx import problems
 
 NM = 10
-HP_MAX_CELLS, HP_MAX_LINKS, HP_MT = 64, 192, 7
+HP_MAX_CELLS, HP_MAX_LINKS, HP_MT, HP_FV = 64, 192, 7, 8
 
 
 def hp_stride(na, kstep, c, M):
@@ -26,11 +26,12 @@ def hp_stride(na, kstep, c, M):
     return n + ((c - n) % M + M) % M
 
 
-def hp_layout(nsp, na, ninn, ntr, ea, mixed):
+def hp_layout(nsp, na, nin, ntr, ea, mixed):
     """(S, NA, XSB, total bytes): the angle-major copy has NA rows (angles,
     padded to the K step) of S = nsp rounded up to odd entries, in a region
     that first holds the ntr neighbour-trace values; mixed adds a node-major
-    BF16 copy of row stride XSB."""
+    BF16 copy of row stride XSB; the inn region holds nin = max(ninn, nffc)
+    entries."""
     kstep = 16 if mixed else 4
     S, NA = nsp | 1, (na + kstep - 1) // kstep * kstep
     XSB = hp_stride(na, 16, 8, 16) if mixed else 0
@@ -38,16 +39,37 @@ def hp_layout(nsp, na, ninn, ntr, ea, mixed):
     o = (o + 15) & ~15
     o += nsp * XSB * 2
     o = (o + 15) & ~15
-    o += ninn * ea + 3 * na * ea + na * 4
+    o += nin * ea + 3 * na * ea + na * 4
     return S, NA, XSB, (o + 15) & ~15
 
 
-def admitted(T, c):
+def face_tables(T):
+    """cell_fo (column-local offset of each cell's inflow-face sums: one
+    block of nx x nth per face that carries links) and col_ffc, as
+    hpcol_build computes them."""
+    cell_fo = np.zeros(max(T.ncell, 1), np.int64)
+    col_ffc = np.zeros(T.ncol, np.int64)
+    for c in range(T.ncol):
+        fo = 0
+        for k in range(T.col_cell_start[c], T.col_cell_start[c + 1]):
+            lf = 0
+            for l in range(T.link_start[k], T.link_start[k + 1]):
+                lf |= 1 << int(T.link_face[l])
+            cell_fo[k] = fo
+            fo += bin(lf).count("1") * int(T.col_nx[c]) * int(T.cell_nth[k])
+        col_ffc[c] = fo
+    return cell_fo, col_ffc
+
+
+def admitted(T, c, col_ffc=None):
     """The host's rule for the column kernel: square columns of 3..10 nodes,
-    at most 64 cells and 384 links."""
+    at most 64 cells and 192 links, at most HP_FV x 512 inflow-face sums."""
     k0, k1 = T.col_cell_start[c], T.col_cell_start[c + 1]
+    if col_ffc is None:
+        col_ffc = face_tables(T)[1]
     return (T.col_nx[c] == T.col_ny[c] and T.col_nx[c] >= 3 and k1 - k0 <= HP_MAX_CELLS
-            and T.link_start[k1] - T.link_start[k0] <= HP_MAX_LINKS)
+            and T.link_start[k1] - T.link_start[k0] <= HP_MAX_LINKS
+            and col_ffc[c] <= HP_FV * 512)
 
 
 def test_bf16_stride_spreads_operand_reads_over_banks():
@@ -86,9 +108,10 @@ def _on_bdry(bd, i, j, nx, ny):
 
 def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024):
     link_inn, col_inn = host_tables(T)
+    cell_fo, col_ffc = face_tables(T)
     y = np.full(T.ndof, np.nan)
     for c in range(T.ncol):
-        assert admitted(T, c)
+        assert admitted(T, c, col_ffc)
         k0, k1 = int(T.col_cell_start[c]), int(T.col_cell_start[c + 1])
         ncells = k1 - k0
         nx, ny = int(T.col_nx[c]), int(T.col_ny[c])
@@ -96,13 +119,14 @@ def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024)
         ncd = nsp * na
         cdof = int(T.cell_dof_off[k0])
         L0, nl = int(T.link_start[k0]), int(T.link_start[k1] - T.link_start[k0])
-        ninn = int(col_inn[c])
+        ninn, nffc = int(col_inn[c]), int(col_ffc[c])
+        nin = max(ninn, nffc)
         assert ncells <= HP_MAX_CELLS and nl <= HP_MAX_LINKS and nsp <= HP_MT * 16
         ntr = int(T.col_ntrace[c])
         for ea, mixed in ((8, False), (4, False), (4, True)):
-            assert hp_layout(nsp, na, ninn, ntr, ea, mixed)[3] <= lds_cap
-        S, NA, _, _ = hp_layout(nsp, na, ninn, ntr, 8, False)
-        XSB = hp_layout(nsp, na, ninn, ntr, 4, True)[2]
+            assert hp_layout(nsp, na, nin, ntr, ea, mixed)[3] <= lds_cap
+        S, NA, _, _ = hp_layout(nsp, na, nin, ntr, 8, False)
+        XSB = hp_layout(nsp, na, nin, ntr, 4, True)[2]
         region = max(NA * S, ntr)
         x0, y0, x1, y1 = T.col_geom[c]
         dx, dy = x1 - x0, y1 - y0
@@ -165,7 +189,7 @@ def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024)
                 assert 0 <= addr < T.ndof
                 v = x[addr]
             tr[e] = v
-        inn = np.full(max(ninn, 1), np.nan)
+        inn = np.full(max(nin, 1), np.nan)
         l_inn = [int(link_inn[L0 + li]) for li in range(nl)] + [ninn]
         for e in range(ninn):
             li = int(np.searchsorted(l_inn[:nl], e, side="right") - 1)
@@ -185,6 +209,48 @@ def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024)
                 assert et + a * nt < T.Eth_pool.size and ti < l_tr[li + 1]
                 acc += T.Eth_pool[et + a * nt] * tr[ti]
             inn[e] = acc
+        # phase 1: inflow-face sums, entry e -> (cell kk, h-th link face F,
+        # own face node m0, angle r), computed from inn and then written over it
+        c_fo = [int(cell_fo[k]) for k in range(k0, k1)] + [nffc]
+        c_lf = []
+        for k in range(k0, k1):
+            lf = 0
+            for l in range(T.link_start[k], T.link_start[k + 1]):
+                lf |= 1 << int(T.link_face[l])
+            c_lf.append(lf)
+        fv = np.full(max(nffc, 1), np.nan)
+        for e in range(nffc):
+            # the kernel's search: the largest kk with c_fo[kk] <= e
+            kk = int(np.searchsorted(c_fo[:ncells], e, side="right") - 1)
+            assert c_fo[kk] <= e < c_fo[kk + 1]
+            nt, lf = c_nth[kk], c_lf[kk]
+            le = e - c_fo[kk]
+            row = int((le + 0.5) / nt)
+            r = le - row * nt
+            hs = int((row + 0.5) / nx)
+            m0 = row - hs * nx
+            assert row == le // nt and hs == row // nx and 0 <= r < nt and 0 <= m0 < nx
+            faces = [F for F in range(4) if (lf >> F) & 1]
+            assert hs < len(faces)
+            F = faces[hs]
+            dth = T.cell_geom[k0 + kk, 1] - T.cell_geom[k0 + kk, 0]
+            fxy = ((dx if F & 1 else dy) * 0.25) * dth
+            acc = 0.0
+            for li in range(c_l0[kk], c_l0[kk + 1]):
+                l = L0 + li
+                if int(T.link_face[l]) != F:
+                    continue
+                c1 = T.cell_col[int(T.link_src[l])]
+                n1 = int(T.col_nx[c1] if F & 1 else T.col_ny[c1])
+                es = int(T.link_Es_off[l]) + m0
+                f = 0.0
+                for m1 in range(n1):
+                    ii = l_inn[li] + m1 * nt + r
+                    assert es + m1 * nx < T.Es_pool.size and ii < ninn
+                    f += T.Es_pool[es + m1 * nx] * inn[ii]
+                acc += fxy * f
+            fv[e] = acc
+        inn[:nffc] = fv[:nffc]
         # phase 2: matrix-core operands (FP64 K step 4, BF16 K step 16)
         g0 = int(G_off[c])
         assert g0 + na * na <= G.size
@@ -232,24 +298,17 @@ def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024)
                             tf = [ct, sn, -ct, -sn][F]
                             val += ((dx if F & 1 else dy) * 0.25) * dth * \
                                 (wx[p] if F & 1 else wy[q]) * w_r * tf * xv
-                for li in range(c_l0[kk], c_l0[kk + 1]):
-                    l = L0 + li
-                    F = int(T.link_face[l])
-                    s1 = int(T.link_src[l])
-                    c1 = T.cell_col[s1]
-                    n1 = int(T.col_nx[c1] if F & 1 else T.col_ny[c1])
+                slot = 0
+                for F in range(4):
+                    if not (c_lf[kk] >> F) & 1:
+                        continue
                     m0 = {0: q if p == nx - 1 else -1, 2: q if p == 0 else -1,
                           1: p if q == ny - 1 else -1, 3: p if q == 0 else -1}[F]
-                    if m0 < 0:
-                        continue
-                    n0 = nx if F & 1 else ny
-                    es = int(T.link_Es_off[l]) + m0
-                    f = 0.0
-                    for m1 in range(n1):
-                        ii = l_inn[li] + m1 * nt + r
-                        assert es + m1 * n0 < T.Es_pool.size and ii < ninn
-                        f += T.Es_pool[es + m1 * n0] * inn[ii]
-                    val += ((dx if F & 1 else dy) * 0.25) * dth * f
+                    if m0 >= 0:
+                        ii = c_fo[kk] + (slot * nx + m0) * nt + r
+                        assert c_fo[kk] <= ii < c_fo[kk + 1]
+                        val += inn[ii]
+                    slot += 1
                 sgw = dx * dy * sigma[sp0 + s] * wx[p] * wy[q]
                 val -= sgw * scat[s, R]
                 masked = bool(bd) and _on_bdry(bd, p, q, nx, ny)
@@ -294,12 +353,13 @@ def test_hpcol_host_tables_fit_c3_and_c4():
     for p_col in (0.3, 0.4):
         T = MeshTables(problems.random_hp_mesh(0, p_col=p_col))
         _, col_inn = host_tables(T)
+        _, col_ffc = face_tables(T)
         for c in range(T.ncol):
             k0, k1 = T.col_cell_start[c], T.col_cell_start[c + 1]
             nsp, na = int(T.col_nx[c] * T.col_ny[c]), int(T.col_nang[c])
             assert k1 - k0 <= HP_MAX_CELLS
             assert T.link_start[k1] - T.link_start[k0] <= HP_MAX_LINKS
-            assert admitted(T, c)
+            assert admitted(T, c, col_ffc)
+            nin = max(int(col_inn[c]), int(col_ffc[c]))
             for ea, mixed in ((8, False), (4, False), (4, True)):
-                assert hp_layout(nsp, na, int(col_inn[c]), int(T.col_ntrace[c]), ea,
-                                 mixed)[3] <= 148 * 1024
+                assert hp_layout(nsp, na, nin, int(T.col_ntrace[c]), ea, mixed)[3] <= 148 * 1024
