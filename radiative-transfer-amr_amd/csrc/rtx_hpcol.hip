@@ -35,6 +35,7 @@
 // operands on v_mfma_f32_16x16x16bf16_1k with FP32 accumulation).
 #include <algorithm>
 #include <type_traits>
+#include <utility>
 
 #include "rtx_internal.h"
 
@@ -88,7 +89,7 @@ __host__ __device__ inline int hp_stride(int na, int kstep, int c, int M) {
 }
 
 struct HpLayout {           // dynamic LDS of one column, in bytes from the base
-    int xs, xb, inn, cth, sth, wr, acell, total;
+    int xs, xb, inn, es, cth, sth, wr, acell, total;
     int S, NA, XSB;         // angle-major row (node) stride, padded angle rows, BF16 stride
 };
 
@@ -96,9 +97,10 @@ struct HpLayout {           // dynamic LDS of one column, in bytes from the base
 // zero rows) of S = nsp rounded up to odd entries; the same region first
 // holds the ntr neighbour-trace values of the column's links.  The inn
 // region holds nin = max(ninn, nffc) entries: the E^theta-contracted traces,
-// then the inflow-face sums written over them.  A: LDS element type.
+// then the inflow-face sums written over them; the es region the column's
+// nes distinct E^s values.  A: LDS element type.
 template <typename A, int PREC>
-__host__ __device__ inline HpLayout hp_layout(int nsp, int na, int nin, int ntr) {
+__host__ __device__ inline HpLayout hp_layout(int nsp, int na, int nin, int ntr, int nes) {
     HpLayout L;
     const int ea = int(sizeof(A));
     const int kstep = (PREC == RTX_PREC_MIXED) ? 16 : 4;
@@ -114,6 +116,8 @@ __host__ __device__ inline HpLayout hp_layout(int nsp, int na, int nin, int ntr)
     o = (o + 15) & ~15;
     L.inn = o;
     o += nin * ea;
+    L.es = o;
+    o += nes * ea;
     L.cth = o;
     o += na * ea;
     L.sth = o;
@@ -146,8 +150,10 @@ template <typename T, typename A, int PREC, int N>
 __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int accumulate,
                                           HpShared<A> &sh, unsigned char *lds, int c, int k0,
                                           int ncells, int na, int cdof, int L0, int nl,
-                                          int ninn, int nffc, int ntr, double dx, double dy,
-                                          const T *__restrict__ x, T *__restrict__ y) {
+                                          int ninn, int nffc, int ntr, int es0, int nes,
+                                          const double *__restrict__ h_es, double dx,
+                                          double dy, const T *__restrict__ x,
+                                          T *__restrict__ y) {
     constexpr int NSP = N * N;
     constexpr int MT = (NSP + 15) / 16;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -155,11 +161,12 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
     const bool restrict_ = (ops & RTX_OP_RESTRICT) != 0;
     const bool bconv = (ops & RTX_OP_BCONV) != 0;
     const int ncd = NSP * na;
-    const HpLayout Lo = hp_layout<A, PREC>(NSP, na, ninn > nffc ? ninn : nffc, ntr);
+    const HpLayout Lo = hp_layout<A, PREC>(NSP, na, ninn > nffc ? ninn : nffc, ntr, nes);
     const int S = Lo.S, XSB = Lo.XSB;
     A *xs = reinterpret_cast<A *>(lds + Lo.xs);
     uint16_t *xb = reinterpret_cast<uint16_t *>(lds + Lo.xb);
     A *inn = reinterpret_cast<A *>(lds + Lo.inn);
+    A *es = reinterpret_cast<A *>(lds + Lo.es);
     const A *cth = reinterpret_cast<const A *>(lds + Lo.cth);
     const A *sth = reinterpret_cast<const A *>(lds + Lo.sth);
     const A *wr = reinterpret_cast<const A *>(lds + Lo.wr);
@@ -179,6 +186,8 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
         vv[u] = (t < ncd) ? A(x[cdof + t]) : A(0);
     }
     if (bconv) {
+        // the column's distinct E^s blocks -> LDS (read after two barriers)
+        for (int e = tid; e < nes; e += HP_THREADS) es[e] = A(h_es[es0 + e]);
         A *tr = xs;
         for (int e0 = tid; e0 < ntr; e0 += XU * HP_THREADS) {
             int addr[XU];
@@ -262,12 +271,12 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
                     const int w1 = sh.l_w[li];
                     if ((w1 & 3) != F) continue;
                     const int n1 = w1 >> 2;
-                    const double *Es = m.Es_pool + sh.l_es[li] + m0;
+                    const A *Es = es + sh.l_es[li] + m0;
                     const A *in_ = inn + sh.l_inn[li] + r;
-                    // the E^s row's (at most RTX_NMAX) loads in flight together
+                    // the E^s row's (at most RTX_NMAX) reads in flight together
                     A ev[HP_NM];
 #pragma unroll
-                    for (int m1 = 0; m1 < HP_NM; ++m1) ev[m1] = (m1 < n1) ? A(Es[m1 * N]) : A(0);
+                    for (int m1 = 0; m1 < HP_NM; ++m1) ev[m1] = (m1 < n1) ? Es[m1 * N] : A(0);
                     A f = A(0);
 #pragma unroll
                     for (int m1 = 0; m1 < HP_NM; ++m1)
@@ -513,10 +522,11 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
     const int L0 = m.link_start[k0], nl = m.link_start[k0 + ncells] - L0;
     const int ga0 = m.col_ang_start[c];
     const int ninn = h.col_inn[c], nffc = h.col_ffc[c], ntr = m.col_ntrace[c];
+    const int es0 = h.col_es0[c], nes = h.col_nes[c];
     const bool restrict_ = (ops & RTX_OP_RESTRICT) != 0;
     const double x0 = m.col_geom[4 * c + 0], y0 = m.col_geom[4 * c + 1];
     const double dx = m.col_geom[4 * c + 2] - x0, dy = m.col_geom[4 * c + 3] - y0;
-    const HpLayout Lo = hp_layout<A, PREC>(nsp, na, ninn > nffc ? ninn : nffc, ntr);
+    const HpLayout Lo = hp_layout<A, PREC>(nsp, na, ninn > nffc ? ninn : nffc, ntr, nes);
     A *cth = reinterpret_cast<A *>(hp_lds + Lo.cth);
     A *sth = reinterpret_cast<A *>(hp_lds + Lo.sth);
     A *wr = reinterpret_cast<A *>(hp_lds + Lo.wr);
@@ -557,7 +567,7 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
         const int n1 = (F & 1) ? m.col_nx[c1] : m.col_ny[c1];
         sh.l_w[li] = F | (n1 << 2);
         sh.l_inn[li] = h.link_inn[l];
-        sh.l_es[li] = m.link_Es_off[l];
+        sh.l_es[li] = h.link_esl[l];
         sh.l_m1[li] = m.link_meta[4 * l + 1];
         sh.l_src[li] = m.link_meta[4 * l];
         sh.l_tr[li] = m.link_meta[4 * l + 2];
@@ -590,7 +600,7 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
 #define HP_CASE(NN)                                                                           \
     case NN:                                                                                  \
         hp_column<T, A, PREC, NN>(m, ops, accumulate, sh, hp_lds, c, k0, ncells, na, cdof, L0, \
-                                  nl, ninn, nffc, ntr, dx, dy, x, y);                         \
+                                  nl, ninn, nffc, ntr, es0, nes, h.es_pool, dx, dy, x, y);   \
         break;
         HP_CASE(3) HP_CASE(4) HP_CASE(5) HP_CASE(6) HP_CASE(7) HP_CASE(8) HP_CASE(9) HP_CASE(10)
 #undef HP_CASE
@@ -602,12 +612,12 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
 
 // Bytes of dynamic LDS a column needs (FP64 or FP32 storage, mixed adds the
 // BF16 copy).
-static size_t hp_bytes(int nsp, int na, int nin, int ntr, int prec) {
+static size_t hp_bytes(int nsp, int na, int nin, int ntr, int nes, int prec) {
     if (prec == RTX_PREC_F64)
-        return size_t(hp_layout<double, RTX_PREC_F64>(nsp, na, nin, ntr).total);
+        return size_t(hp_layout<double, RTX_PREC_F64>(nsp, na, nin, ntr, nes).total);
     if (prec == RTX_PREC_F32)
-        return size_t(hp_layout<float, RTX_PREC_F32>(nsp, na, nin, ntr).total);
-    return size_t(hp_layout<float, RTX_PREC_MIXED>(nsp, na, nin, ntr).total);
+        return size_t(hp_layout<float, RTX_PREC_F32>(nsp, na, nin, ntr, nes).total);
+    return size_t(hp_layout<float, RTX_PREC_MIXED>(nsp, na, nin, ntr, nes).total);
 }
 
 // Static LDS of k_hpcol (the tables above), for the capacity check.
@@ -618,6 +628,9 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
     p->hp_ok = false;
     std::vector<int32_t> link_inn(std::max(d->nlink, 1), 0), col_inn(nc, 0);
     std::vector<int32_t> cell_fo(std::max(d->ncell, 1), 0), col_ffc(nc, 0);
+    std::vector<int32_t> link_esl(std::max(d->nlink, 1), 0), col_es0(nc, 0), col_nes(nc, 0);
+    std::vector<double> es_pool;
+    std::vector<std::pair<int64_t, int>> seen;   // (pool offset, staged offset) of one column
     std::vector<char> fits(nc, 0);
     std::vector<size_t> need[3];
     for (int k = 0; k < 3; ++k) need[k].assign(nc, 0);
@@ -646,13 +659,35 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
             fo += __builtin_popcount(lf) * d->col_nx[c] * d->cell_nth[k];
         }
         col_ffc[c] = fo;
+        // the column's distinct E^s blocks (n0 x n1 each, keyed by pool
+        // offset), copied in link order
+        seen.clear();
+        col_es0[c] = int(es_pool.size());
+        int nes = 0;
+        for (int l = d->link_start[k0]; l < d->link_start[k1]; ++l) {
+            const int64_t off = d->link_Es_off[l];
+            int pos = -1;
+            for (const auto &sp : seen)
+                if (sp.first == off) { pos = sp.second; break; }
+            if (pos < 0) {
+                const int s = d->link_src[l], F = d->link_face[l], c1 = d->cell_col[s];
+                const int n0 = (F & 1) ? d->col_nx[c] : d->col_ny[c];
+                const int n1 = (F & 1) ? d->col_nx[c1] : d->col_ny[c1];
+                pos = nes;
+                seen.emplace_back(off, pos);
+                es_pool.insert(es_pool.end(), d->Es_pool + off, d->Es_pool + off + int64_t(n0) * n1);
+                nes += n0 * n1;
+            }
+            link_esl[l] = pos;
+        }
+        col_nes[c] = nes;
         const int nlc = d->link_start[k1] - d->link_start[k0];
         // square columns of HP_NMIN..RTX_NMAX nodes (the compiled node counts)
         bool ok = (k1 - k0) <= HP_MAX_CELLS && nlc <= HP_MAX_LINKS &&
                   d->col_nx[c] == d->col_ny[c] && d->col_nx[c] >= HP_NMIN && na <= 4096 &&
                   fo <= HP_FV * HP_THREADS;
         for (int pr = 0; pr < 3; ++pr) {
-            need[pr][c] = hp_bytes(nsp, na, std::max(o, fo), d->col_ntrace[c], pr);
+            need[pr][c] = hp_bytes(nsp, na, std::max(o, fo), d->col_ntrace[c], nes, pr);
             ok = ok && need[pr][c] <= cap;
         }
         fits[c] = ok ? 1 : 0;
@@ -682,7 +717,8 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
         for (int k = d->col_cell_start[c]; k < d->col_cell_start[c + 1]; ++k) gcells.push_back(k);
     const std::vector<int32_t> gitems = scatter_items(p, over);
     size_t total = link_inn.size() + col_inn.size() + cell_fo.size() + col_ffc.size() +
-                   gcells.size() + gitems.size();
+                   link_esl.size() + col_es0.size() + col_nes.size() + gcells.size() +
+                   gitems.size();
     for (int k = 0; k < HP_NCLS; ++k) total += lists[k].size();
     RTX_HIP(hipMalloc(&p->hp_alloc, total * sizeof(int32_t)));
     int32_t *base = static_cast<int32_t *>(p->hp_alloc);
@@ -698,6 +734,14 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
     p->hp.col_inn = put(col_inn);
     p->hp.cell_fo = put(cell_fo);
     p->hp.col_ffc = put(col_ffc);
+    p->hp.link_esl = put(link_esl);
+    p->hp.col_es0 = put(col_es0);
+    p->hp.col_nes = put(col_nes);
+    if (es_pool.empty()) es_pool.push_back(0.0);
+    RTX_HIP(hipMalloc(&p->hp_es, es_pool.size() * sizeof(double)));
+    RTX_HIP(hipMemcpy(p->hp_es, es_pool.data(), es_pool.size() * sizeof(double),
+                      hipMemcpyHostToDevice));
+    p->hp.es_pool = static_cast<const double *>(p->hp_es);
     for (int k = 0; k < HP_NCLS; ++k) {
         p->hp_cls[k].cols = put(lists[k]);
         p->hp_cls[k].n = int(lists[k].size());

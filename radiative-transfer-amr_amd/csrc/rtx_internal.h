@@ -90,6 +90,11 @@ struct HpTables {
     // per cell: column-local offset of its inflow-face sums; per column:
     // their count
     const int32_t *cell_fo = nullptr, *col_ffc = nullptr;
+    // E^s staging: per column, its distinct E^s blocks copied contiguously
+    // into es_pool from col_es0 (col_nes values); per link, the column-local
+    // offset of its block in that copy
+    const int32_t *link_esl = nullptr, *col_es0 = nullptr, *col_nes = nullptr;
+    const double *es_pool = nullptr;
 };
 struct HpClass {
     int n = 0;                      // columns
@@ -166,7 +171,7 @@ struct rtx_plan {
     std::vector<int32_t> hp_over_cols;
     const int32_t *hp_gen_cells = nullptr, *hp_gen_items = nullptr;
     int hp_n_gen_cells = 0, hp_n_gen_items = 0;
-    void *hp_alloc = nullptr;
+    void *hp_alloc = nullptr, *hp_es = nullptr;
     hipStream_t hp_side = nullptr;
     hipEvent_t hp_fork = nullptr, hp_join = nullptr;
     // host copies: per-column cell start (ncol + 1), spatial nodes, angles;

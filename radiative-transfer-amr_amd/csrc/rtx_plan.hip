@@ -534,6 +534,7 @@ extern "C" void rtx_plan_destroy(rtx_plan *p) {
     if (p->bins) (void)hipFree(p->bins);
     if (p->gen_alloc) (void)hipFree(p->gen_alloc);
     if (p->hp_alloc) (void)hipFree(p->hp_alloc);
+    if (p->hp_es) (void)hipFree(p->hp_es);
     if (p->hp_fork) (void)hipEventDestroy(p->hp_fork);
     if (p->hp_join) (void)hipEventDestroy(p->hp_join);
     if (p->hp_side) (void)hipStreamDestroy(p->hp_side);

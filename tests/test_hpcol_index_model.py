@@ -26,12 +26,12 @@ def hp_stride(na, kstep, c, M):
     return n + ((c - n) % M + M) % M
 
 
-def hp_layout(nsp, na, nin, ntr, ea, mixed):
+def hp_layout(nsp, na, nin, ntr, ea, mixed, nes=0):
     """(S, NA, XSB, total bytes): the angle-major copy has NA rows (angles,
     padded to the K step) of S = nsp rounded up to odd entries, in a region
     that first holds the ntr neighbour-trace values; mixed adds a node-major
     BF16 copy of row stride XSB; the inn region holds nin = max(ninn, nffc)
-    entries."""
+    entries, the es region the column's nes distinct E^s values."""
     kstep = 16 if mixed else 4
     S, NA = nsp | 1, (na + kstep - 1) // kstep * kstep
     XSB = hp_stride(na, 16, 8, 16) if mixed else 0
@@ -39,7 +39,7 @@ def hp_layout(nsp, na, nin, ntr, ea, mixed):
     o = (o + 15) & ~15
     o += nsp * XSB * 2
     o = (o + 15) & ~15
-    o += nin * ea + 3 * na * ea + na * 4
+    o += nin * ea + nes * ea + 3 * na * ea + na * 4
     return S, NA, XSB, (o + 15) & ~15
 
 
@@ -59,6 +59,33 @@ def face_tables(T):
             fo += bin(lf).count("1") * int(T.col_nx[c]) * int(T.cell_nth[k])
         col_ffc[c] = fo
     return cell_fo, col_ffc
+
+
+def es_tables(T):
+    """link_esl, col_es0, col_nes and the staged pool, as hpcol_build builds
+    them: each column's distinct E^s blocks (by pool offset, n0 x n1 values)
+    copied contiguously in link order."""
+    link_esl = np.zeros(max(T.nlink, 1), np.int64)
+    col_es0 = np.zeros(T.ncol, np.int64)
+    col_nes = np.zeros(T.ncol, np.int64)
+    pool = []
+    for c in range(T.ncol):
+        seen = {}
+        col_es0[c] = len(pool)
+        nes = 0
+        for l in range(T.link_start[T.col_cell_start[c]], T.link_start[T.col_cell_start[c + 1]]):
+            off = int(T.link_Es_off[l])
+            if off not in seen:
+                F = int(T.link_face[l])
+                c1 = T.cell_col[int(T.link_src[l])]
+                n0 = int(T.col_nx[c] if F & 1 else T.col_ny[c])
+                n1 = int(T.col_nx[c1] if F & 1 else T.col_ny[c1])
+                seen[off] = nes
+                pool.extend(T.Es_pool[off:off + n0 * n1])
+                nes += n0 * n1
+            link_esl[l] = seen[off]
+        col_nes[c] = nes
+    return link_esl, col_es0, col_nes, np.array(pool + [0.0])
 
 
 def admitted(T, c, col_ffc=None):
@@ -109,6 +136,7 @@ def _on_bdry(bd, i, j, nx, ny):
 def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024):
     link_inn, col_inn = host_tables(T)
     cell_fo, col_ffc = face_tables(T)
+    link_esl, col_es0, col_nes, es_pool = es_tables(T)
     y = np.full(T.ndof, np.nan)
     for c in range(T.ncol):
         assert admitted(T, c, col_ffc)
@@ -120,13 +148,14 @@ def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024)
         cdof = int(T.cell_dof_off[k0])
         L0, nl = int(T.link_start[k0]), int(T.link_start[k1] - T.link_start[k0])
         ninn, nffc = int(col_inn[c]), int(col_ffc[c])
-        nin = max(ninn, nffc)
+        nin, nes, es0 = max(ninn, nffc), int(col_nes[c]), int(col_es0[c])
+        es = es_pool[es0:es0 + nes]                  # the LDS copy
         assert ncells <= HP_MAX_CELLS and nl <= HP_MAX_LINKS and nsp <= HP_MT * 16
         ntr = int(T.col_ntrace[c])
         for ea, mixed in ((8, False), (4, False), (4, True)):
-            assert hp_layout(nsp, na, nin, ntr, ea, mixed)[3] <= lds_cap
-        S, NA, _, _ = hp_layout(nsp, na, nin, ntr, 8, False)
-        XSB = hp_layout(nsp, na, nin, ntr, 4, True)[2]
+            assert hp_layout(nsp, na, nin, ntr, ea, mixed, nes)[3] <= lds_cap
+        S, NA, _, _ = hp_layout(nsp, na, nin, ntr, 8, False, nes)
+        XSB = hp_layout(nsp, na, nin, ntr, 4, True, nes)[2]
         region = max(NA * S, ntr)
         x0, y0, x1, y1 = T.col_geom[c]
         dx, dy = x1 - x0, y1 - y0
@@ -242,12 +271,13 @@ def hpcol_model(T, kappa, sigma, G, G_off, x, restrict=True, lds_cap=160 * 1024)
                     continue
                 c1 = T.cell_col[int(T.link_src[l])]
                 n1 = int(T.col_nx[c1] if F & 1 else T.col_ny[c1])
-                es = int(T.link_Es_off[l]) + m0
+                eo = int(link_esl[l]) + m0
                 f = 0.0
                 for m1 in range(n1):
                     ii = l_inn[li] + m1 * nt + r
-                    assert es + m1 * nx < T.Es_pool.size and ii < ninn
-                    f += T.Es_pool[es + m1 * nx] * inn[ii]
+                    assert eo + m1 * nx < nes and ii < ninn
+                    assert es[eo + m1 * nx] == T.Es_pool[int(T.link_Es_off[l]) + m0 + m1 * nx]
+                    f += es[eo + m1 * nx] * inn[ii]
                 acc += fxy * f
             fv[e] = acc
         inn[:nffc] = fv[:nffc]
@@ -354,6 +384,7 @@ def test_hpcol_host_tables_fit_c3_and_c4():
         T = MeshTables(problems.random_hp_mesh(0, p_col=p_col))
         _, col_inn = host_tables(T)
         _, col_ffc = face_tables(T)
+        _, _, col_nes, _ = es_tables(T)
         for c in range(T.ncol):
             k0, k1 = T.col_cell_start[c], T.col_cell_start[c + 1]
             nsp, na = int(T.col_nx[c] * T.col_ny[c]), int(T.col_nang[c])
@@ -362,4 +393,5 @@ def test_hpcol_host_tables_fit_c3_and_c4():
             assert admitted(T, c, col_ffc)
             nin = max(int(col_inn[c]), int(col_ffc[c]))
             for ea, mixed in ((8, False), (4, False), (4, True)):
-                assert hp_layout(nsp, na, nin, int(T.col_ntrace[c]), ea, mixed)[3] <= 148 * 1024
+                assert hp_layout(nsp, na, nin, int(T.col_ntrace[c]), ea, mixed,
+                                 int(col_nes[c]))[3] <= 148 * 1024
