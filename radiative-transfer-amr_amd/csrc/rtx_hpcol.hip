@@ -48,7 +48,10 @@ constexpr int HP_MT = (RTX_NMAX * RTX_NMAX + 15) / 16;   // node tiles of 16
 constexpr int HP_XCD = 8;
 constexpr int HP_TG = 2;          // node tiles per scattering pass (accumulators in registers)
 constexpr int HP_NMIN = 3;        // node counts compiled for the column kernel
-constexpr int HP_FV = 8;          // inflow-face sums per thread (a column holds <= HP_FV * HP_THREADS)
+constexpr int HP_FV = 8;
+constexpr int HP_LINK_LOG = 8;    // 2^8 >= HP_MAX_LINKS
+constexpr int HP_CELL_LOG = 6;    // 2^6 >= HP_MAX_CELLS
+static_assert((1 << HP_LINK_LOG) >= HP_MAX_LINKS && (1 << HP_CELL_LOG) >= HP_MAX_CELLS, "search range");          // inflow-face sums per thread (a column holds <= HP_FV * HP_THREADS)
 
 typedef double hp_d4 __attribute__((ext_vector_type(4)));
 typedef float hp_f4 __attribute__((ext_vector_type(4)));
@@ -69,12 +72,16 @@ __device__ inline uint16_t hp_bf16(float f) {
     return uint16_t(u >> 16);
 }
 
-// Largest i in [lo, hi) with starts[i] <= t (starts ascending, starts[lo] <= t).
-__device__ inline int hp_find(const int *starts, int lo, int hi, int t) {
-    hi -= 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (starts[mid] <= t) lo = mid; else hi = mid - 1;
+// Largest i in [0, hi) with starts[i] <= t (starts ascending, starts[0] <= t,
+// hi <= 2^LOG).  A fixed number of steps, so the searches of an unrolled
+// loop's iterations are independent and their LDS reads overlap.
+template <int LOG>
+__device__ inline int hp_find(const int *starts, int hi, int t) {
+    int lo = 0;
+#pragma unroll
+    for (int s = 1 << (LOG - 1); s > 0; s >>= 1) {
+        const int mid = lo + s;
+        if (mid < hi && starts[mid] <= t) lo = mid;
     }
     return lo;
 }
@@ -196,7 +203,7 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
                 const int e = e0 + u * HP_THREADS;
                 addr[u] = -1;
                 if (e < ntr) {
-                    const int li = hp_find(sh.l_tr, 0, nl, e);
+                    const int li = hp_find<HP_LINK_LOG>(sh.l_tr, nl, e);
                     const int w1 = sh.l_m1[li];
                     const int F = w1 & 3, nx1 = (w1 >> 2) & 15, ny1 = (w1 >> 6) & 15;
                     const int nt1 = (w1 >> 10) & 15, bd1 = restrict_ ? ((w1 >> 14) & 15) : 0;
@@ -221,23 +228,32 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
             }
         }
         __syncthreads();
-        for (int e = tid; e < ninn; e += HP_THREADS) {
-            const int li = hp_find(sh.l_inn, 0, nl, e);
-            const int w1 = sh.l_m1[li];
-            const int nt1 = (w1 >> 10) & 15, kk = w1 >> 18;
-            const int nt = sh.c_nth[kk];
-            const int le = e - sh.l_inn[li], m1 = hp_div(le, 1.0f / float(nt)), r = le - m1 * nt;
-            const double *Et = m.Eth_pool + sh.l_et[li] + r;
-            const A *trl = tr + sh.l_tr[li] + m1 * nt1;
-            // the E^theta column's (at most RTX_NMAX) loads in flight together
-            A ev[HP_NM];
+        // two entries per thread and step, so that their searches and E^theta
+        // loads are in flight together
+        for (int e0 = tid; e0 < ninn; e0 += 2 * HP_THREADS) {
 #pragma unroll
-            for (int a = 0; a < HP_NM; ++a) ev[a] = (a < nt1) ? A(Et[a * nt]) : A(0);
-            A acc = A(0);
+            for (int u = 0; u < 2; ++u) {
+                const int e = e0 + u * HP_THREADS;
+                if (e < ninn) {
+                    const int li = hp_find<HP_LINK_LOG>(sh.l_inn, nl, e);
+                    const int w1 = sh.l_m1[li];
+                    const int nt1 = (w1 >> 10) & 15, kk = w1 >> 18;
+                    const int nt = sh.c_nth[kk];
+                    const int le = e - sh.l_inn[li], m1 = hp_div(le, 1.0f / float(nt));
+                    const int r = le - m1 * nt;
+                    const double *Et = m.Eth_pool + sh.l_et[li] + r;
+                    const A *trl = tr + sh.l_tr[li] + m1 * nt1;
+                    // the E^theta column's (at most RTX_NMAX) loads in flight together
+                    A ev[HP_NM];
 #pragma unroll
-            for (int a = 0; a < HP_NM; ++a)
-                if (a < nt1) acc += ev[a] * trl[a];
-            inn[e] = acc;
+                    for (int a = 0; a < HP_NM; ++a) ev[a] = (a < nt1) ? A(Et[a * nt]) : A(0);
+                    A acc = A(0);
+#pragma unroll
+                    for (int a = 0; a < HP_NM; ++a)
+                        if (a < nt1) acc += ev[a] * trl[a];
+                    inn[e] = acc;
+                }
+            }
         }
         __syncthreads();
         // inflow-face sums.  Entry (kk, h, m0, r): cell kk, the h-th of its
@@ -254,7 +270,7 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
             const int e = tid + u * HP_THREADS;
             fv[u] = A(0);
             if (e < nffc) {
-                const int kk = hp_find(sh.c_fo, 0, ncells, e);
+                const int kk = hp_find<HP_CELL_LOG>(sh.c_fo, ncells, e);
                 const int nt = sh.c_nth[kk], lf = sh.c_lf[kk];
                 const int le = e - sh.c_fo[kk];
                 const int row = hp_div(le, 1.0f / float(nt)), r = le - row * nt;
@@ -515,14 +531,12 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
     if (ci >= ncols) return;
     const int c = cols[ci];
     const int tid = threadIdx.x;
-    const int k0 = m.col_cell_start[c], ncells = m.col_cell_start[c + 1] - k0;
-    const int nx = m.col_nx[c], ny = m.col_ny[c], nsp = nx * ny;
-    const int na = m.col_nang[c];
-    const int cdof = m.cell_dof_off[k0];
-    const int L0 = m.link_start[k0], nl = m.link_start[k0 + ncells] - L0;
-    const int ga0 = m.col_ang_start[c];
-    const int ninn = h.col_inn[c], nffc = h.col_ffc[c], ntr = m.col_ntrace[c];
-    const int es0 = h.col_es0[c], nes = h.col_nes[c];
+    // the column's scalars from its header (square columns: ny = nx)
+    const int32_t *hd = h.col_hdr + size_t(c) * HP_HDR;
+    const int k0 = hd[0], ncells = hd[1], nx = hd[2], ny = nx, nsp = nx * ny;
+    const int na = hd[3], cdof = hd[4], L0 = hd[5], nl = hd[6], ga0 = hd[7];
+    const int ninn = hd[8], nffc = hd[9], ntr = hd[10], es0 = hd[11], nes = hd[12];
+    const int sp0 = hd[13];
     const bool restrict_ = (ops & RTX_OP_RESTRICT) != 0;
     const double x0 = m.col_geom[4 * c + 0], y0 = m.col_geom[4 * c + 1];
     const double dx = m.col_geom[4 * c + 2] - x0, dy = m.col_geom[4 * c + 3] - y0;
@@ -536,17 +550,14 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
     if (tid < ncells) {
         const int k = k0 + tid;
         const int nt = m.cell_nth[k], ao = m.cell_ang_off[k];
-        const int lb = m.link_start[k], le = m.link_start[k + 1];
         sh.c_nth[tid] = nt;
         sh.c_bd[tid] = restrict_ ? m.cell_bdry[k] : 0;
         sh.c_quad[tid] = m.cell_quad[k];
         sh.c_ao[tid] = ao;
-        sh.c_l0[tid] = lb - L0;
+        sh.c_l0[tid] = m.link_start[k] - L0;
         sh.c_fo[tid] = h.cell_fo[k];
         sh.c_dth[tid] = A(m.cell_geom[2 * k + 1] - m.cell_geom[2 * k]);
-        int lf = 0;
-        for (int l = lb; l < le; ++l) lf |= 1 << m.link_face[l];
-        sh.c_lf[tid] = lf;
+        sh.c_lf[tid] = h.cell_lf[k];
         for (int a = 0; a < nt; ++a) acell[ao + a] = tid | (a << 8);
     }
     if (tid == 0) {
@@ -563,9 +574,7 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
     }
     for (int li = tid; li < nl; li += HP_THREADS) {
         const int l = L0 + li;
-        const int s = m.link_src[l], F = m.link_face[l], c1 = m.cell_col[s];
-        const int n1 = (F & 1) ? m.col_nx[c1] : m.col_ny[c1];
-        sh.l_w[li] = F | (n1 << 2);
+        sh.l_w[li] = h.link_w[l];
         sh.l_inn[li] = h.link_inn[l];
         sh.l_es[li] = h.link_esl[l];
         sh.l_m1[li] = m.link_meta[4 * l + 1];
@@ -576,8 +585,8 @@ void k_hpcol(DevMesh m, HpTables h, uint32_t ops, int accumulate,
     if (tid < nsp) {
         const int i = tid / ny, j = tid - (tid / ny) * ny;
         const double w2 = m.lgl_w[nx * HP_NM + i] * m.lgl_w[ny * HP_NM + j];
-        sh.w2k[tid] = A(w2 * m.kappa[m.col_sp_off[c] + tid]);
-        sh.sgw[tid] = A(dx * dy * m.sigma[m.col_sp_off[c] + tid] * w2);
+        sh.w2k[tid] = A(w2 * m.kappa[sp0 + tid]);
+        sh.sgw[tid] = A(dx * dy * m.sigma[sp0 + tid] * w2);
     }
     if (tid < HP_NM * HP_NM) {
         const int pp = tid / HP_NM, ii = tid % HP_NM;
@@ -630,6 +639,8 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
     std::vector<int32_t> cell_fo(std::max(d->ncell, 1), 0), col_ffc(nc, 0);
     std::vector<int32_t> link_esl(std::max(d->nlink, 1), 0), col_es0(nc, 0), col_nes(nc, 0);
     std::vector<double> es_pool;
+    std::vector<int32_t> col_hdr(size_t(nc) * HP_HDR, 0), cell_lf(std::max(d->ncell, 1), 0);
+    std::vector<int32_t> link_w(std::max(d->nlink, 1), 0);
     std::vector<std::pair<int64_t, int>> seen;   // (pool offset, staged offset) of one column
     std::vector<char> fits(nc, 0);
     std::vector<size_t> need[3];
@@ -654,7 +665,12 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
         int fo = 0;
         for (int k = k0; k < k1; ++k) {
             int lf = 0;
-            for (int l = d->link_start[k]; l < d->link_start[k + 1]; ++l) lf |= 1 << d->link_face[l];
+            for (int l = d->link_start[k]; l < d->link_start[k + 1]; ++l) {
+                const int F = d->link_face[l], c1 = d->cell_col[d->link_src[l]];
+                lf |= 1 << F;
+                link_w[l] = F | (((F & 1) ? d->col_nx[c1] : d->col_ny[c1]) << 2);
+            }
+            cell_lf[k] = lf;
             cell_fo[k] = fo;
             fo += __builtin_popcount(lf) * d->col_nx[c] * d->cell_nth[k];
         }
@@ -682,6 +698,21 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
         }
         col_nes[c] = nes;
         const int nlc = d->link_start[k1] - d->link_start[k0];
+        int32_t *hc = col_hdr.data() + size_t(c) * HP_HDR;
+        hc[0] = k0;
+        hc[1] = k1 - k0;
+        hc[2] = d->col_nx[c];
+        hc[3] = na;
+        hc[4] = d->cell_dof_off[k0];
+        hc[5] = d->link_start[k0];
+        hc[6] = nlc;
+        hc[7] = d->col_ang_start[c];
+        hc[8] = o;
+        hc[9] = fo;
+        hc[10] = d->col_ntrace[c];
+        hc[11] = col_es0[c];
+        hc[12] = nes;
+        hc[13] = d->col_sp_off[c];
         // square columns of HP_NMIN..RTX_NMAX nodes (the compiled node counts)
         bool ok = (k1 - k0) <= HP_MAX_CELLS && nlc <= HP_MAX_LINKS &&
                   d->col_nx[c] == d->col_ny[c] && d->col_nx[c] >= HP_NMIN && na <= 4096 &&
@@ -717,8 +748,8 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
         for (int k = d->col_cell_start[c]; k < d->col_cell_start[c + 1]; ++k) gcells.push_back(k);
     const std::vector<int32_t> gitems = scatter_items(p, over);
     size_t total = link_inn.size() + col_inn.size() + cell_fo.size() + col_ffc.size() +
-                   link_esl.size() + col_es0.size() + col_nes.size() + gcells.size() +
-                   gitems.size();
+                   link_esl.size() + col_es0.size() + col_nes.size() + col_hdr.size() +
+                   cell_lf.size() + link_w.size() + gcells.size() + gitems.size();
     for (int k = 0; k < HP_NCLS; ++k) total += lists[k].size();
     RTX_HIP(hipMalloc(&p->hp_alloc, total * sizeof(int32_t)));
     int32_t *base = static_cast<int32_t *>(p->hp_alloc);
@@ -737,6 +768,9 @@ int hpcol_build(rtx_plan *p, const rtx_mesh_desc *d) {
     p->hp.link_esl = put(link_esl);
     p->hp.col_es0 = put(col_es0);
     p->hp.col_nes = put(col_nes);
+    p->hp.col_hdr = put(col_hdr);
+    p->hp.cell_lf = put(cell_lf);
+    p->hp.link_w = put(link_w);
     if (es_pool.empty()) es_pool.push_back(0.0);
     RTX_HIP(hipMalloc(&p->hp_es, es_pool.size() * sizeof(double)));
     RTX_HIP(hipMemcpy(p->hp_es, es_pool.data(), es_pool.size() * sizeof(double),

@@ -95,7 +95,14 @@ struct HpTables {
     // offset of its block in that copy
     const int32_t *link_esl = nullptr, *col_es0 = nullptr, *col_nes = nullptr;
     const double *es_pool = nullptr;
+    // per column a header of HP_HDR ints (first cell, cells, nodes per
+    // direction, angles, first DOF, first link, links, first angle, the
+    // inn / face-sum / trace / E^s counts, E^s start, first spatial node), so
+    // the kernel reads a column's scalars in one step; per cell the mask of
+    // faces that carry links; per link face | (neighbour face nodes << 2)
+    const int32_t *col_hdr = nullptr, *cell_lf = nullptr, *link_w = nullptr;
 };
+constexpr int HP_HDR = 16;
 struct HpClass {
     int n = 0;                      // columns
     const int32_t *cols = nullptr;  // device list (key order)

@@ -395,3 +395,27 @@ def test_hpcol_host_tables_fit_c3_and_c4():
             for ea, mixed in ((8, False), (4, False), (4, True)):
                 assert hp_layout(nsp, na, nin, int(T.col_ntrace[c]), ea, mixed,
                                  int(col_nes[c]))[3] <= 148 * 1024
+
+
+def _hp_find(starts, hi, t, log):
+    """The kernel's fixed-step search, step for step."""
+    lo, s = 0, 1 << (log - 1)
+    while s > 0:
+        mid = lo + s
+        if mid < hi and starts[mid] <= t:
+            lo = mid
+        s >>= 1
+    return lo
+
+
+def test_fixed_step_search_is_the_largest_start_at_or_below():
+    """Largest i < hi with starts[i] <= t, for ascending starts with repeats
+    (cells without face sums repeat their offset), at every hi up to 2^log."""
+    rng = np.random.default_rng(7)
+    for log, hmax in ((8, 192), (6, 64)):
+        for _ in range(200):
+            hi = int(rng.integers(1, hmax + 1))
+            starts = np.concatenate([[0], np.cumsum(rng.integers(0, 4, hi - 1))]).astype(int)
+            for t in range(int(starts[-1]) + 3):
+                ref = int(np.searchsorted(starts, t, side="right") - 1)
+                assert _hp_find(starts, hi, t, log) == ref
