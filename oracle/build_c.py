@@ -16,13 +16,15 @@ OUT = os.path.join(HERE, "liboracle_spmv.so")
 
 def build(verbose=False):
     # no FMA contraction: row sums round like scipy's csr_matvec
-    cmd = ["gcc", "-O3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", SRC, "-o", OUT + ".tmp"]
+    # a per-process temporary name: parallel test workers may build at once
+    tmp = "{}.{}.tmp".format(OUT, os.getpid())
+    cmd = ["gcc", "-O3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", SRC, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("gcc failed for {}:\n{}\n{}".format(SRC, r.stdout, r.stderr))
-    os.replace(OUT + ".tmp", OUT)
+    os.replace(tmp, OUT)
     return OUT
 
 
