@@ -163,18 +163,23 @@ int rtx_set_path(rtx_plan *plan, int mode);
    assembly has no launch shapes). */
 int rtx_autotune(rtx_plan *plan, int precision, const void *d_x, void *d_y, int reps,
                  void *stream);
-/* Force the uniform-bin kernel's parts per column (1, 2 or 4 where valid for
-   the plan's shape; 0 restores the default).  No counterpart in the
-   reference. */
+/* Uniform-bin kernel: parts = 1, 2 or 4 (where valid for the plan's shape)
+   forces the line-pass kernel with that many parts per column; 0 restores
+   the default: in FP64 the output-layout kernel (one column per workgroup,
+   every term of an output added where the scattering MFMA leaves it) when
+   all columns are resident at once (four per CU), else the line-pass kernel
+   with one part; in FP32 and mixed the line-pass kernel with one part.  No
+   counterpart in the reference. */
 int rtx_set_uniform_parts(rtx_plan *plan, int parts);
 /* Run fused order bin `bin` (0..5) on the generic kernels (generic = 1) or
    on the fused kernel (0); rebuilds the generic lists (synchronises the
    device).  No counterpart in the reference. */
 int rtx_set_bin_generic(rtx_plan *plan, int bin, int generic);
-/* Launch-shape summary (24 values): uniform bin usable, its parts per column;
+/* Launch-shape summary (25 values): uniform bin usable, its parts per column;
    parts, columns and generic flag of the six fused order bins (part count by
    LDS class); columns and cells on the generic kernels; whether the hp
-   column kernel is the default path, and the columns it takes.  Returns the
+   column kernel is the default path, and the columns it takes; whether the
+   uniform bin's FP64 apply runs the output-layout kernel.  Returns the
    count. */
 int rtx_plan_info(const rtx_plan *plan, int64_t *out, int n);
 

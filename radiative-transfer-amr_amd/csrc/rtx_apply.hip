@@ -537,8 +537,9 @@ extern "C" int rtx_autotune(rtx_plan *p, int precision, const void *d_x, void *d
         const int mask = uniform_valid_parts(p);
         float best = 1e30f;
         int bestH = p->uni_H;
-        for (int H : {1, 2, 4}) {
-            if (!(mask & H)) continue;
+        // 0: the output-layout kernel; 1, 2, 4: the line-pass kernel's parts
+        for (int H : {0, 1, 2, 4}) {
+            if (H && !(mask & H)) continue;
             p->uni_H = H;
             const float us = time_launches(
                 [&]() { return launch_uniform(p, ops, precision, d_x, d_y, st, 0); }, reps, st, &rc);
@@ -627,7 +628,10 @@ extern "C" int rtx_set_bin_generic(rtx_plan *p, int bin, int generic) {
     return rebuild_generic(p);
 }
 
-/* Force the uniform kernel's parts per column (1, 2, 4; 0 = default). */
+/* Force the line-pass uniform kernel with 1, 2 or 4 parts per column; 0
+   restores the default (FP64: the output-layout kernel, one column per
+   workgroup, when every column is resident at once, else the line-pass
+   kernel; FP32 and mixed: the line-pass kernel with one part). */
 extern "C" int rtx_set_uniform_parts(rtx_plan *p, int H) {
     RTX_CHECK(p, "rtx_set_uniform_parts: null plan");
     RTX_CHECK(H == 0 || (p->uni_ok && (H == 1 || H == 2 || H == 4) &&
@@ -640,10 +644,12 @@ extern "C" int rtx_set_uniform_parts(rtx_plan *p, int H) {
 /* Launch-shape summary: out[0] uniform bin usable, [1] its parts per column,
    [2 + 3 b .. 4 + 3 b] parts, columns and generic flag of fused bin b
    (b < 6), [20], [21] columns and cells on the generic kernels, [22] the hp
-   column kernel is the default path, [23] columns it takes.  Returns the
-   count (24). */
+   column kernel is the default path, [23] columns it takes, [24] the uniform
+   bin's FP64 apply runs the output-layout kernel (1: the default when every
+   column is resident at once, four per CU) or the line-pass kernel (0).
+   Returns the count (25). */
 extern "C" int rtx_plan_info(const rtx_plan *p, int64_t *out, int n) {
-    constexpr int NV = 6 + 3 * RTX_NBINS;
+    constexpr int NV = 7 + 3 * RTX_NBINS;
     if (!p) return 0;
     int64_t v[NV] = {};
     v[0] = p->uni_ok ? 1 : 0;
@@ -659,6 +665,7 @@ extern "C" int rtx_plan_info(const rtx_plan *p, int64_t *out, int n) {
     int hp_cols = 0;
     for (int k = 0; k < HP_NCLS; ++k) hp_cols += p->hp_cls[k].n;
     v[23] = hp_cols;
+    v[24] = (p->uni_ok && p->uni_H == 0 && p->m.ncol <= 4 * p->n_cu) ? 1 : 0;
     for (int i = 0; i < NV && i < n && out; ++i) out[i] = v[i];
     return NV;
 }

@@ -138,6 +138,7 @@ struct FusedBin {
 
 struct rtx_plan {
     int device = 0;
+    int n_cu = 0;                // compute units of the device
     rtx::DevMesh m{};
     void *arena = nullptr;       // one device allocation for all tables
     size_t arena_bytes = 0;

@@ -167,6 +167,8 @@ extern "C" int rtx_plan_create(const rtx_mesh_desc *d, rtx_plan **out) {
 
     auto *p = new rtx_plan();
     hipError_t he = hipGetDevice(&p->device);
+    if (he == hipSuccess)
+        he = hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, p->device);
     if (he != hipSuccess) {
         set_error("hipGetDevice: %s", hipGetErrorString(he));
         delete p;

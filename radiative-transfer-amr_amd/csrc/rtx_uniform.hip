@@ -8,6 +8,11 @@
 // no searches, constant divisions, batched register prefetch of the column,
 // the part's neighbour traces and the MFMA B operands, and XCD-aware
 // (column, part) blocks: each XCD owns a contiguous run of columns.
+//
+// Two kernels: k_uniform adds the cell-local terms by lines of nodes into an
+// output tile in LDS (1, 2 or 4 parts per column); k_uniform_e computes
+// every term of an output in the scattering MFMA's accumulator layout, with
+// no output tile, so more columns are resident per CU (the FP64 default).
 #include <type_traits>
 
 #include "rtx_internal.h"
@@ -539,6 +544,435 @@ __global__ __launch_bounds__(UTHREADS, 4) void k_uniform(DevMesh m, uint32_t ops
     }
 }
 
+// Waves of the output-layout kernel: one per 16-wide output-angle tile, at
+// least four (the loads and the trace and flux passes use every thread).
+template <int NTH, int NC>
+constexpr int ue_waves() {
+    return (NC * NTH + 15) / 16 > 4 ? (NC * NTH + 15) / 16 : 4;
+}
+
+// The same operator on one whole column per workgroup, with every term of an
+// output computed where the scattering MFMA leaves it: lane (l16, lg) of wave
+// w owns output angle R = 16 w + l16 and the nodes of its accumulator rows,
+// and adds to the MFMA result the mass, convection, outflow and inflow-face
+// terms of those (node, angle) outputs in the order k_uniform adds them (so
+// the two kernels agree bit for bit).  No output tile in LDS: the column
+// tile, the traces and the flux slots are all the LDS it needs, so more
+// columns are resident per CU (C2: 34 KB, four per CU, every column of the
+// mesh in one round).  The outputs go through the dead column tile for a
+// coalesced store.
+template <typename T, typename A, int PREC, int N, int NTH, int NC,
+          int ETHREADS = 64 * ue_waves<NTH, NC>()>
+// at least as many waves per SIMD as the workgroup has waves: four columns
+// per CU (C2: 5 waves per SIMD, at most 102 VGPRs)
+__global__ __launch_bounds__(ETHREADS, ETHREADS / 64) void k_uniform_e(DevMesh m, uint32_t ops, int accumulate,
+                                                        const T *__restrict__ x,
+                                                        T *__restrict__ y, int nlink_lds) {
+    constexpr int NSP = N * N, NA = NC * NTH, NCD = NSP * NA, CD = NSP * NTH;
+    constexpr int MT = (NSP + 15) / 16;
+    constexpr int NTILE = (NA + 15) / 16;
+    constexpr int NKS = (NA + 3) / 4;
+    constexpr int NKB = (NA + 15) / 16;
+    constexpr int MAXL = 4 * NC;
+    constexpr int TRL = N * NTH;
+    constexpr int CPP = CD <= ETHREADS ? ETHREADS / CD : 0;
+    constexpr int XS = CPP ? CPP * CD : ETHREADS;
+    constexpr int XU = (NCD + XS - 1) / XS;
+    constexpr int LPP = ETHREADS / TRL;
+    constexpr int TS = LPP * TRL;
+    constexpr int NAP = u_pad(NA, NTH);
+    static_assert(NTILE <= ETHREADS / 64, "one output-angle tile per wavefront");
+    static_assert(NSP * NAP >= NCD, "the column tile stages the outputs");
+    extern __shared__ __align__(16) unsigned char u_lds[];
+    A *xs = reinterpret_cast<A *>(u_lds);      // [NSP][NAP]: column x, angle A of node s
+    constexpr bool XB = PREC == RTX_PREC_MIXED;
+    constexpr int NAPB = u_bf_stride(NA);
+    constexpr size_t XB_OFF = u_bf_off(size_t(NSP * NAP) * sizeof(A));
+    uint16_t *xb = reinterpret_cast<uint16_t *>(u_lds + XB_OFF);
+    A *tr = XB ? reinterpret_cast<A *>(u_lds + XB_OFF + u_bf_bytes(NSP, NA)) : xs + NSP * NAP;
+    A *fl = tr + nlink_lds * TRL;
+    __shared__ A cth[NA], sth[NA], cw[NA];
+    __shared__ A kap[NSP], sgw[NSP];
+    __shared__ A wx_[N], WD[N * N];
+    __shared__ A dth_[NC];
+    __shared__ int cbd[NC], cqd[NC];
+    __shared__ int nfc[NSP];
+    __shared__ int l_src[MAXL], l_meta[MAXL], l_es[MAXL], l_et[MAXL];
+    __shared__ int cfx[NC], cfy[NC];
+
+    // XCD j takes the contiguous columns [j cpx, (j + 1) cpx)
+    const int b = blockIdx.x;
+    const int cpx = (m.ncol + UXCD - 1) / UXCD;
+    const int c = (b % UXCD) * cpx + b / UXCD;
+    if (c >= m.ncol) return;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int k0 = c * NC;
+    const int cdof = c * NCD;
+    const int L0 = m.link_start[k0];
+    const int nl = m.link_start[k0 + NC] - L0;
+    const bool restrict_ = (ops & RTX_OP_RESTRICT) != 0;
+    const bool bconv = (ops & RTX_OP_BCONV) != 0;
+    const double dx = m.col_geom[4 * c + 2] - m.col_geom[4 * c + 0];
+    const double dy = m.col_geom[4 * c + 3] - m.col_geom[4 * c + 1];
+    const int ga0 = c * NA;
+
+    // ---- batched loads issued before any use
+    typedef typename std::conditional<PREC == RTX_PREC_F64, double, float>::type EB;
+    EB bpre[(PREC == RTX_PREC_MIXED) ? 1 : NKS];
+    const int l16 = lane & 15, lg = lane >> 4;
+    const int R = wave * 16 + l16;             // output angle of this lane
+    const bool rok = (wave < NTILE) && (R < NA);
+    constexpr bool MFMA_FIRST = PREC == RTX_PREC_F64;
+    auto load_b = [&]() {
+        if constexpr (PREC != RTX_PREC_MIXED) {
+            const EB *Gp = (PREC == RTX_PREC_F64) ? (const EB *)(m.G + m.col_G_off[c])
+                                                  : (const EB *)(m.G_f + m.col_G_off[c]);
+            const bool scat = (ops & RTX_OP_SCAT) != 0;
+            if constexpr (NA % 16 == 0) {
+                if (scat && wave < NTILE) {
+                    const EB *Gl = Gp + size_t(lg) * NA + R;
+#pragma unroll
+                    for (int u = 0; u < NKS; ++u) bpre[u] = Gl[size_t(u) * 4 * NA];
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < NKS; ++u) {
+                    const int Ak = u * 4 + lg;
+                    bpre[u] = (scat && rok && Ak < NA) ? Gp[size_t(Ak) * NA + R] : EB(0);
+                }
+            }
+        }
+    };
+    if constexpr (!MFMA_FIRST) load_b();
+    const int t0 = tid < XS ? tid : NCD;
+    A xv[XU];
+#pragma unroll
+    for (int u = 0; u < XU; ++u) {
+        const int t = t0 + u * XS;
+        xv[u] = (t < NCD) ? A(x[cdof + t]) : A(0);
+    }
+    double r_c = 0, r_s = 0, r_w = 0, r_kap = 0, r_sig = 0, r_wd = 0, r_g = 0;
+    int r_bd = 0, r_q = -1, r_src = 0, r_meta = 0, r_es = 0, r_et = 0;
+    if (tid < NA) {
+        r_c = m.ang_trig[3 * (ga0 + tid)];
+        r_s = m.ang_trig[3 * (ga0 + tid) + 1];
+        r_w = m.ang_trig[3 * (ga0 + tid) + 2];
+    }
+    if (tid < NSP) {
+        r_kap = m.kappa[c * NSP + tid];
+        r_sig = m.sigma[c * NSP + tid];
+    }
+    if (tid < N * N) {
+        const int p = tid / N, i = tid % N;
+        r_wd = m.lgl_w[N * UNM + i] * m.lgl_D[(size_t(N) * UNM + p) * UNM + i];
+    }
+    if (tid < NC) {
+        const int k = k0 + tid;
+        r_bd = m.cell_bdry[k];
+        r_q = m.cell_quad[k];
+        r_g = m.cell_geom[2 * k + 1] - m.cell_geom[2 * k];
+    }
+    if (bconv && tid < nl) {
+        const int l = L0 + tid;
+        r_src = m.link_meta[4 * l + 0];
+        r_meta = m.link_meta[4 * l + 1];
+        r_es = m.link_Es_off[l];
+        r_et = m.link_Eth_off[l];
+    }
+    if (tid < NA) {
+        cth[tid] = A(r_c);
+        sth[tid] = A(r_s);
+        cw[tid] = A(r_w);
+    }
+    if (tid < NSP) {
+        kap[tid] = A(r_kap);
+        sgw[tid] = A(dx * dy * m.lgl_w[N * UNM + tid / N] * m.lgl_w[N * UNM + tid % N] * r_sig);
+    }
+    if (tid < N * N) WD[tid] = A(r_wd);
+    if (tid < N) wx_[tid] = A(m.lgl_w[N * UNM + tid]);
+    if (tid < NSP) nfc[tid] = u_node_faces(tid / N, tid % N, N);
+    if (tid < NC) {
+        cbd[tid] = restrict_ ? r_bd : 0;
+        cqd[tid] = r_q;
+        dth_[tid] = A(r_g);
+        cfx[tid] = 0;
+        cfy[tid] = 0;
+    }
+    if (bconv && tid < nl) {
+        l_src[tid] = r_src;
+        l_meta[tid] = r_meta;
+        l_es[tid] = r_es;
+        l_et[tid] = r_et;
+    }
+    __syncthreads();
+    if (bconv && tid < nl) {
+        const int kl = r_meta >> 18, F = r_meta & 3;   // column-local target cell
+        const int slots = (tid * TRL) << 4;
+        if (F & 1) cfy[kl] = slots | (1 + (F == 1 ? N - 1 : 0));
+        else cfx[kl] = slots | (1 + (F == 0 ? N - 1 : 0));
+    }
+
+    // ---- phase 1: column input (masked) and the neighbour traces
+    {
+        const int lt0 = t0 % CD, sn0 = lt0 / NTH, a0 = lt0 % NTH, nf0 = nfc[sn0 < NSP ? sn0 : 0];
+#pragma unroll
+        for (int u = 0; u < XU; ++u) {
+            const int t = t0 + u * XS;
+            if (t < NCD) {
+                A v = xv[u];
+                int kk, sn, a, nf;
+                if constexpr (CPP > 0) {
+                    kk = t0 / CD + u * CPP;
+                    sn = sn0;
+                    a = a0;
+                    nf = nf0;
+                } else {
+                    const int lt = t % CD;
+                    kk = t / CD;
+                    sn = lt / NTH;
+                    a = lt % NTH;
+                    nf = nfc[sn];
+                }
+                if (cbd[kk] & nf) v = A(0);
+                xs[sn * NAP + kk * NTH + a] = v;
+                if constexpr (XB) xb[sn * NAPB + kk * NTH + a] = u_bf16(float(v));
+            }
+        }
+    }
+    if constexpr (XB) {
+        constexpr int PC = NKB * 16 - NA, PR = (MT * 16 - NSP) * NKB * 16;
+        if constexpr (PC > 0)
+            for (int e = tid; e < NSP * PC; e += ETHREADS) xb[(e / PC) * NAPB + NA + e % PC] = 0;
+        for (int e = tid; e < PR; e += ETHREADS)
+            xb[(NSP + e / (NKB * 16)) * NAPB + e % (NKB * 16)] = 0;
+    }
+    if (bconv) {
+        constexpr int TU = (MAXL * TRL + TS - 1) / TS;
+        const int e0 = tid < TS ? tid : MAXL * TRL;
+        const int le = e0 % TRL, m1 = le / NTH, a = le % NTH;
+        int addr[TU];
+        bool zero[TU];
+        A v[TU];
+#pragma unroll
+        for (int u = 0; u < TU; ++u) {
+            const int li = e0 / TRL + u * LPP;
+            addr[u] = -1;
+            zero[u] = false;
+            if (li < nl) {
+                const int w1 = l_meta[li];
+                const int F = w1 & 3, bd1 = restrict_ ? ((w1 >> 14) & 15) : 0;
+                int i1, j1;
+                if (F == 0) { i1 = 0; j1 = m1; }
+                else if (F == 2) { i1 = N - 1; j1 = m1; }
+                else if (F == 1) { i1 = m1; j1 = 0; }
+                else { i1 = m1; j1 = N - 1; }
+                addr[u] = l_src[li] + (i1 * N + j1) * NTH + a;
+                zero[u] = (bd1 & nfc[i1 * N + j1]) != 0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TU; ++u) v[u] = (addr[u] >= 0) ? A(x[addr[u]]) : A(0);
+#pragma unroll
+        for (int u = 0; u < TU; ++u) {
+            const int li = e0 / TRL + u * LPP;
+            if (li < nl) tr[li * TRL + le] = zero[u] ? A(0) : v[u];
+        }
+    }
+    if constexpr (MFMA_FIRST) load_b();
+    __syncthreads();
+
+    // ---- scattering on MFMA into registers: FP64 issues it first, FP32 and
+    // BF16 after the flux pass (as k_uniform)
+    typedef typename std::conditional<PREC == RTX_PREC_F64, u_dbl4, u_flt4>::type acc4;
+    acc4 sacc[MT];
+    const bool scat_w = (ops & RTX_OP_SCAT) && wave < NTILE;
+#pragma unroll
+    for (int t = 0; t < MT; ++t) sacc[t] = acc4{0, 0, 0, 0};
+    auto scatter = [&]() {
+        if constexpr (PREC == RTX_PREC_F64 || PREC == RTX_PREC_F32) {
+            constexpr bool TWO = !MFMA_FIRST;
+            acc4 acc2[MT];
+#pragma unroll
+            for (int t = 0; t < MT; ++t) acc2[t] = acc4{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks) {
+                const int Ak = ks * 4 + lg;
+#pragma unroll
+                for (int t = 0; t < MT; ++t) {
+                    const int s = t * 16 + l16;
+                    const EB a = (Ak < NA && s < NSP) ? EB(xs[s * NAP + Ak]) : EB(0);
+                    acc4 &dst = (TWO && (ks & 1)) ? acc2[t] : sacc[t];
+                    if constexpr (PREC == RTX_PREC_F64)
+                        dst = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bpre[ks], dst, 0, 0, 0);
+                    else
+                        dst = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bpre[ks], dst, 0, 0, 0);
+                }
+            }
+            if constexpr (TWO) {
+#pragma unroll
+                for (int t = 0; t < MT; ++t) sacc[t] += acc2[t];
+            }
+        } else {
+            const uint16_t *G = m.G_bf16 + m.col_G_off[c];
+            u_sht4 bv[NKB];
+#pragma unroll
+            for (int ks = 0; ks < NKB; ++ks)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int Ak = ks * 16 + 4 * lg + u;
+                    bv[ks][u] = (rok && Ak < NA) ? short(G[size_t(Ak) * NA + R]) : short(0);
+                }
+#pragma unroll
+            for (int ks = 0; ks < NKB; ++ks) {
+#pragma unroll
+                for (int t = 0; t < MT; ++t) {
+                    const int s = t * 16 + l16;
+                    const u_sht4 av = *reinterpret_cast<const u_sht4 *>(xb + s * NAPB + ks * 16 + 4 * lg);
+                    sacc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[ks], sacc[t], 0, 0, 0);
+                }
+            }
+        }
+    };
+    if (MFMA_FIRST && scat_w) scatter();
+
+    // ---- upwind inflow couplings, one (link, face node, angle) per thread,
+    // into a flux slot of its own (as k_uniform's phase 3)
+    if (bconv) {
+        const int f0 = tid < TS ? tid : MAXL * TRL;
+        const int le = f0 % TRL, m0 = le / NTH, r = le % NTH;
+        for (int li = f0 / TRL; li < nl; li += LPP) {
+            const int w1 = l_meta[li];
+            const int F = w1 & 3, kk = w1 >> 18;      // column-local target cell
+            const double *Es = m.Es_pool + l_es[li];
+            const double *Et = m.Eth_pool + l_et[li];
+            A et[NTH], es[N];
+#pragma unroll
+            for (int a = 0; a < NTH; ++a) et[a] = A(Et[a * NTH + r]);
+#pragma unroll
+            for (int a = 0; a < N; ++a) es[a] = A(Es[a * N + m0]);
+            const A *trl = tr + li * TRL;
+            A acc = A(0);
+#pragma unroll 1
+            for (int m1 = 0; m1 < N; ++m1) {
+                A inner = A(0);
+#pragma unroll
+                for (int a = 0; a < NTH; ++a) inner += et[a] * trl[m1 * NTH + a];
+                acc += es[m1] * inner;
+            }
+            fl[li * TRL + le] = A(((F & 1) ? dx : dy) * 0.25) * dth_[kk] * acc;
+        }
+    }
+    if (!MFMA_FIRST && scat_w) scatter();
+    __syncthreads();
+
+    // ---- every term of this lane's outputs, in k_uniform's order: mass,
+    // x-derivative, outflow faces (phase 2 x-lines), y-derivative (y-lines),
+    // scattering (phase 4), inflow faces x then y (phase 5)
+    A outv[MT][4];
+    if (wave < NTILE) {
+        const int Rc = rok ? R : NA - 1;
+        const int kk = Rc / NTH, r = Rc % NTH;
+        const A dth = dth_[kk], wr = cw[Rc], c_ = cth[Rc], s_ = sth[Rc];
+        const int quad = cqd[kk];
+        const A cm = A(dx * dy * 0.125);
+        const bool obc = bconv && quad >= 0;
+        const bool yfirst = (quad & 1) != 0;
+        const int Fx = yfirst ? ((quad + 1) & 3) : quad, Fy = yfirst ? quad : ((quad + 1) & 3);
+        const A xtf = Fx == 0 ? c_ : -c_, ytf = Fy == 1 ? s_ : -s_;
+        const A yb = A(dx * 0.25) * dth;
+        const int xtg = Fx == 0 ? N - 1 : 0;
+        const int fxs = cfx[kk], fys = cfy[kk];
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int s = t * 16 + ((PREC == RTX_PREC_F64) ? (lg + 4 * v) : (4 * lg + v));
+                A acc = A(0);
+                if (s < NSP) {
+                    const int pp = s / N, q = s % N;
+                    const A xn = xs[s * NAP + Rc];
+                    if (ops & RTX_OP_MASS) acc += cm * dth * wx_[pp] * wx_[q] * wr * kap[pp * N + q] * xn;
+                    if (ops & RTX_OP_QUADW) acc += cm * dth * wx_[pp] * wx_[q] * wr * xn;
+                    if (ops & RTX_OP_ICONV) {
+                        const A cx = A(dy * 0.25) * dth * wx_[q] * wr * c_;
+                        A sx = A(0);
+#pragma unroll
+                        for (int i = 0; i < N; ++i) sx += WD[pp * N + i] * xs[(i * N + q) * NAP + Rc];
+                        acc -= cx * sx;
+                    }
+                    if (obc) {
+                        const A xbv = A(dy * 0.25) * dth * wx_[q] * wr * xtf;
+                        const bool yon = q == (Fy == 1 ? N - 1 : 0);
+                        const A yt = yb * wx_[pp] * wr * ytf * xn;
+                        if (yfirst && yon) acc += yt;
+                        if (pp == xtg) acc += xbv * xn;
+                        if (!yfirst && yon) acc += yt;
+                    }
+                    if (ops & RTX_OP_ICONV) {
+                        const A cy = A(dx * 0.25) * dth * wx_[pp] * wr * s_;
+                        A sy = A(0);
+#pragma unroll
+                        for (int j = 0; j < N; ++j) sy += WD[q * N + j] * xs[(pp * N + j) * NAP + Rc];
+                        acc -= cy * sy;
+                    }
+                    if (ops & RTX_OP_SCAT) acc -= sgw[s] * A(sacc[t][v]);
+                    if (bconv) {
+                        if (pp + 1 == (fxs & 15)) acc += fl[(fxs >> 4) + q * NTH + r];
+                        if (q + 1 == (fys & 15)) acc += fl[(fys >> 4) + pp * NTH + r];
+                    }
+                }
+                outv[t][v] = acc;
+            }
+        }
+    }
+    __syncthreads();
+    // the outputs into the dead column tile in DOF order (cell, node, angle)
+    if (rok) {
+        const int kk = R / NTH, r = R % NTH;
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int s = t * 16 + ((PREC == RTX_PREC_F64) ? (lg + 4 * v) : (4 * lg + v));
+                if (s < NSP) xs[kk * CD + s * NTH + r] = outv[t][v];
+            }
+    }
+    __syncthreads();
+    // ---- coalesced write-back, masked
+    for (int e = tid; e < NCD; e += ETHREADS) {
+        const int kk = e / CD, lt = e % CD, sn = lt / NTH;
+        const bool masked = (cbd[kk] & nfc[sn]) != 0;
+        const A v = masked ? A(0) : xs[e];
+        if (accumulate) {
+            if (!masked) y[cdof + e] = T(A(y[cdof + e]) + v);
+        } else {
+            y[cdof + e] = T(v);
+        }
+    }
+}
+
+template <typename T, typename A, int PREC, int N, int NTH, int NC>
+int launch_e(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t st, int acc) {
+    constexpr int NSP = N * N, NA = NC * NTH;
+    constexpr int NAP = u_pad(NA, NTH);
+    constexpr int MAXL = 4 * NC, TRL = N * NTH;
+    RTX_CHECK(p->max_part_links[1] <= MAXL, "uniform apply: %d links in a column (max %d)",
+              p->max_part_links[1], MAXL);
+    constexpr bool XB = PREC == RTX_PREC_MIXED;
+    const size_t tiles = XB ? u_bf_off(size_t(NSP * NAP) * sizeof(A)) + u_bf_bytes(NSP, NA)
+                            : size_t(NSP * NAP) * sizeof(A);
+    const size_t lds = tiles + size_t(2 * p->max_part_links[1] * TRL) * sizeof(A);
+    auto kern = k_uniform_e<T, A, PREC, N, NTH, NC>;
+    RTX_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                int(lds)));
+    const int nblocks = ((p->m.ncol + UXCD - 1) / UXCD) * UXCD;
+    kern<<<nblocks, 64 * ue_waves<NTH, NC>(), lds, st>>>(p->m, ops, acc, (const T *)dx, (T *)dy,
+                                                         p->max_part_links[1]);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
 template <typename T, typename A, int PREC, int N, int NTH, int NC, int H>
 int launch_one(rtx_plan *p, uint32_t ops, const void *dx, void *dy, hipStream_t st, int acc) {
     constexpr int NSP = N * N, NA = NC * NTH;
@@ -585,12 +1019,24 @@ int launch_h(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipS
     }
 }
 
+// Every column of the plan resident at once in the output-layout kernel
+// (four workgroups per CU).
+inline bool uniform_one_round(const rtx_plan *p) { return p->m.ncol <= 4 * p->n_cu; }
+
 template <int N, int NTH, int NC>
 int launch_shape(rtx_plan *p, uint32_t ops, int prec, const void *dx, void *dy, hipStream_t st,
                  int acc) {
-    // default: one part per column (x read once, no padded output tiles
-    // between parts) when valid; rtx_autotune may pick another valid count
+    // default: FP64 runs the output-layout kernel when every column is
+    // resident at once (four workgroups per CU; C2, 1024 columns: 29.5 us
+    // against 30.4 us), else the line-pass kernel, whose eight-wave
+    // workgroups pipeline better over several rounds (64 x 64 x 16 cells:
+    // 94 us against 109 us); FP32 and mixed run the line-pass kernel (mixed
+    // C2 15.3 us against 15.7 us).  A part count set by
+    // rtx_set_uniform_parts or rtx_autotune selects the line-pass kernel
+    // with that many parts.
     int H = p->uni_H;
+    if (H == 0 && prec == RTX_PREC_F64 && uniform_one_round(p))
+        return launch_e<double, double, RTX_PREC_F64, N, NTH, NC>(p, ops, dx, dy, st, acc);
     if (H == 0) H = u_valid<NTH, NC, 1>() ? 1 : (u_valid<NTH, NC, 2>() ? 2 : 4);
     if (H == 1) return launch_h<N, NTH, NC, 1>(p, ops, prec, dx, dy, st, acc);
     if (H == 2) return launch_h<N, NTH, NC, 2>(p, ops, prec, dx, dy, st, acc);

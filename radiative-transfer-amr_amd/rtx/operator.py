@@ -147,15 +147,16 @@ class TransportOperator:
         """Launch shapes of the apply: uniform bin (usable, parts per column),
         the fused order bins (parts, columns; by LDS class) and the column and cell
         counts left to the generic kernels."""
-        out = (C.c_int64 * 24)()
-        n = self.lib.rtx_plan_info(self.plan, out, 24)
+        out = (C.c_int64 * 25)()
+        n = self.lib.rtx_plan_info(self.plan, out, 25)
         v = list(out)[:n]
         bins = [tuple(v[2 + 3 * b: 5 + 3 * b]) for b in range(6)]
         return {"uniform": bool(v[0]), "uniform_parts": v[1],
                 "fused_bins": [(H, nc) for H, nc, g in bins if nc and not g],
                 "bins_on_generic": [nc for H, nc, g in bins if nc and g],
                 "generic_cols": v[20], "generic_cells": v[21],
-                "column_kernel": bool(v[22]), "column_kernel_cols": v[23]}
+                "column_kernel": bool(v[22]), "column_kernel_cols": v[23],
+                "uniform_output_layout": bool(v[24])}
 
     def set_bin_generic(self, b, generic=True):
         """Run fused order bin ``b`` on the generic kernels (or back)."""

@@ -248,6 +248,56 @@ def test_uniform_apply_bit_reproducible(cuda, n, nref_ang, nref_spt):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n,nref_ang,nref_spt", [(5, 4, 2), (4, 2, 2), (3, 3, 2), (5, 3, 2)],
+                         ids=["5-5-16", "4-4-4", "3-3-8", "5-5-8"])
+def test_uniform_output_layout_kernel(cuda, n, nref_ang, nref_spt):
+    """The default FP64 uniform kernel (every term of an output added where
+    the scattering MFMA leaves it) against the line-pass kernel (one part per
+    column) on compiled shapes: the same operations in the same order, so the
+    results agree to the last bits (1e-15 relative; equality is reported);
+    both match the oracle at 1e-12; repeated applies are bit-identical;
+    accumulate mode adds exactly.  FP32 and mixed default to the line-pass
+    kernel with one part."""
+    import torch
+    from rtx.operator import (TransportOperator, OP_SYSTEM, OP_RESTRICT, OP_ACCUM, PREC_F32,
+                              PREC_MIXED)
+    mesh = problems.uniform_mesh([1.0, 1.0], [n, n, n], nref_ang, nref_spt)
+    wl = problems.absorber_c2()
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    info = op.plan_info()
+    assert info["uniform"] and info["uniform_output_layout"]
+    print("shape", n, nref_ang, "ndof", op.ndof, flush=True)
+    x = np.random.default_rng(21).uniform(-1.0, 1.0, op.ndof)
+    xd = _dev(x)
+    ops = OP_SYSTEM | OP_RESTRICT
+    ye = [op.apply(xd, ops=ops).clone() for _ in range(3)]
+    for yy in ye[1:]:
+        assert torch.equal(yy, ye[0])
+    op.set_uniform_parts(1)
+    assert not op.plan_info()["uniform_output_layout"]
+    yl = op.apply(xd, ops=ops)
+    yl32 = op.apply(xd.float(), ops=ops, precision=PREC_F32)
+    ylm = op.apply(xd.float(), ops=ops, precision=PREC_MIXED)
+    op.set_uniform_parts(0)
+    err = float((ye[0] - yl).norm() / yl.norm())
+    print("output-layout vs line-pass FP64: rel", err, "equal", bool(torch.equal(ye[0], yl)), flush=True)
+    assert err < 1e-15, err
+    V = asm.assemble(mesh, wl.kappa, wl.sigma, wl.Phi)
+    A = (V["BC"] - V["IC"] + V["M"] + V["S"]).tocsr()
+    mask = asm.intr_mask(mesh)
+    ref = mask * (A @ (mask * x))
+    assert _rel(ye[0].cpu().numpy(), ref) < TOL64
+    y32 = op.apply(xd.float(), ops=ops, precision=PREC_F32)
+    assert float((y32 - yl32).norm() / yl32.norm()) < 1e-6
+    ym = op.apply(xd.float(), ops=ops, precision=PREC_MIXED)
+    assert float((ym - ylm).norm() / ylm.norm()) < 1e-6
+    y0 = torch.rand_like(xd)
+    y1 = y0.clone()
+    op.apply(xd, y1, ops=OP_SYSTEM | OP_ACCUM)
+    assert torch.allclose(y1 - y0, op.apply(xd, ops=OP_SYSTEM), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.gpu
 def test_hp_default_apply_bit_reproducible(cuda):
     """On an hp mesh the default launch shapes (the hp column kernel: one
     workgroup per column, every output written by one lane; the generic

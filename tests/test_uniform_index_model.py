@@ -159,3 +159,48 @@ def test_bf16_tile_padding_and_operands(n, nth, nc):
                 first = s * napb + ks * 16 + 4 * lg
                 assert (first * 2) % 8 == 0
                 assert all((s, ks * 16 + 4 * lg + u) in written for u in range(4))
+
+
+def ue_waves(nth, nc):
+    return max((nc * nth + 15) // 16, 4)
+
+
+@pytest.mark.parametrize("n,nth,nc", BINS)
+def test_output_layout_kernel_indices(n, nth, nc):
+    """k_uniform_e: lane (l16, lg) of wave w < NTILE owns output angle
+    R = 16 w + l16 and node rows s = 16 t + lg + 4 v (FP64 MFMA layout) or
+    16 t + 4 lg + v (FP32 / BF16); its outputs, staged in the column tile at
+    (cell, node, angle) = kk CD + s nth + r, cover the column's DOFs exactly
+    once and fit the tile; the derivative reads of the x- and y-lines stay
+    inside it; the first-round rule fits the LDS of four workgroups per CU
+    (C2 shape, FP64: column tile + traces and flux slots of 2 NC links)."""
+    nsp, na = n * n, nc * nth
+    ncd, cd, nap = nsp * na, nsp * nth, u_pad(na, nth)
+    mt, ntile = (nsp + 15) // 16, (na + 15) // 16
+    waves = ue_waves(nth, nc)
+    assert ntile <= waves and nsp * nap >= ncd
+    for f64 in (True, False):
+        staged = []
+        for w in range(ntile):
+            for lane in range(64):
+                l16, lg = lane % 16, lane // 16
+                R = 16 * w + l16
+                if R >= na:
+                    continue
+                kk, r = R // nth, R % nth
+                for t in range(mt):
+                    for v in range(4):
+                        s = 16 * t + ((lg + 4 * v) if f64 else (4 * lg + v))
+                        if s >= nsp:
+                            continue
+                        p, q = s // n, s % n
+                        for i in range(n):
+                            assert 0 <= (i * n + q) * nap + R < nsp * nap
+                            assert 0 <= (p * n + i) * nap + R < nsp * nap
+                        staged.append(kk * cd + s * nth + r)
+        assert sorted(staged) == list(range(ncd))
+    if (n, nth, nc) == (5, 5, 16):
+        trl, links = n * nth, 2 * nc
+        dyn = nsp * nap * 8 + 2 * links * trl * 8
+        static = 4112                    # the FP64 instance's static LDS (hipcc resource report)
+        assert 4 * (dyn + static) <= 160 * 1024
