@@ -173,7 +173,7 @@ def measured_traffic():
     WRITE_SIZE summary of this same workload (counter values scaled by the
     factors a 1 GiB copy kernel gives), or None when it is absent."""
     import re
-    path = os.path.join(ROOT, "profiles", "k_uniform_c2_f64.md")
+    path = os.path.join(ROOT, "profiles", "k_uniform_e_c2_f64.md")
     try:
         m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(path).read(), re.M)
     except OSError:
@@ -396,7 +396,7 @@ def main():
                      "unit": "TFLOP/s" if fp64_bound else "GB/s",
                      "frac": tflops / FP64_PEAK_TFLOPS if fp64_bound else gbs / HBM_PEAK_GBS,
                      "traffic": measured_traffic(),
-                     "kernel": "k_uniform (operator apply A_II, FP64, order bin 5x5x16)",
+                     "kernel": "k_uniform_e (operator apply A_II, FP64, output layout, order bin 5x5x16)",
                      "algorithmic_bytes": bytes_, "algorithmic_flops": flops,
                      "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
                      "fp64_tflops": tflops, "fp64_frac": tflops / FP64_PEAK_TFLOPS},
