@@ -743,6 +743,89 @@ __global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *o
     }
 }
 
+// z = B r from the FP8 pool, one wavefront per block of at most 128 rows
+// (the register-built blocks), four blocks per workgroup: lane (rg, cg),
+// rg = lane mod RG (RG = ldp / 16 row groups of 16), owns rows
+// 16 rg .. 16 rg + 15 and columns cg, cg + CG, ... (CG = 64 / RG, at most 16
+// columns per lane), all its 16-byte code loads and r entries in flight
+// together; the CG partial sums of a row are added in a per-wave LDS tile.
+// A quarter of the threads per block of k_bj_apply_fp8 and no block-wide
+// barrier, so more blocks' loads are in flight per CU.
+constexpr int BJ_W_MAXN = 128;
+constexpr int BJ_W_KMAX = 16;      // columns per lane: ceil(n / CG) <= 16 for n <= 128
+constexpr int BJ_W_WAVES = 4;
+__global__ __launch_bounds__(64 * BJ_W_WAVES) void k_bj_apply_fp8_w(
+    DevMesh m, const int64_t *off8, const int64_t *soff, const int32_t *__restrict__ blk, int nblk,
+    const uint8_t *__restrict__ inv, const float *__restrict__ scales,
+    const float *__restrict__ r, float *__restrict__ z) {
+    __shared__ float part[BJ_W_WAVES][BJ_W_MAXN * 8];   // CG x ldp <= 1024 per wave
+    __shared__ float rsh[BJ_W_WAVES][BJ_W_MAXN];        // r_j s_j of the block
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.x * BJ_W_WAVES + wave;
+    if (b >= nblk) return;                               // whole waves; no block barrier below
+    const int k = blk[3 * b], a0 = blk[3 * b + 1], g = blk[3 * b + 2];
+    const int c = m.cell_col[k];
+    const int nth = m.cell_nth[k];
+    const int n = m.col_nx[c] * m.col_ny[c] * g;
+    const int ldp = (n + 15) / 16 * 16, RG = ldp / 16, CG = 64 / RG;
+    const int rg = lane % RG, cg = lane / RG;
+    const int off = m.cell_dof_off[k] + a0;
+    const uint4 *B = reinterpret_cast<const uint4 *>(inv + off8[b]) + rg;
+    const int ldq = ldp / 16;
+    const float *sc = scales + soff[b];
+    // r_j s_j once per block entry (two per lane)
+    float *rs = rsh[wave];
+    for (int i = lane; i < n; i += 64) rs[i] = r[off + (i / g) * nth + i % g] * sc[i];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float acc[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    typedef float bj_f2 __attribute__((ext_vector_type(2)));
+    // two batches of eight columns, each batch's loads in flight together
+    // (all sixteen at once: 132 VGPRs, three waves per SIMD, slower)
+#pragma unroll 1
+    for (int u0 = 0; u0 < BJ_W_KMAX; u0 += BJ_W_KMAX / 2) {
+        uint4 q[BJ_W_KMAX / 2];
+        float rj[BJ_W_KMAX / 2];
+#pragma unroll
+        for (int u = 0; u < BJ_W_KMAX / 2; ++u) {
+            const int j = cg + (u0 + u) * CG;
+            const bool ok = cg < CG && j < n;
+            q[u] = ok ? B[size_t(j) * ldq] : make_uint4(0u, 0u, 0u, 0u);
+            rj[u] = ok ? rs[j] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < BJ_W_KMAX / 2; ++u) {
+            const uint32_t w[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const bj_f2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(int(w[e]), false);
+                const bj_f2 hi = __builtin_amdgcn_cvt_pk_f32_fp8(int(w[e]), true);
+                acc[4 * e + 0] += lo.x * rj[u];
+                acc[4 * e + 1] += lo.y * rj[u];
+                acc[4 * e + 2] += hi.x * rj[u];
+                acc[4 * e + 3] += hi.y * rj[u];
+            }
+        }
+    }
+    float *pw = part[wave];
+    if (cg < CG) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) pw[cg * ldp + rg * 16 + v] = acc[v];
+    }
+    // the wave's own LDS writes, read back by other lanes of the same wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int i = lane; i < n; i += 64) {
+        float s = 0.f;
+        for (int qq = 0; qq < CG; ++qq) s += pw[qq * ldp + i];
+        z[off + (i / g) * nth + i % g] = s;
+    }
+}
+
 }  // namespace rtx
 
 using namespace rtx;
@@ -952,6 +1035,11 @@ extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, 
         k_bj_apply_bf16<BJ_APPLY16_THREADS><<<p->bj_nblk, BJ_APPLY16_THREADS, 0, st>>>(
             p->m, p->bj_off16, p->bj_blk, (const uint16_t *)d_inv, (const float *)d_r,
             (float *)d_z);
+    else if (precision == RTX_PREC_FP8 && p->bj_max_block <= BJ_W_MAXN)
+        k_bj_apply_fp8_w<<<(p->bj_nblk + BJ_W_WAVES - 1) / BJ_W_WAVES, 64 * BJ_W_WAVES, 0, st>>>(
+            p->m, p->bj_off8, p->bj_soff, p->bj_blk, p->bj_nblk, (const uint8_t *)d_inv,
+            reinterpret_cast<const float *>((const uint8_t *)d_inv + p->bj_codes8),
+            (const float *)d_r, (float *)d_z);
     else if (precision == RTX_PREC_FP8)
         k_bj_apply_fp8<BJ_APPLY8_THREADS><<<p->bj_nblk, BJ_APPLY8_THREADS, 0, st>>>(
             p->m, p->bj_off8, p->bj_soff, p->bj_blk, (const uint8_t *)d_inv,
