@@ -185,7 +185,7 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
     // inn; then x is written over them, angle-major and masked.  DOF t of
     // the column lies in the cell of angle t / NSP (cell kk's DOFs start at
     // NSP * c_ao[kk]).
-    constexpr int XU = 8;
+    constexpr int XU = 4;                 // x and trace values per thread and pass
     A vv[XU];
 #pragma unroll
     for (int u = 0; u < XU; ++u) {
@@ -376,18 +376,18 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
                 // the copy are zero there, so they add nothing
                 auto gload = [&](int Ak) -> E { return gcol[size_t(Ak < na ? Ak : na - 1) * na]; };
                 const A *xcol = xs + lg * S;
-                // four K steps of G per load group; the next three groups
-                // are in flight while the current one multiplies
-                E bc[4], bn[4], bm[4], bf[4];
+                // four K steps of G per load group; the next two groups are
+                // in flight while the current one multiplies (three: 20 bytes
+                // per lane of register spills in FP64)
+                E bc[4], bn[4], bf[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     bc[u] = gload(4 * u + lg);
                     bn[u] = gload(16 + 4 * u + lg);
-                    bm[u] = gload(32 + 4 * u + lg);
                 }
                 for (int kb = 0; kb < na; kb += 16) {
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) bf[u] = gload(kb + 48 + 4 * u + lg);
+                    for (int u = 0; u < 4; ++u) bf[u] = gload(kb + 32 + 4 * u + lg);
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         if (kb + 4 * u < na) {            // uniform across the wave
@@ -407,8 +407,7 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         bc[u] = bn[u];
-                        bn[u] = bm[u];
-                        bm[u] = bf[u];
+                        bn[u] = bf[u];
                     }
                 }
             } else {
