@@ -241,6 +241,11 @@ int rtx_bjacobi_pack_fp8(rtx_plan *plan, const float *d_inv32, uint8_t *d_inv8, 
    factorisation itself is always FP64. */
 int rtx_bjacobi_apply(rtx_plan *plan, int precision, const void *d_inv, const void *d_r,
                       void *d_z, void *stream);
+/* rtx_bjacobi_apply(RTX_PREC_FP8) with d_r in FP16 (IEEE binary16) and d_z in
+   FP32: the preconditioner of GMRES on an FP16 basis.  No counterpart in the
+   reference (PETSc bjacobi, src/rt/solve.py:86-89, applies in FP64). */
+int rtx_bjacobi_apply_fp8_h(rtx_plan *plan, const void *d_inv, const void *d_r, void *d_z,
+                            void *stream);
 
 /* ---- GMRES building blocks (PETSc KSPGMRES of src/rt/solve.py:81-100). */
 
@@ -266,8 +271,11 @@ int rtx_arnoldi_step_f32(const float *d_V, int64_t ld, int nv, int64_t n, float 
 /* Arnoldi step j of a GMRES cycle of m steps (nv = j + 1 basis vectors,
    FP64 basis for basis_precision 0, FP32 for 1) fused with its Givens
    update: the same results as rtx_arnoldi_step (or _f32) followed by
-   rtx_gmres_givens(j, m, d_hcol, d_state).  No counterpart in the
-   reference (PETSc keeps the least-squares problem on the host). */
+   rtx_gmres_givens(j, m, d_hcol, d_state).  basis_precision 2: an FP16
+   basis (IEEE binary16) with d_w in FP32, one pass only (refine 0), the
+   steps of rtx_arnoldi_step_f32 with the new column rounded to FP16 after
+   the normalisation.  No counterpart in the reference (PETSc keeps the
+   least-squares problem on the host and its basis in FP64). */
 int rtx_gmres_arnoldi(int basis_precision, const void *d_V, int64_t ld, int64_t n, void *d_w,
                       double *d_hcol, double *d_work, int refine, int j, int m,
                       double *d_state, void *stream);
@@ -313,10 +321,15 @@ int rtx_combine(const double *d_V, int64_t ld, int nv, const double *d_y, int64_
    FP32 (1); accumulated in FP64.  No counterpart in the reference. */
 int rtx_combine_f32(const float *d_V, int64_t ld, int nv, const double *d_y, int64_t n,
                     void *d_x, int x_precision, int accumulate, void *stream);
+/* The same for an FP16 basis (IEEE binary16).  No counterpart in the
+   reference. */
+int rtx_combine_f16(const void *d_V, int64_t ld, int nv, const double *d_y, int64_t n,
+                    void *d_x, int x_precision, int accumulate, void *stream);
 /* y = a x + b y (precision RTX_PREC_F64 or RTX_PREC_F32). */
 int rtx_axpby(int precision, int64_t n, double a, const void *d_x, double b, void *d_y,
               void *stream);
-/* y = a x + b y, x FP32 and y FP64 (direction 0) or x FP64 and y FP32 (1). */
+/* y = a x + b y, x FP32 and y FP64 (direction 0), x FP64 and y FP32 (1), or
+   x FP64 and y FP16 (2). */
 int rtx_axpby_mixed(int direction, int64_t n, double a, const void *d_x, double b, void *d_y,
                     void *stream);
 /* direction 0: FP64 -> FP32, 1: FP32 -> FP64. */

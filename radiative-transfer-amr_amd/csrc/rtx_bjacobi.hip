@@ -678,13 +678,13 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_pack_fp8(DevMesh m, const int
 // z = B r from the FP8 pool: thread (cg, rg) owns rows 16 rg .. 16 rg + 15
 // and columns cg, cg + CG, ...; one 16-byte load of codes per column,
 // decoded by the hardware conversion; the column scales are folded into r.
-template <int NT>
+template <int NT, typename RT = float>
 __global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *off8,
                                                      const int64_t *soff,
                                                      const int32_t *__restrict__ blk,
                                                      const uint8_t *__restrict__ inv,
                                                      const float *__restrict__ scales,
-                                                     const float *__restrict__ r,
+                                                     const RT *__restrict__ r,
                                                      float *__restrict__ z) {
     __shared__ float rs[RTX_NMAX * RTX_NMAX * RTX_NMAX];
     __shared__ float part[NT * 16];
@@ -706,7 +706,7 @@ __global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *o
         q[u] = (cg < CG && j < n) ? B[size_t(j) * ldq] : make_uint4(0u, 0u, 0u, 0u);
     }
     const float *sc = scales + soff[b];
-    for (int i = tid; i < n; i += NT) rs[i] = r[off + (i / g) * nth + i % g] * sc[i];
+    for (int i = tid; i < n; i += NT) rs[i] = float(r[off + (i / g) * nth + i % g]) * sc[i];
     __syncthreads();
     float acc[16];
 #pragma unroll
@@ -754,10 +754,11 @@ __global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *o
 constexpr int BJ_W_MAXN = 128;
 constexpr int BJ_W_KMAX = 16;      // columns per lane: ceil(n / CG) <= 16 for n <= 128
 constexpr int BJ_W_WAVES = 4;
+template <typename RT = float>
 __global__ __launch_bounds__(64 * BJ_W_WAVES) void k_bj_apply_fp8_w(
     DevMesh m, const int64_t *off8, const int64_t *soff, const int32_t *__restrict__ blk, int nblk,
     const uint8_t *__restrict__ inv, const float *__restrict__ scales,
-    const float *__restrict__ r, float *__restrict__ z) {
+    const RT *__restrict__ r, float *__restrict__ z) {
     __shared__ float part[BJ_W_WAVES][BJ_W_MAXN * 8];   // CG x ldp <= 1024 per wave
     __shared__ float rsh[BJ_W_WAVES][BJ_W_MAXN];        // r_j s_j of the block
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -775,7 +776,7 @@ __global__ __launch_bounds__(64 * BJ_W_WAVES) void k_bj_apply_fp8_w(
     const float *sc = scales + soff[b];
     // r_j s_j once per block entry (two per lane)
     float *rs = rsh[wave];
-    for (int i = lane; i < n; i += 64) rs[i] = r[off + (i / g) * nth + i % g] * sc[i];
+    for (int i = lane; i < n; i += 64) rs[i] = float(r[off + (i / g) * nth + i % g]) * sc[i];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1036,7 +1037,8 @@ extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, 
             p->m, p->bj_off16, p->bj_blk, (const uint16_t *)d_inv, (const float *)d_r,
             (float *)d_z);
     else if (precision == RTX_PREC_FP8 && p->bj_max_block <= BJ_W_MAXN)
-        k_bj_apply_fp8_w<<<(p->bj_nblk + BJ_W_WAVES - 1) / BJ_W_WAVES, 64 * BJ_W_WAVES, 0, st>>>(
+        k_bj_apply_fp8_w<float><<<(p->bj_nblk + BJ_W_WAVES - 1) / BJ_W_WAVES, 64 * BJ_W_WAVES, 0,
+                                  st>>>(
             p->m, p->bj_off8, p->bj_soff, p->bj_blk, p->bj_nblk, (const uint8_t *)d_inv,
             reinterpret_cast<const float *>((const uint8_t *)d_inv + p->bj_codes8),
             (const float *)d_r, (float *)d_z);
@@ -1049,6 +1051,34 @@ extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, 
         set_error("rtx_bjacobi_apply: unsupported precision %d", precision);
         return RTX_ERR_ARG;
     }
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+/* z = M r with the FP8 inverses, r in FP16 (IEEE binary16) and z in FP32:
+   the preconditioner of GMRES on an FP16 basis.  Same inverses and
+   arithmetic as rtx_bjacobi_apply(RTX_PREC_FP8); only the input type
+   differs. */
+extern "C" int rtx_bjacobi_apply_fp8_h(rtx_plan *p, const void *d_inv, const void *d_r, void *d_z,
+                                       void *stream) {
+    RTX_CHECK(p && d_inv && d_r && d_z, "rtx_bjacobi_apply_fp8_h: null argument");
+    RTX_CHECK(d_r != d_z, "rtx_bjacobi_apply_fp8_h: r and z must not alias");
+    if (!p->bj_ready) {
+        set_error("rtx_bjacobi_apply_fp8_h: call rtx_bjacobi_factor first");
+        return RTX_ERR_STATE;
+    }
+    RTX_CHECK(p->bj_off8, "rtx_bjacobi_apply_fp8_h: call rtx_bjacobi_size first");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const float *scales = reinterpret_cast<const float *>((const uint8_t *)d_inv + p->bj_codes8);
+    if (p->bj_max_block <= BJ_W_MAXN)
+        k_bj_apply_fp8_w<_Float16><<<(p->bj_nblk + BJ_W_WAVES - 1) / BJ_W_WAVES,
+                                     64 * BJ_W_WAVES, 0, st>>>(
+            p->m, p->bj_off8, p->bj_soff, p->bj_blk, p->bj_nblk, (const uint8_t *)d_inv, scales,
+            (const _Float16 *)d_r, (float *)d_z);
+    else
+        k_bj_apply_fp8<BJ_APPLY8_THREADS, _Float16><<<p->bj_nblk, BJ_APPLY8_THREADS, 0, st>>>(
+            p->m, p->bj_off8, p->bj_soff, p->bj_blk, (const uint8_t *)d_inv, scales,
+            (const _Float16 *)d_r, (float *)d_z);
     RTX_HIP(hipGetLastError());
     return RTX_OK;
 }

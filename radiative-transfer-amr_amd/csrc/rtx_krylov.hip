@@ -49,6 +49,21 @@ __device__ inline void ld4(const double *p, double (&o)[4]) {
     const double2 b = *reinterpret_cast<const double2 *>(p + 2);
     o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
 }
+// FP16 basis vectors: four consecutive values in one 8-byte load / store
+typedef _Float16 kh4 __attribute__((ext_vector_type(4)));
+__device__ inline void ld4(const _Float16 *p, double (&o)[4]) {
+    const kh4 v = *reinterpret_cast<const kh4 *>(p);
+    o[0] = double(v[0]); o[1] = double(v[1]); o[2] = double(v[2]); o[3] = double(v[3]);
+}
+__device__ inline void st4(_Float16 *p, const _Float16 (&v)[4]) {
+    *reinterpret_cast<kh4 *>(p) = kh4{v[0], v[1], v[2], v[3]};
+}
+// a double rounded to a basis storage type (FP16 through FP32)
+template <typename T>
+__device__ inline T to_store(double x) {
+    if constexpr (std::is_same<T, _Float16>::value) return _Float16(float(x));
+    else return T(x);
+}
 __device__ inline void st4(float *p, const float (&v)[4]) {
     *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
 }
@@ -69,9 +84,9 @@ __device__ inline void row_range(int64_t n, int b, int nb, int64_t &i0, int64_t 
 
 // partial[j * nb + b] = sum over block b's rows of V_j[i] w[i], j < nv;
 // if with_ww, partial[nv * nb + b] = sum w[i]^2 (computed by chunk 0).
-template <typename T, bool VEC>
+template <typename T, bool VEC, typename TW = T>
 __global__ __launch_bounds__(KT) void k_mdot(const T *__restrict__ V, int64_t ld, int nv,
-                                             const T *__restrict__ w, int64_t n,
+                                             const TW *__restrict__ w, int64_t n,
                                              double *__restrict__ partial, int with_ww,
                                              const int *__restrict__ skip) {
     if (skip && *skip) return;
@@ -187,10 +202,10 @@ __global__ __launch_bounds__(KT) void k_update(const T *__restrict__ V, int64_t 
 // a relative norm below 1e-6 counts as a breakdown (v_out = 0, h = 0).
 // Block 0 writes hcol[0..nv] and, when state is given, folds the column
 // into the Givens factorisation (k_normalize_givens's recurrence).
-template <typename T, bool VEC>
+template <typename T, bool VEC, typename TW = T>
 __global__ __launch_bounds__(KT) void k_update_pyth(const T *__restrict__ V, int64_t ld, int nv,
                                                     const double *__restrict__ hd,
-                                                    const T *__restrict__ w,
+                                                    const TW *__restrict__ w,
                                                     T *__restrict__ v_out, int64_t n,
                                                     double *__restrict__ hcol, int j, int m,
                                                     double *state) {
@@ -224,13 +239,13 @@ __global__ __launch_bounds__(KT) void k_update_pyth(const T *__restrict__ V, int
         }
         T out[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) out[u] = T(acc[u] * sc);
+        for (int u = 0; u < 4; ++u) out[u] = to_store<T>(acc[u] * sc);
         st4(v_out + i, out);
     }
     for (int64_t i = i0 + 4 * nq + threadIdx.x; i < i1; i += KT) {
         double acc = double(w[i]);
         for (int jj = 0; jj < nv; ++jj) acc -= double(V[int64_t(jj) * ld + i]) * hs[jj];
-        v_out[i] = T(acc * sc);
+        v_out[i] = to_store<T>(acc * sc);
     }
     if (b != 0 || threadIdx.x != 0) return;
     for (int q = 0; q < nv; ++q) hcol[q] = hs[q];
@@ -293,7 +308,7 @@ __global__ __launch_bounds__(KT) void k_combine(const TV *__restrict__ V, int64_
     for (; i < n; i += int64_t(gridDim.x) * KT) {
         double acc = accumulate ? double(x[i]) : 0.0;
         for (int j = 0; j < nv; ++j) acc += double(V[int64_t(j) * ld + i]) * ys[j];
-        x[i] = TX(acc);
+        x[i] = to_store<TX>(acc);
     }
 }
 
@@ -303,7 +318,7 @@ __global__ void k_axpby_mixed(int64_t n, double a, const TX *__restrict__ x, dou
                               TY *__restrict__ y) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     for (; i < n; i += int64_t(gridDim.x) * blockDim.x)
-        y[i] = TY(a * double(x[i]) + (b == 0.0 ? 0.0 : b * double(y[i])));
+        y[i] = to_store<TY>(a * double(x[i]) + (b == 0.0 ? 0.0 : b * double(y[i])));
 }
 
 // y = a x + b y (device scalars optional)
@@ -543,6 +558,42 @@ static int arnoldi_step(const T *d_V, int64_t ld, int nv, int64_t n, T *d_w, dou
     return RTX_OK;
 }
 
+// Arnoldi step on an FP16 basis with an FP32 w (the inner solves of mixed
+// refinement): classical Gram-Schmidt, one pass, the norm from the dot
+// products (accumulated in FP64), in the FP32 basis's three launches; the
+// new basis vector is rounded to FP16 once, after the normalisation.
+static int arnoldi_step_h(const _Float16 *d_V, int64_t ld, int nv, int64_t n, const float *d_w,
+                          double *d_hcol, double *d_work, int refine, void *stream, int gj,
+                          int gm, double *d_state) {
+    RTX_CHECK(d_V && d_w && d_hcol && d_work, "rtx_gmres_arnoldi: null argument");
+    RTX_CHECK(refine == 0, "rtx_gmres_arnoldi: an FP16 basis takes one pass (refine 0)");
+    RTX_CHECK(nv >= 1 && nv < KMAXV - 1, "rtx_gmres_arnoldi: nv=%d outside [1, %d)", nv,
+              KMAXV - 1);
+    RTX_CHECK(n >= 1 && ld >= n, "rtx_gmres_arnoldi: bad n/ld");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    double *part = d_work;
+    double *hd = d_work + int64_t(KMAXV + 1) * KNB;
+    const int nchunk = (nv + KCH - 1) / KCH;
+    _Float16 *V_out = const_cast<_Float16 *>(d_V) + int64_t(nv) * ld;
+    const bool vec = (ld % 4 == 0) && aligned16(d_V) && aligned16(d_w) &&
+                     (reinterpret_cast<uintptr_t>(V_out) & 7) == 0;
+    if (vec) {
+        k_mdot<_Float16, true, float><<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n, part,
+                                                                        1, nullptr);
+        k_reduce<<<nv + 1, KT, 0, st>>>(part, KNB, hd, 0, nullptr);
+        k_update_pyth<_Float16, true, float><<<KNB, KT, 0, st>>>(d_V, ld, nv, hd, d_w, V_out, n,
+                                                                d_hcol, gj, gm, d_state);
+    } else {
+        k_mdot<_Float16, false, float><<<dim3(KNB, nchunk), KT, 0, st>>>(d_V, ld, nv, d_w, n,
+                                                                         part, 1, nullptr);
+        k_reduce<<<nv + 1, KT, 0, st>>>(part, KNB, hd, 0, nullptr);
+        k_update_pyth<_Float16, false, float><<<KNB, KT, 0, st>>>(d_V, ld, nv, hd, d_w, V_out, n,
+                                                                 d_hcol, gj, gm, d_state);
+    }
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
 /* One Arnoldi step: orthogonalise w against V_0..V_{nv-1} (columns of length n,
    stride ld) by classical Gram-Schmidt; refine 0: one pass, 1: a second pass
    when the norm dropped by more than the "twice is enough" factor, 2: always
@@ -564,8 +615,12 @@ extern "C" int rtx_gmres_arnoldi(int basis_precision, const void *d_V, int64_t l
                                  int m, double *d_state, void *stream) {
     RTX_CHECK(d_state && m >= 1 && m < KMAXV - 1 && j >= 0 && j < m,
               "rtx_gmres_arnoldi: bad argument (j=%d, m=%d)", j, m);
-    RTX_CHECK(basis_precision == 0 || basis_precision == 1,
-              "rtx_gmres_arnoldi: basis_precision must be 0 (FP64) or 1 (FP32)");
+    RTX_CHECK(basis_precision >= 0 && basis_precision <= 2,
+              "rtx_gmres_arnoldi: basis_precision must be 0 (FP64), 1 (FP32) or 2 (FP16)");
+    if (basis_precision == 2)
+        return arnoldi_step_h(static_cast<const _Float16 *>(d_V), ld, j + 1, n,
+                              static_cast<const float *>(d_w), d_hcol, d_work, refine, stream, j,
+                              m, d_state);
     if (basis_precision == 0)
         return arnoldi_step<double>(static_cast<const double *>(d_V), ld, j + 1, n,
                                     static_cast<double *>(d_w), d_hcol, d_work, refine, stream,
@@ -704,16 +759,38 @@ extern "C" int rtx_combine_f32(const float *d_V, int64_t ld, int nv, const doubl
     return RTX_OK;
 }
 
-/* y = a x + b y with x FP32 and y FP64 (direction 0) or x FP64 and y FP32
-   (direction 1). */
+/* d_x (+)= sum_j y_j V_j for an FP16 basis (device, IEEE binary16); d_x is
+   FP64 (x_precision 0) or FP32 (x_precision 1); accumulation in FP64. */
+extern "C" int rtx_combine_f16(const void *d_V, int64_t ld, int nv, const double *d_y,
+                               int64_t n, void *d_x, int x_precision, int accumulate,
+                               void *stream) {
+    RTX_CHECK(d_V && d_y && d_x && nv >= 1 && nv < KMAXV && (x_precision == 0 || x_precision == 1),
+              "rtx_combine_f16: bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const _Float16 *V = static_cast<const _Float16 *>(d_V);
+    if (x_precision == 0)
+        k_combine<_Float16, double><<<2048, KT, 0, st>>>(V, ld, nv, d_y, (double *)d_x, n,
+                                                         accumulate);
+    else
+        k_combine<_Float16, float><<<2048, KT, 0, st>>>(V, ld, nv, d_y, (float *)d_x, n,
+                                                        accumulate);
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+/* y = a x + b y with x FP32 and y FP64 (direction 0), x FP64 and y FP32
+   (direction 1), or x FP64 and y FP16 (direction 2). */
 extern "C" int rtx_axpby_mixed(int direction, int64_t n, double a, const void *d_x, double b,
                                void *d_y, void *stream) {
-    RTX_CHECK(d_x && d_y && n >= 0 && (direction == 0 || direction == 1),
+    RTX_CHECK(d_x && d_y && n >= 0 && direction >= 0 && direction <= 2,
               "rtx_axpby_mixed: bad argument");
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (direction == 0)
         k_axpby_mixed<float, double><<<2048, 256, 0, st>>>(n, a, (const float *)d_x, b,
                                                            (double *)d_y);
+    else if (direction == 2)
+        k_axpby_mixed<double, _Float16><<<2048, 256, 0, st>>>(n, a, (const double *)d_x, b,
+                                                              (_Float16 *)d_y);
     else
         k_axpby_mixed<double, float><<<2048, 256, 0, st>>>(n, a, (const double *)d_x, b,
                                                            (float *)d_y);

@@ -73,6 +73,8 @@ SIGNATURES = [
     ("rtx_bjacobi_pack_bf16", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rtx_bjacobi_apply", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p]),
+    ("rtx_bjacobi_apply_fp8_h", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]),
     ("rtx_arnoldi_workspace", C.c_int, [C.c_int, C.POINTER(C.c_int64)]),
     ("rtx_arnoldi_step", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_int64, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
@@ -83,6 +85,8 @@ SIGNATURES = [
     ("rtx_combine", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64,
                               C.c_void_p, C.c_int, C.c_void_p]),
     ("rtx_combine_f32", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64,
+                                  C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    ("rtx_combine_f16", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64,
                                   C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     ("rtx_axpby_mixed", C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_void_p, C.c_double,
                                   C.c_void_p, C.c_void_p]),

@@ -39,7 +39,7 @@ import torch
 from . import _lib, problems
 from ._lib import check, PREC_F64, PREC_F32, PREC_MIXED, PREC_FP8, OP_SYSTEM, OP_RESTRICT
 from .operator import stream_handle
-from .krylov import BlockJacobi, InteriorSystem, GMRES, gmres, refine, _norm
+from .krylov import BlockJacobi, InteriorSystem, GMRES, gmres, refine, pick_basis, _norm
 
 
 def _vp(t):
@@ -530,14 +530,15 @@ class Session:
     precision "mixed": FP64 iterative refinement around flexible GMRES with
     FP32 / BF16 applies and block-Jacobi inverses stored in ``bj_precision``
     (PREC_FP8: e4m3 with column scales, the default; PREC_MIXED: BF16;
-    PREC_F32: FP32) on an FP32 (``basis`` "f32") or
-    FP64 Arnoldi basis; "fp64": GMRES in FP64.  ``pc`` is a TwoLevelPC or
+    PREC_F32: FP32) on an FP16, FP32 or FP64 Arnoldi basis (``basis``
+    "auto": FP16 with FP8 inverses and no coarse correction, else FP32);
+    "fp64": GMRES in FP64.  ``pc`` is a TwoLevelPC or
     None (plain block-Jacobi).  A coarse model whose held-out relative error
     (``model.validation_error``) exceeds ``gate`` is not used at all: the
     session runs plain block-Jacobi and reports ``gated``."""
 
     def __init__(self, op, pc=None, bj=None, precision="mixed", restart=None,
-                 bj_precision=None, basis="f32", gate=0.5):
+                 bj_precision=None, basis="auto", gate=0.5):
         self.gated = False
         if pc is not None:
             ve = getattr(getattr(pc, "model", None), "validation_error", None)
@@ -577,6 +578,8 @@ class Session:
         else:
             self.hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
             self.lo = InteriorSystem(op, PREC_MIXED, the_pc, bj_precision)
+            if basis == "auto":
+                basis = pick_basis(self.lo, flexible)
             self.gm = GMRES(self.lo, op.ndof, op.device, restart=restart, flexible=flexible,
                             basis=basis)
 

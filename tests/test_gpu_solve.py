@@ -109,8 +109,8 @@ def test_refinement_mixed_reaches_fp64_residual(cuda):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant", ["plain", "flexible", "mixed", "mixed_f64_basis", "hp_plain",
-                                     "hp_mixed"])
+@pytest.mark.parametrize("variant", ["plain", "flexible", "mixed", "mixed_f64_basis",
+                                     "mixed_f16_basis", "hp_plain", "hp_mixed", "hp_mixed_f16_basis"])
 def test_graph_replay_matches_eager(cuda, variant):
     """Chunks of GMRES iterations replayed as HIP graphs reproduce the eager
     run exactly (same kernels, same order): iterate, residual history and
@@ -120,7 +120,7 @@ def test_graph_replay_matches_eager(cuda, variant):
     import torch
     from rtx.operator import TransportOperator
     from rtx.krylov import BlockJacobi, InteriorSystem, gmres, refine
-    from rtx._lib import PREC_F64, PREC_F32, PREC_MIXED
+    from rtx._lib import PREC_F64, PREC_F32, PREC_MIXED, PREC_FP8
     if variant.startswith("hp"):
         # hp mesh on the default (generic, atomic-free) launch shapes
         from test_gpu_apply import small_random_hp
@@ -131,16 +131,17 @@ def test_graph_replay_matches_eager(cuda, variant):
         mesh, wl = _c1_small()
         tol, max_it = 1e-10, 500
     op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
-    pc = BlockJacobi(op, keep_f32=True)
+    f16 = variant == "mixed_f16_basis"
+    pc = BlockJacobi(op, keep_f32=not f16, keep_fp8=f16)
     rhs = op.mask(_dev(np.random.default_rng(4).uniform(-1, 1, op.ndof)))
     runs = []
     for graphs in (False, True, True):
         if variant.startswith("mixed"):
             hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
-            lo = InteriorSystem(op, PREC_MIXED, pc, PREC_F32)
+            lo = InteriorSystem(op, PREC_MIXED, pc, PREC_FP8 if f16 else PREC_F32)
+            basis = {"mixed_f64_basis": "f64", "mixed_f16_basis": "f16"}.get(variant, "f32")
             x, info = refine(hi, lo, rhs, None, rtol=tol, inner_rtol=1e-2, restart=7,
-                             max_inner=max_it, chunk=3, graphs=graphs,
-                             basis="f64" if variant == "mixed_f64_basis" else "f32")
+                             max_inner=max_it, chunk=3, graphs=graphs, basis=basis)
             assert info["converged_reason"] > 0
             runs.append((x.cpu().numpy(), info["residual_history"], info["iteration_number"]))
         else:
@@ -244,6 +245,103 @@ def test_arnoldi_fp32_basis(cuda):
         assert np.linalg.norm(rec - W[j]) < 1e-5 * np.linalg.norm(W[j])
         ref = out[torch.float64][1][j - 1]
         assert np.abs(Hs[j - 1] - ref).max() < 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_arnoldi_fp16_basis(cuda):
+    """Arnoldi steps on an FP16 basis (FP32 w, FP64 dot products and
+    Hessenberg column, new column rounded to FP16 after the normalisation):
+    the basis is orthonormal to FP16 rounding (unit roundoff 2^-11), every
+    input vector is reproduced by its Hessenberg column to that accuracy,
+    and the columns agree with the FP64 step's."""
+    import ctypes as C
+    import torch
+    from rtx import _lib
+    from rtx.krylov import _vp
+    lib = _lib.load()
+    n, k = 200_003, 12
+    rng = np.random.default_rng(5)
+    W = rng.normal(size=(k, n))
+    ws = C.c_int64()
+    _lib.check(lib.rtx_arnoldi_workspace(k + 1, C.byref(ws)), "workspace")
+    work = torch.zeros(int(ws.value), dtype=torch.float64, device="cuda")
+    _lib.check(lib.rtx_gmres_state_size(k, C.byref(ws)), "state size")
+    state = torch.zeros(int(ws.value), dtype=torch.float64, device="cuda")
+    _lib.check(lib.rtx_gmres_begin(k, 1.0, _vp(state), None), "begin")
+    V = torch.zeros((k + 1, n), dtype=torch.float16, device="cuda")
+    V[0] = torch.from_numpy(W[0] / np.linalg.norm(W[0])).to(torch.float16)
+    Hs = []
+    for j in range(1, k):
+        w = torch.from_numpy(W[j]).to(torch.float32).cuda()
+        hcol = torch.zeros(k + 2, dtype=torch.float64, device="cuda")
+        # step j - 1 of the cycle orthogonalises against j vectors
+        _lib.check(lib.rtx_gmres_arnoldi(2, _vp(V), n, n, _vp(w), _vp(hcol), _vp(work), 0,
+                                         j - 1, k, _vp(state), None), "fp16 step")
+        Hs.append(hcol[:j + 1].cpu().numpy())
+    torch.cuda.synchronize()
+    Vh = V[:k].double().cpu().numpy()
+    G = Vh @ Vh.T
+    assert np.abs(G - np.eye(k)).max() < 3e-3, np.abs(G - np.eye(k)).max()
+    # the FP64 step on the same (FP16-rounded) basis vectors and inputs
+    Vd = torch.from_numpy(Vh).cuda()
+    for j in range(1, k):
+        rec = Hs[j - 1] @ Vh[:j + 1]
+        assert np.linalg.norm(rec - W[j]) < 3e-3 * np.linalg.norm(W[j])
+        V64 = torch.zeros((j + 1, n), dtype=torch.float64, device="cuda")
+        V64[:j] = Vd[:j]
+        w64 = torch.from_numpy(W[j].astype(np.float32).astype(np.float64)).cuda()
+        h64 = torch.zeros(j + 2, dtype=torch.float64, device="cuda")
+        _lib.check(lib.rtx_arnoldi_step(_vp(V64), n, j, n, _vp(w64), _vp(h64), _vp(work), 0,
+                                        None), "fp64 step")
+        ref = h64[:j + 1].cpu().numpy()
+        assert np.abs(Hs[j - 1][:j] - ref[:j]).max() < 1e-9 * np.abs(ref).max()
+        assert abs(Hs[j - 1][j] - ref[j]) < 1e-6 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_fp8_apply_fp16_input(cuda):
+    """The FP8 block-Jacobi apply from an FP16 r equals the apply from the
+    same values in FP32 bit for bit (one conversion of each entry, the same
+    arithmetic), on whole cells and on split cells."""
+    import torch
+    from rtx.operator import TransportOperator
+    from rtx.krylov import BlockJacobi
+    from rtx._lib import PREC_FP8
+    from test_gpu_apply import small_random_hp
+    for mesh, wl in ((_c1_small()[0], problems.manufactured()),
+                     (small_random_hp(2), problems.two_scatterers_c3())):
+        op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+        pc = BlockJacobi(op, keep_f64=False, keep_fp8=True)
+        r16 = torch.from_numpy(np.random.default_rng(9).uniform(-1, 1, op.ndof)).to(
+            torch.float16).cuda()
+        z_h = pc.apply_fp8_h(r16, torch.zeros(op.ndof, dtype=torch.float32, device="cuda"))
+        z_f = pc.apply(r16.float(), torch.zeros(op.ndof, dtype=torch.float32, device="cuda"),
+                       PREC_FP8)
+        assert torch.equal(z_h, z_f)
+
+
+@pytest.mark.gpu
+def test_refine_fp16_basis_matches_spsolve(cuda):
+    """Mixed iterative refinement with an FP16 inner basis and FP8
+    block-Jacobi (the default of mixed precision) reaches the same discrete
+    solution as spsolve (FP64 outer residual 1e-10)."""
+    from rtx.operator import TransportOperator
+    from rtx.krylov import BlockJacobi, InteriorSystem, refine, pick_basis
+    from rtx._lib import PREC_F64, PREC_MIXED, PREC_FP8
+    mesh, wl = _c1_small()
+    A, f, b, mask = _oracle_system(mesh, wl)
+    u_ref, _, _ = osys.exact_solve(A, f, b, mask)
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    pc = BlockJacobi(op, keep_f64=False, keep_fp8=True)
+    hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
+    lo = InteriorSystem(op, PREC_MIXED, pc, PREC_FP8)
+    assert pick_basis(lo, False) == "f16" and pick_basis(lo, True) == "f32"
+    bb = np.where(mask, 0.0, b)
+    rhs_full = np.where(mask, f - A @ bb, 0.0)
+    x, info = refine(hi, lo, _dev(rhs_full), rtol=1e-10, inner_rtol=1e-3, basis="f16")
+    assert info["converged_reason"] > 0 and info["rel_residual"] <= 1e-10
+    u = np.where(mask, x.cpu().numpy(), b)
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < TOL_SOL
 
 
 @pytest.mark.gpu
