@@ -48,10 +48,10 @@ constexpr int HP_MT = (RTX_NMAX * RTX_NMAX + 15) / 16;   // node tiles of 16
 constexpr int HP_XCD = 8;
 constexpr int HP_TG = 2;          // node tiles per scattering pass (accumulators in registers)
 constexpr int HP_NMIN = 3;        // node counts compiled for the column kernel
-constexpr int HP_FV = 8;
+constexpr int HP_FV = 8;          // inflow-face sums per thread (a column holds <= HP_FV * HP_THREADS)
 constexpr int HP_LINK_LOG = 8;    // 2^8 >= HP_MAX_LINKS
 constexpr int HP_CELL_LOG = 6;    // 2^6 >= HP_MAX_CELLS
-static_assert((1 << HP_LINK_LOG) >= HP_MAX_LINKS && (1 << HP_CELL_LOG) >= HP_MAX_CELLS, "search range");          // inflow-face sums per thread (a column holds <= HP_FV * HP_THREADS)
+static_assert((1 << HP_LINK_LOG) >= HP_MAX_LINKS && (1 << HP_CELL_LOG) >= HP_MAX_CELLS, "search range");
 
 typedef double hp_d4 __attribute__((ext_vector_type(4)));
 typedef float hp_f4 __attribute__((ext_vector_type(4)));

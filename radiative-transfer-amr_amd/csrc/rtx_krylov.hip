@@ -608,9 +608,10 @@ extern "C" int rtx_arnoldi_step(const double *d_V, int64_t ld, int nv, int64_t n
 }
 
 /* Arnoldi step j of a GMRES cycle of m steps fused with its Givens update:
-   rtx_arnoldi_step (FP64 basis, basis_precision 0, or FP32, 1) on
-   nv = j + 1 vectors, then rtx_gmres_givens(j, m, d_hcol, d_state) folded
-   into the final normalisation.  No counterpart in the reference (PETSc
+   rtx_arnoldi_step (FP64 basis, basis_precision 0, FP32, 1, or FP16 with an
+   FP32 w, 2, one pass only) on nv = j + 1 vectors, then
+   rtx_gmres_givens(j, m, d_hcol, d_state) folded into the final
+   normalisation.  No counterpart in the reference (PETSc
    keeps the least-squares problem on the host). */
 extern "C" int rtx_gmres_arnoldi(int basis_precision, const void *d_V, int64_t ld, int64_t n,
                                  void *d_w, double *d_hcol, double *d_work, int refine, int j,

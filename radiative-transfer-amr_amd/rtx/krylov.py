@@ -505,8 +505,8 @@ def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=
     correction solve A d = r by GMRES whose operator and preconditioner run in
     reduced precision (``system_lo``) on a ``basis`` "f16", "f32" or "f64"
     Arnoldi basis ("auto": FP16 when the preconditioner is the FP8
-    block-Jacobi and the method is not flexible, else FP32).  One GMRES object serves every inner solve, so its graphs
-    are captured once.
+    block-Jacobi and the method is not flexible, else FP32).  One GMRES
+    object serves every inner solve, so its graphs are captured once.
     ``flexible`` selects flexible inner GMRES (for a nonlinear
     preconditioner); ``after_outer(outer, |r|)`` sees every FP64 residual;
     ``inner`` may pass a GMRES object on ``system_lo`` to reuse (with its
