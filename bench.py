@@ -531,7 +531,7 @@ def _timed_solve(op, rhs, precision, make_pc=None, inner_rtol=1e-2):
     from rtx.krylov import BlockJacobi
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    bj = BlockJacobi(op, keep_f64=precision == "fp64", keep_fp8=precision != "fp64")
+    bj = BlockJacobi(op, keep_f64=precision == "fp64", keep_fp4=precision != "fp64")
     pc = make_pc(bj) if make_pc is not None else None
     # fp64: GMRES(50) as the reference's KSP; mixed: inner GMRES(20)
     sess = learned.Session(op, pc=pc, bj=bj, precision=precision)

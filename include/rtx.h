@@ -49,7 +49,9 @@ enum {
     RTX_PREC_F32 = 1,      /* x, y FP32; FP32 arithmetic                            */
     RTX_PREC_MIXED = 2,    /* x, y FP32; convection FP32, scattering BF16 on MFMA
                               with FP32 accumulation                                */
-    RTX_PREC_FP8 = 3       /* block-Jacobi apply only: FP8 (e4m3) inverses with an
+    RTX_PREC_FP8 = 3,      /* block-Jacobi apply only: FP8 (e4m3) inverses with an
+                              FP32 scale per column, FP32 vectors and arithmetic    */
+    RTX_PREC_FP4 = 4       /* block-Jacobi apply only: FP4 (e2m1) inverses with an
                               FP32 scale per column, FP32 vectors and arithmetic    */
 };
 
@@ -234,16 +236,30 @@ int rtx_bjacobi_size_fp8(rtx_plan *plan, int64_t *n_bytes);
    max |column| / 448 per column, into the FP8 pool.  No counterpart in the
    reference. */
 int rtx_bjacobi_pack_fp8(rtx_plan *plan, const float *d_inv32, uint8_t *d_inv8, void *stream);
+/* Bytes of the FP4 inverse pool: the FP8 pool's layout at half the bytes
+   (two e2m1 codes per byte, the even row in the low nibble; every column
+   8-byte aligned), then one FP32 scale per column (at a 16-byte aligned
+   offset). */
+int rtx_bjacobi_size_fp4(rtx_plan *plan, int64_t *n_bytes);
+/* Quantise the FP32 inverses to FP4 e2m1 (nearest, ties to the even code,
+   saturating) with the scale max |column| / 6 per column, into the FP4
+   pool.  No counterpart in the reference. */
+int rtx_bjacobi_pack_fp4(rtx_plan *plan, const float *d_inv32, uint8_t *d_inv4, void *stream);
 /* z = blockdiag(inv) r, precision RTX_PREC_F64 (double inverses, vectors),
    RTX_PREC_F32 (float inverses, vectors), RTX_PREC_MIXED (the packed BF16
-   pool of rtx_bjacobi_pack_bf16, float vectors, FP32 accumulation) or
-   RTX_PREC_FP8 (the pool of rtx_bjacobi_pack_fp8, float vectors); the
+   pool of rtx_bjacobi_pack_bf16, float vectors, FP32 accumulation),
+   RTX_PREC_FP8 (the pool of rtx_bjacobi_pack_fp8, float vectors) or
+   RTX_PREC_FP4 (the pool of rtx_bjacobi_pack_fp4, float vectors); the
    factorisation itself is always FP64. */
 int rtx_bjacobi_apply(rtx_plan *plan, int precision, const void *d_inv, const void *d_r,
                       void *d_z, void *stream);
-/* rtx_bjacobi_apply(RTX_PREC_FP8) with d_r in FP16 (IEEE binary16) and d_z in
-   FP32: the preconditioner of GMRES on an FP16 basis.  No counterpart in the
-   reference (PETSc bjacobi, src/rt/solve.py:86-89, applies in FP64). */
+/* rtx_bjacobi_apply(RTX_PREC_FP8 or RTX_PREC_FP4) with d_r in FP16 (IEEE
+   binary16) and d_z in FP32: the preconditioner of GMRES on an FP16 basis.
+   No counterpart in the reference (PETSc bjacobi, src/rt/solve.py:86-89,
+   applies in FP64). */
+int rtx_bjacobi_apply_h(rtx_plan *plan, int precision, const void *d_inv, const void *d_r,
+                        void *d_z, void *stream);
+/* rtx_bjacobi_apply_h(RTX_PREC_FP8, ...). */
 int rtx_bjacobi_apply_fp8_h(rtx_plan *plan, const void *d_inv, const void *d_r, void *d_z,
                             void *stream);
 

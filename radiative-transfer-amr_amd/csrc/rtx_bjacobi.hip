@@ -675,10 +675,116 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_pack_fp8(DevMesh m, const int
     }
 }
 
-// z = B r from the FP8 pool: thread (cg, rg) owns rows 16 rg .. 16 rg + 15
-// and columns cg, cg + CG, ...; one 16-byte load of codes per column,
-// decoded by the hardware conversion; the column scales are folded into r.
-template <int NT, typename RT = float>
+// FP4 e2m1 (OCP: bias 1, one mantissa bit, largest 6, no infinities or
+// NaN; magnitudes 0, 0.5, 1, 1.5, 2, 3, 4, 6 for codes 0..7): nearest code,
+// ties to the even code; |v| above 6 saturates.
+__device__ inline uint32_t bj_to_e2m1(float v) {
+    const uint32_t sgn = v < 0.f ? 8u : 0u;
+    const float a = fabsf(v);
+    // midpoints between consecutive magnitudes; at a midpoint the even code
+    // (0, 2, 4, 6) wins
+    uint32_t c;
+    if (a <= 0.25f) c = 0;
+    else if (a < 0.75f) c = 1;
+    else if (a <= 1.25f) c = 2;
+    else if (a < 1.75f) c = 3;
+    else if (a <= 2.5f) c = 4;
+    else if (a < 3.5f) c = 5;
+    else if (a <= 5.f) c = 6;
+    else c = 7;
+    return sgn | c;
+}
+
+// FP4 copy of block b: the FP8 copy's layout at half the bytes (column j's
+// codes at off8[b] / 2 + j * ldp / 2, row i in byte i / 2, the even row in
+// the low nibble), scale_j = max_i |B_ij| / 6 at scales[soff[b] + j];
+// code = e2m1(B_ij / scale_j).
+__global__ __launch_bounds__(BJ_THREADS) void k_bj_pack_fp4(DevMesh m, const int64_t *off,
+                                                            const int64_t *off8,
+                                                            const int64_t *soff,
+                                                            const int32_t *__restrict__ blk,
+                                                            const float *__restrict__ src,
+                                                            uint8_t *__restrict__ dst,
+                                                            float *__restrict__ scales) {
+    __shared__ float inv_s[RTX_NMAX * RTX_NMAX * RTX_NMAX];
+    const int b = blockIdx.x;
+    const int n = bj_block_n(m, blk, b), ldp = (n + 15) / 16 * 16, hp = ldp / 2;
+    const float *S = src + off[b];
+    uint8_t *D = dst + off8[b] / 2;
+    for (int j = threadIdx.x; j < n; j += BJ_THREADS) {
+        float mx = 0.f;
+        for (int i = 0; i < n; ++i) mx = fmaxf(mx, fabsf(S[size_t(j) * n + i]));
+        const float sc = mx > 0.f ? mx / 6.f : 1.f;
+        scales[soff[b] + j] = sc;
+        inv_s[j] = 1.f / sc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * hp; e += BJ_THREADS) {
+        const int j = e / hp, i = 2 * (e % hp);
+        const uint32_t lo = i < n ? bj_to_e2m1(S[size_t(j) * n + i] * inv_s[j]) : 0u;
+        const uint32_t hi = i + 1 < n ? bj_to_e2m1(S[size_t(j) * n + i + 1] * inv_s[j]) : 0u;
+        D[e] = uint8_t(lo | (hi << 4));
+    }
+}
+
+// Sixteen consecutive codes of one column (rows 16 rg .. 16 rg + 15),
+// decoded to FP32 by the gfx950 hardware conversions and multiplied into
+// the row sums: acc[v] += code_v * rj.  FP8 e4m3: one 16-byte load, two
+// codes per conversion; FP4 e2m1: one 8-byte load, two codes per
+// conversion (scale 1).
+template <int BITS>
+struct BjCodes;
+template <>
+struct BjCodes<8> {
+    typedef uint4 V;
+    static constexpr int SHIFT = 0;    // byte offset of a block = off8[b] >> SHIFT
+    __device__ static V zero() { return make_uint4(0u, 0u, 0u, 0u); }
+    __device__ static void fma16(const V &v, float rj, float (&acc)[16]) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const f2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(int(w[u]), false);
+            const f2 hi = __builtin_amdgcn_cvt_pk_f32_fp8(int(w[u]), true);
+            acc[4 * u + 0] += lo.x * rj;
+            acc[4 * u + 1] += lo.y * rj;
+            acc[4 * u + 2] += hi.x * rj;
+            acc[4 * u + 3] += hi.y * rj;
+        }
+    }
+};
+template <>
+struct BjCodes<4> {
+    typedef uint2 V;
+    static constexpr int SHIFT = 1;
+    __device__ static V zero() { return make_uint2(0u, 0u); }
+    __device__ static void fma16(const V &v, float rj, float (&acc)[16]) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const uint32_t w[2] = {v.x, v.y};
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            // byte s of word u holds rows 8 u + 2 s (low nibble) and + 1
+            const f2 p0 = __builtin_amdgcn_cvt_scalef32_pk_f32_fp4(w[u], 1.f, 0);
+            const f2 p1 = __builtin_amdgcn_cvt_scalef32_pk_f32_fp4(w[u], 1.f, 1);
+            const f2 p2 = __builtin_amdgcn_cvt_scalef32_pk_f32_fp4(w[u], 1.f, 2);
+            const f2 p3 = __builtin_amdgcn_cvt_scalef32_pk_f32_fp4(w[u], 1.f, 3);
+            acc[8 * u + 0] += p0.x * rj;
+            acc[8 * u + 1] += p0.y * rj;
+            acc[8 * u + 2] += p1.x * rj;
+            acc[8 * u + 3] += p1.y * rj;
+            acc[8 * u + 4] += p2.x * rj;
+            acc[8 * u + 5] += p2.y * rj;
+            acc[8 * u + 6] += p3.x * rj;
+            acc[8 * u + 7] += p3.y * rj;
+        }
+    }
+};
+
+// z = B r from the FP8 (BITS 8) or FP4 (BITS 4) pool: thread (cg, rg) owns
+// rows 16 rg .. 16 rg + 15 and columns cg, cg + CG, ...; one load of sixteen
+// codes per column, decoded by the hardware conversion; the column scales
+// are folded into r.
+template <int NT, typename RT = float, int BITS = 8>
 __global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *off8,
                                                      const int64_t *soff,
                                                      const int32_t *__restrict__ blk,
@@ -696,14 +802,16 @@ __global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *o
     const int ldp = (n + 15) / 16 * 16, RG = ldp / 16, CG = NT / RG;
     const int off = m.cell_dof_off[k] + a0;
     const int rg = tid % RG, cg = tid / RG;
-    const uint4 *B = reinterpret_cast<const uint4 *>(inv + off8[b]) + rg;
+    typedef BjCodes<BITS> Codes;
+    typedef typename Codes::V CV;
+    const CV *B = reinterpret_cast<const CV *>(inv + (off8[b] >> Codes::SHIFT)) + rg;
     const int ldq = ldp / 16;
     constexpr int PF = 4;
-    uint4 q[PF];
+    CV q[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
         const int j = cg + u * CG;
-        q[u] = (cg < CG && j < n) ? B[size_t(j) * ldq] : make_uint4(0u, 0u, 0u, 0u);
+        q[u] = (cg < CG && j < n) ? B[size_t(j) * ldq] : Codes::zero();
     }
     const float *sc = scales + soff[b];
     for (int i = tid; i < n; i += NT) rs[i] = float(r[off + (i / g) * nth + i % g]) * sc[i];
@@ -711,20 +819,7 @@ __global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *o
     float acc[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) acc[u] = 0.f;
-    // the hardware FP8 (e4m3) to FP32 conversion, two codes per instruction
-    typedef float bj_f2 __attribute__((ext_vector_type(2)));
-    auto fma16 = [&](const uint4 &v, float rj) {
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const bj_f2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(int(w[u]), false);
-            const bj_f2 hi = __builtin_amdgcn_cvt_pk_f32_fp8(int(w[u]), true);
-            acc[4 * u + 0] += lo.x * rj;
-            acc[4 * u + 1] += lo.y * rj;
-            acc[4 * u + 2] += hi.x * rj;
-            acc[4 * u + 3] += hi.y * rj;
-        }
-    };
+    auto fma16 = [&](const CV &v, float rj) { Codes::fma16(v, rj, acc); };
     if (cg < CG) {
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
@@ -743,7 +838,7 @@ __global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *o
     }
 }
 
-// z = B r from the FP8 pool, one wavefront per block of at most 128 rows
+// z = B r from the FP8 (or FP4) pool, one wavefront per block of at most 128 rows
 // (the register-built blocks), four blocks per workgroup: lane (rg, cg),
 // rg = lane mod RG (RG = ldp / 16 row groups of 16), owns rows
 // 16 rg .. 16 rg + 15 and columns cg, cg + CG, ... (CG = 64 / RG, at most 16
@@ -754,7 +849,7 @@ __global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *o
 constexpr int BJ_W_MAXN = 128;
 constexpr int BJ_W_KMAX = 16;      // columns per lane: ceil(n / CG) <= 16 for n <= 128
 constexpr int BJ_W_WAVES = 4;
-template <typename RT = float>
+template <typename RT = float, int BITS = 8>
 __global__ __launch_bounds__(64 * BJ_W_WAVES) void k_bj_apply_fp8_w(
     DevMesh m, const int64_t *off8, const int64_t *soff, const int32_t *__restrict__ blk, int nblk,
     const uint8_t *__restrict__ inv, const float *__restrict__ scales,
@@ -771,7 +866,9 @@ __global__ __launch_bounds__(64 * BJ_W_WAVES) void k_bj_apply_fp8_w(
     const int ldp = (n + 15) / 16 * 16, RG = ldp / 16, CG = 64 / RG;
     const int rg = lane % RG, cg = lane / RG;
     const int off = m.cell_dof_off[k] + a0;
-    const uint4 *B = reinterpret_cast<const uint4 *>(inv + off8[b]) + rg;
+    typedef BjCodes<BITS> Codes;
+    typedef typename Codes::V CV;
+    const CV *B = reinterpret_cast<const CV *>(inv + (off8[b] >> Codes::SHIFT)) + rg;
     const int ldq = ldp / 16;
     const float *sc = scales + soff[b];
     // r_j s_j once per block entry (two per lane)
@@ -783,33 +880,23 @@ __global__ __launch_bounds__(64 * BJ_W_WAVES) void k_bj_apply_fp8_w(
     float acc[16];
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-    typedef float bj_f2 __attribute__((ext_vector_type(2)));
-    // two batches of eight columns, each batch's loads in flight together
-    // (all sixteen at once: 132 VGPRs, three waves per SIMD, slower)
+    // FP8: two batches of eight columns, each batch's loads in flight
+    // together (all sixteen at once: 132 VGPRs, three waves per SIMD,
+    // slower); FP4: the half-size loads of all sixteen columns at once
+    constexpr int BATCH = BITS == 4 ? BJ_W_KMAX : BJ_W_KMAX / 2;
 #pragma unroll 1
-    for (int u0 = 0; u0 < BJ_W_KMAX; u0 += BJ_W_KMAX / 2) {
-        uint4 q[BJ_W_KMAX / 2];
-        float rj[BJ_W_KMAX / 2];
+    for (int u0 = 0; u0 < BJ_W_KMAX; u0 += BATCH) {
+        CV q[BATCH];
+        float rj[BATCH];
 #pragma unroll
-        for (int u = 0; u < BJ_W_KMAX / 2; ++u) {
+        for (int u = 0; u < BATCH; ++u) {
             const int j = cg + (u0 + u) * CG;
             const bool ok = cg < CG && j < n;
-            q[u] = ok ? B[size_t(j) * ldq] : make_uint4(0u, 0u, 0u, 0u);
+            q[u] = ok ? B[size_t(j) * ldq] : Codes::zero();
             rj[u] = ok ? rs[j] : 0.f;
         }
 #pragma unroll
-        for (int u = 0; u < BJ_W_KMAX / 2; ++u) {
-            const uint32_t w[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const bj_f2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(int(w[e]), false);
-                const bj_f2 hi = __builtin_amdgcn_cvt_pk_f32_fp8(int(w[e]), true);
-                acc[4 * e + 0] += lo.x * rj[u];
-                acc[4 * e + 1] += lo.y * rj[u];
-                acc[4 * e + 2] += hi.x * rj[u];
-                acc[4 * e + 3] += hi.y * rj[u];
-            }
-        }
+        for (int u = 0; u < BATCH; ++u) Codes::fma16(q[u], rj[u], acc);
     }
     float *pw = part[wave];
     if (cg < CG) {
@@ -890,6 +977,8 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
     p->bj_entries16 = off16.back();
     p->bj_codes8 = (off8.back() + 15) / 16 * 16;
     p->bj_bytes8 = p->bj_codes8 + soff.back() * int64_t(sizeof(float));
+    p->bj_codes4 = (off8.back() / 2 + 15) / 16 * 16;
+    p->bj_bytes4 = p->bj_codes4 + soff.back() * int64_t(sizeof(float));
     *n_entries = off.back();
     return RTX_OK;
 }
@@ -912,6 +1001,28 @@ extern "C" int rtx_bjacobi_pack_fp8(rtx_plan *p, const float *d_inv32, uint8_t *
     k_bj_pack_fp8<<<p->bj_nblk, BJ_THREADS, 0, static_cast<hipStream_t>(stream)>>>(
         p->m, p->bj_off, p->bj_off8, p->bj_soff, p->bj_blk, d_inv32, d_inv8,
         reinterpret_cast<float *>(d_inv8 + p->bj_codes8));
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+extern "C" int rtx_bjacobi_size_fp4(rtx_plan *p, int64_t *n_bytes) {
+    RTX_CHECK(p && n_bytes, "rtx_bjacobi_size_fp4: null argument");
+    if (!p->bj_off8) {
+        int64_t n;
+        const int rc = rtx_bjacobi_size(p, &n);
+        if (rc) return rc;
+    }
+    *n_bytes = p->bj_bytes4;
+    return RTX_OK;
+}
+
+extern "C" int rtx_bjacobi_pack_fp4(rtx_plan *p, const float *d_inv32, uint8_t *d_inv4,
+                                    void *stream) {
+    RTX_CHECK(p && d_inv32 && d_inv4, "rtx_bjacobi_pack_fp4: null argument");
+    RTX_CHECK(p->bj_off8 && p->bj_nblk > 0, "rtx_bjacobi_pack_fp4: call rtx_bjacobi_size first");
+    k_bj_pack_fp4<<<p->bj_nblk, BJ_THREADS, 0, static_cast<hipStream_t>(stream)>>>(
+        p->m, p->bj_off, p->bj_off8, p->bj_soff, p->bj_blk, d_inv32, d_inv4,
+        reinterpret_cast<float *>(d_inv4 + p->bj_codes4));
     RTX_HIP(hipGetLastError());
     return RTX_OK;
 }
@@ -1017,6 +1128,23 @@ extern "C" int rtx_bjacobi_set_method(rtx_plan *p, int method) {
     return RTX_OK;
 }
 
+// The FP8 / FP4 applies: the wavefront-per-block kernel for blocks of at
+// most BJ_W_MAXN rows, else the slab kernel; r of type RT, z FP32.
+template <int BITS, typename RT>
+static void launch_bj_coded(const rtx_plan *p, const void *d_inv, const RT *r, float *z,
+                            hipStream_t st) {
+    const uint8_t *codes = static_cast<const uint8_t *>(d_inv);
+    const float *scales =
+        reinterpret_cast<const float *>(codes + (BITS == 4 ? p->bj_codes4 : p->bj_codes8));
+    if (p->bj_max_block <= BJ_W_MAXN)
+        k_bj_apply_fp8_w<RT, BITS><<<(p->bj_nblk + BJ_W_WAVES - 1) / BJ_W_WAVES,
+                                     64 * BJ_W_WAVES, 0, st>>>(
+            p->m, p->bj_off8, p->bj_soff, p->bj_blk, p->bj_nblk, codes, scales, r, z);
+    else
+        k_bj_apply_fp8<BJ_APPLY8_THREADS, RT, BITS><<<p->bj_nblk, BJ_APPLY8_THREADS, 0, st>>>(
+            p->m, p->bj_off8, p->bj_soff, p->bj_blk, codes, scales, r, z);
+}
+
 extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, const void *d_r,
                                  void *d_z, void *stream) {
     RTX_CHECK(p && d_inv && d_r && d_z, "rtx_bjacobi_apply: null argument");
@@ -1036,17 +1164,10 @@ extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, 
         k_bj_apply_bf16<BJ_APPLY16_THREADS><<<p->bj_nblk, BJ_APPLY16_THREADS, 0, st>>>(
             p->m, p->bj_off16, p->bj_blk, (const uint16_t *)d_inv, (const float *)d_r,
             (float *)d_z);
-    else if (precision == RTX_PREC_FP8 && p->bj_max_block <= BJ_W_MAXN)
-        k_bj_apply_fp8_w<float><<<(p->bj_nblk + BJ_W_WAVES - 1) / BJ_W_WAVES, 64 * BJ_W_WAVES, 0,
-                                  st>>>(
-            p->m, p->bj_off8, p->bj_soff, p->bj_blk, p->bj_nblk, (const uint8_t *)d_inv,
-            reinterpret_cast<const float *>((const uint8_t *)d_inv + p->bj_codes8),
-            (const float *)d_r, (float *)d_z);
     else if (precision == RTX_PREC_FP8)
-        k_bj_apply_fp8<BJ_APPLY8_THREADS><<<p->bj_nblk, BJ_APPLY8_THREADS, 0, st>>>(
-            p->m, p->bj_off8, p->bj_soff, p->bj_blk, (const uint8_t *)d_inv,
-            reinterpret_cast<const float *>((const uint8_t *)d_inv + p->bj_codes8),
-            (const float *)d_r, (float *)d_z);
+        launch_bj_coded<8, float>(p, d_inv, (const float *)d_r, (float *)d_z, st);
+    else if (precision == RTX_PREC_FP4)
+        launch_bj_coded<4, float>(p, d_inv, (const float *)d_r, (float *)d_z, st);
     else {
         set_error("rtx_bjacobi_apply: unsupported precision %d", precision);
         return RTX_ERR_ARG;
@@ -1055,30 +1176,32 @@ extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, 
     return RTX_OK;
 }
 
-/* z = M r with the FP8 inverses, r in FP16 (IEEE binary16) and z in FP32:
-   the preconditioner of GMRES on an FP16 basis.  Same inverses and
-   arithmetic as rtx_bjacobi_apply(RTX_PREC_FP8); only the input type
-   differs. */
-extern "C" int rtx_bjacobi_apply_fp8_h(rtx_plan *p, const void *d_inv, const void *d_r, void *d_z,
-                                       void *stream) {
-    RTX_CHECK(p && d_inv && d_r && d_z, "rtx_bjacobi_apply_fp8_h: null argument");
-    RTX_CHECK(d_r != d_z, "rtx_bjacobi_apply_fp8_h: r and z must not alias");
+/* z = M r with the FP8 (RTX_PREC_FP8) or FP4 (RTX_PREC_FP4) inverses, r in
+   FP16 (IEEE binary16) and z in FP32: the preconditioner of GMRES on an
+   FP16 basis.  Same inverses and arithmetic as rtx_bjacobi_apply; only the
+   input type differs. */
+extern "C" int rtx_bjacobi_apply_h(rtx_plan *p, int precision, const void *d_inv, const void *d_r,
+                                   void *d_z, void *stream) {
+    RTX_CHECK(p && d_inv && d_r && d_z, "rtx_bjacobi_apply_h: null argument");
+    RTX_CHECK(d_r != d_z, "rtx_bjacobi_apply_h: r and z must not alias");
+    RTX_CHECK(precision == RTX_PREC_FP8 || precision == RTX_PREC_FP4,
+              "rtx_bjacobi_apply_h: precision must be RTX_PREC_FP8 or RTX_PREC_FP4, not %d",
+              precision);
     if (!p->bj_ready) {
-        set_error("rtx_bjacobi_apply_fp8_h: call rtx_bjacobi_factor first");
+        set_error("rtx_bjacobi_apply_h: call rtx_bjacobi_factor first");
         return RTX_ERR_STATE;
     }
-    RTX_CHECK(p->bj_off8, "rtx_bjacobi_apply_fp8_h: call rtx_bjacobi_size first");
+    RTX_CHECK(p->bj_off8, "rtx_bjacobi_apply_h: call rtx_bjacobi_size first");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const float *scales = reinterpret_cast<const float *>((const uint8_t *)d_inv + p->bj_codes8);
-    if (p->bj_max_block <= BJ_W_MAXN)
-        k_bj_apply_fp8_w<_Float16><<<(p->bj_nblk + BJ_W_WAVES - 1) / BJ_W_WAVES,
-                                     64 * BJ_W_WAVES, 0, st>>>(
-            p->m, p->bj_off8, p->bj_soff, p->bj_blk, p->bj_nblk, (const uint8_t *)d_inv, scales,
-            (const _Float16 *)d_r, (float *)d_z);
+    if (precision == RTX_PREC_FP8)
+        launch_bj_coded<8, _Float16>(p, d_inv, (const _Float16 *)d_r, (float *)d_z, st);
     else
-        k_bj_apply_fp8<BJ_APPLY8_THREADS, _Float16><<<p->bj_nblk, BJ_APPLY8_THREADS, 0, st>>>(
-            p->m, p->bj_off8, p->bj_soff, p->bj_blk, (const uint8_t *)d_inv, scales,
-            (const _Float16 *)d_r, (float *)d_z);
+        launch_bj_coded<4, _Float16>(p, d_inv, (const _Float16 *)d_r, (float *)d_z, st);
     RTX_HIP(hipGetLastError());
     return RTX_OK;
+}
+
+extern "C" int rtx_bjacobi_apply_fp8_h(rtx_plan *p, const void *d_inv, const void *d_r, void *d_z,
+                                       void *stream) {
+    return rtx_bjacobi_apply_h(p, RTX_PREC_FP8, d_inv, d_r, d_z, stream);
 }

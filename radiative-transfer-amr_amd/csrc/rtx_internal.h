@@ -199,6 +199,9 @@ struct rtx_plan {
     // scale offset (floats from the start of the scales)
     int64_t *bj_off8 = nullptr, *bj_soff = nullptr;
     int64_t bj_codes8 = 0, bj_bytes8 = 0;
+    // FP4 copy: the FP8 layout at half the bytes (two codes per byte, block
+    // b's codes at bj_off8[b] / 2), then the same FP32 scales
+    int64_t bj_codes4 = 0, bj_bytes4 = 0;
     bool bj_ready = false;
     int bj_method = -1;          // -1 automatic, 0 LDS Gauss-Jordan, 1 register sweeps
     // blocks: (cell, first angle, angle count) triples; a cell of more than

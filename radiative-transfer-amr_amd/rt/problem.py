@@ -11,7 +11,7 @@ from dg.projection import Projection, push_forward, pull_back
 from dg.quadrature import quad_xyth, lag_eval
 from rtx import _lib
 from rtx.operator import TransportOperator, OP_QUADW, OP_SYSTEM, PREC_F64, PREC_F32, PREC_MIXED
-from rtx._lib import PREC_FP8
+from rtx._lib import PREC_FP4
 from rtx import krylov
 
 _PRECISIONS = {"fp64": PREC_F64, "fp32": PREC_F32, "mixed": PREC_MIXED}
@@ -116,7 +116,7 @@ class Problem:
         convergence_info["pc_type_used"]), rtol=EPS, atol=EPS, max_it=5000,
         restart=50, precision="fp64" | "fp32" | "mixed" (FP64 iterative
         refinement around reduced-precision applies: "mixed" runs FP32
-        convection, BF16 scattering and FP8 block-Jacobi inverses with
+        convection, BF16 scattering and FP4 block-Jacobi inverses with
         inner GMRES(20) unless ``restart`` is given; "fp32" runs FP32
         throughout), inner_rtol=1e-2 (refinement only)."""
         kw = {"verbose": False, "ksp_type": "gmres", "pc_type": "bjacobi",
@@ -146,7 +146,7 @@ class Problem:
         prec = _PRECISIONS[kw["precision"]]
         t_setup = time.perf_counter()
         pc = (krylov.BlockJacobi(op, keep_f64=prec == PREC_F64, keep_f32=prec == PREC_F32,
-                                 keep_fp8=prec == PREC_MIXED)
+                                 keep_fp4=prec == PREC_MIXED)
               if pc_used == "bjacobi" else None)
         t_pc = time.perf_counter()
         # the refinement's FP64 system only forms residuals: no preconditioner
@@ -157,7 +157,7 @@ class Problem:
                                    restart=kw["restart"], max_it=kw["max_it"],
                                    flexible=kw["ksp_type"] == "fgmres")
         else:
-            pc_prec = {PREC_F32: PREC_F32, PREC_MIXED: PREC_FP8}[prec] if pc is not None else prec
+            pc_prec = {PREC_F32: PREC_F32, PREC_MIXED: PREC_FP4}[prec] if pc is not None else prec
             lo = krylov.InteriorSystem(op, prec, pc, pc_prec)
             x, info = krylov.refine(hi, lo, rhs, x0, rtol=max(kw["rtol"], 1e-14),
                                     inner_rtol=kw["inner_rtol"], restart=kw["restart"],

@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(_HERE, "librtx.so")
 
 OP_MASS, OP_SCAT, OP_ICONV, OP_BCONV, OP_RESTRICT, OP_ACCUM, OP_QUADW = 1, 2, 4, 8, 16, 32, 64
 OP_SYSTEM = OP_MASS | OP_SCAT | OP_ICONV | OP_BCONV
-PREC_F64, PREC_F32, PREC_MIXED, PREC_FP8 = 0, 1, 2, 3
+PREC_F64, PREC_F32, PREC_MIXED, PREC_FP8, PREC_FP4 = 0, 1, 2, 3, 4
 NMAX = 10
 
 _i32p = C.POINTER(C.c_int32)
@@ -70,11 +70,15 @@ SIGNATURES = [
     ("rtx_bjacobi_size_bf16", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("rtx_bjacobi_size_fp8", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("rtx_bjacobi_pack_fp8", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtx_bjacobi_size_fp4", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("rtx_bjacobi_pack_fp4", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rtx_bjacobi_pack_bf16", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rtx_bjacobi_apply", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p]),
     ("rtx_bjacobi_apply_fp8_h", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p]),
+    ("rtx_bjacobi_apply_h", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
     ("rtx_arnoldi_workspace", C.c_int, [C.c_int, C.POINTER(C.c_int64)]),
     ("rtx_arnoldi_step", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_int64, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),

@@ -9,8 +9,8 @@ choices, not accidents:
 * the Krylov basis, the Arnoldi orthogonalisation (classical Gram-Schmidt,
   one pass as PETSc's default, optionally re-orthogonalised), the Givens QR
   of the Hessenberg matrix with its back substitution, the solution update
-  and the true residual all stay on the device (FP64, or an FP32 basis for
-  the inner solves of mixed refinement); the host only reads the residual
+  and the true residual all stay on the device (FP64, or an FP32 or FP16
+  basis for the inner solves of mixed refinement); the host only reads the residual
   estimates once per chunk of iterations;
 * convergence is checked on the FP64 true residual at every restart; when
   the recurrence estimate has met the target but the true residual sits
@@ -32,7 +32,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, PREC_F64, PREC_F32, PREC_MIXED, PREC_FP8, OP_SYSTEM, OP_RESTRICT
+from ._lib import (check, PREC_F64, PREC_F32, PREC_MIXED, PREC_FP8, PREC_FP4, OP_SYSTEM,
+                   OP_RESTRICT)
 from .operator import stream_handle
 
 
@@ -49,14 +50,15 @@ class BlockJacobi:
     """Dense inverses of every cell's interior diagonal block, on the device."""
 
     def __init__(self, op, restrict=True, keep_f32=False, keep_f64=True, keep_bf16=False,
-                 keep_fp8=False):
+                 keep_fp8=False, keep_fp4=False):
         self.op, self.lib = op, op.lib
         n = C.c_int64()
         check(self.lib.rtx_bjacobi_size(op.plan, C.byref(n)), "rtx_bjacobi_size")
         self.entries = int(n.value)
         dev = op.device
         self.inv = torch.empty(self.entries, dtype=torch.float64, device=dev)
-        need32 = keep_f32 or keep_bf16 or keep_fp8   # BF16 / FP8 are rounded from FP32
+        # BF16 / FP8 / FP4 are rounded from FP32
+        need32 = keep_f32 or keep_bf16 or keep_fp8 or keep_fp4
         self.inv32 = torch.empty(self.entries, dtype=torch.float32, device=dev) if need32 else None
         st = stream_handle()
         rc = self.lib.rtx_bjacobi_factor(op.plan, 1 if restrict else 0, _vp(self.inv),
@@ -88,7 +90,18 @@ class BlockJacobi:
             self.inv8 = torch.empty(self.bytes8, dtype=torch.uint8, device=dev)
             check(self.lib.rtx_bjacobi_pack_fp8(op.plan, _vp(self.inv32), _vp(self.inv8), st),
                   "rtx_bjacobi_pack_fp8")
-        if keep_bf16 or keep_fp8:
+        self.inv4 = None
+        self.bytes4 = 0
+        if keep_fp4:
+            n4 = C.c_int64()
+            check(self.lib.rtx_bjacobi_size_fp4(op.plan, C.byref(n4)), "rtx_bjacobi_size_fp4")
+            self.bytes4 = int(n4.value)
+            # e2m1 codes (two per byte, 8-byte columns) followed by one FP32
+            # scale per column
+            self.inv4 = torch.empty(self.bytes4, dtype=torch.uint8, device=dev)
+            check(self.lib.rtx_bjacobi_pack_fp4(op.plan, _vp(self.inv32), _vp(self.inv4), st),
+                  "rtx_bjacobi_pack_fp4")
+        if keep_bf16 or keep_fp8 or keep_fp4:
             if not keep_f32:
                 torch.cuda.current_stream().synchronize()
                 self.inv32 = None
@@ -100,23 +113,32 @@ class BlockJacobi:
             return self.entries16 * 2
         if precision == PREC_FP8:
             return self.bytes8
+        if precision == PREC_FP4:
+            return self.bytes4
         return self.entries * {PREC_F64: 8, PREC_F32: 4}[precision]
 
-    def apply_fp8_h(self, r, z, stream=None):
-        """z = M r with the FP8 inverses, r in FP16 and z in FP32 (the
-        preconditioner of an FP16 GMRES basis)."""
-        if self.inv8 is None:
-            raise ValueError("no FP8 inverses (keep_fp8=True)")
-        check(self.lib.rtx_bjacobi_apply_fp8_h(self.op.plan, _vp(self.inv8), _vp(r), _vp(z),
-                                               stream_handle(stream)), "rtx_bjacobi_apply_fp8_h")
+    def apply_h(self, r, z, precision=PREC_FP8, stream=None):
+        """z = M r with the FP8 (PREC_FP8) or FP4 (PREC_FP4) inverses, r in
+        FP16 and z in FP32 (the preconditioner of an FP16 GMRES basis)."""
+        inv = {PREC_FP8: self.inv8, PREC_FP4: self.inv4}.get(precision)
+        if inv is None:
+            raise ValueError("no FP8 / FP4 inverses for precision {} (keep_fp8 / keep_fp4)"
+                             .format(precision))
+        check(self.lib.rtx_bjacobi_apply_h(self.op.plan, precision, _vp(inv), _vp(r), _vp(z),
+                                           stream_handle(stream)), "rtx_bjacobi_apply_h")
         return z
+
+    def apply_fp8_h(self, r, z, stream=None):
+        """apply_h with the FP8 inverses."""
+        return self.apply_h(r, z, PREC_FP8, stream)
 
     def apply(self, r, z, precision=PREC_F64, stream=None):
         """precision PREC_F64: FP64 inverses and vectors; PREC_F32: FP32;
         PREC_MIXED: BF16 inverses, FP32 vectors; PREC_FP8: FP8 (e4m3)
+        inverses with per-column scales, FP32 vectors; PREC_FP4: FP4 (e2m1)
         inverses with per-column scales, FP32 vectors."""
         inv = {PREC_F64: self.inv, PREC_F32: self.inv32, PREC_MIXED: self.inv16,
-               PREC_FP8: self.inv8}[precision]
+               PREC_FP8: self.inv8, PREC_FP4: self.inv4}[precision]
         if inv is None:
             raise ValueError("block-Jacobi inverses for this precision were not kept")
         check(self.lib.rtx_bjacobi_apply(self.op.plan, precision, _vp(inv), _vp(r), _vp(z),
@@ -183,18 +205,18 @@ class InteriorSystem:
 
     def precondition_h(self, r, z):
         """z = M r from an FP16 r into an FP32 z (FP16 basis of the inner
-        solves); the preconditioner must be the FP8 block-Jacobi."""
+        solves); the preconditioner must be the FP8 or FP4 block-Jacobi."""
         if self.pc is None:
             z.copy_(r)
             return z
-        if self.pc_precision != PREC_FP8 or not hasattr(self.pc, "apply_fp8_h"):
-            raise ValueError("an FP16 basis needs the FP8 block-Jacobi preconditioner")
-        return self.pc.apply_fp8_h(r, z)
+        if not self.fp16_basis_ok():
+            raise ValueError("an FP16 basis needs the FP8 or FP4 block-Jacobi preconditioner")
+        return self.pc.apply_h(r, z, self.pc_precision)
 
     def fp16_basis_ok(self):
         """Whether this reduced-precision system can serve an FP16 basis."""
         return (self.precision != PREC_F64 and self.pc is not None
-                and self.pc_precision == PREC_FP8 and hasattr(self.pc, "apply_fp8_h"))
+                and self.pc_precision in (PREC_FP8, PREC_FP4) and hasattr(self.pc, "apply_h"))
 
     def residual(self, b, x, r):
         """r = b - A_II x (always FP64)."""
@@ -489,8 +511,8 @@ def hessenberg_singular_values(H, k):
 
 def pick_basis(system_lo, flexible):
     """The inner solves' Arnoldi basis: FP16 when the reduced-precision
-    system's preconditioner is the FP8 block-Jacobi and the method is not
-    flexible, else FP32.  The inner solves stop at a relative 1e-2, well
+    system's preconditioner is the FP8 or FP4 block-Jacobi and the method is
+    not flexible, else FP32.  The inner solves stop at a relative 1e-2, well
     above FP16's 2^-11 rounding; on the uniform absorber (2,048,000 DOF) and
     the two-scatterer hp mesh (8,263,762 DOF) the inner iteration counts
     were those of the FP32 basis (200 and 202), with half the Gram-Schmidt
@@ -504,7 +526,7 @@ def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=
     """FP64 iterative refinement: r = b - A x in FP64 (``system_hi``), inner
     correction solve A d = r by GMRES whose operator and preconditioner run in
     reduced precision (``system_lo``) on a ``basis`` "f16", "f32" or "f64"
-    Arnoldi basis ("auto": FP16 when the preconditioner is the FP8
+    Arnoldi basis ("auto": FP16 when the preconditioner is the FP8 or FP4
     block-Jacobi and the method is not flexible, else FP32).  One GMRES
     object serves every inner solve, so its graphs are captured once.
     ``flexible`` selects flexible inner GMRES (for a nonlinear

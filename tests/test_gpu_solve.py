@@ -110,7 +110,8 @@ def test_refinement_mixed_reaches_fp64_residual(cuda):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("variant", ["plain", "flexible", "mixed", "mixed_f64_basis",
-                                     "mixed_f16_basis", "hp_plain", "hp_mixed", "hp_mixed_f16_basis"])
+                                     "mixed_f16_basis", "mixed_f16_basis_fp4", "hp_plain",
+                                     "hp_mixed", "hp_mixed_f16_basis", "hp_mixed_f16_basis_fp4"])
 def test_graph_replay_matches_eager(cuda, variant):
     """Chunks of GMRES iterations replayed as HIP graphs reproduce the eager
     run exactly (same kernels, same order): iterate, residual history and
@@ -120,7 +121,9 @@ def test_graph_replay_matches_eager(cuda, variant):
     import torch
     from rtx.operator import TransportOperator
     from rtx.krylov import BlockJacobi, InteriorSystem, gmres, refine
-    from rtx._lib import PREC_F64, PREC_F32, PREC_MIXED, PREC_FP8
+    from rtx._lib import PREC_F64, PREC_F32, PREC_MIXED, PREC_FP8, PREC_FP4
+    fp4 = variant.endswith("_fp4")
+    variant = variant[:-4] if fp4 else variant
     if variant.startswith("hp"):
         # hp mesh on the default (generic, atomic-free) launch shapes
         from test_gpu_apply import small_random_hp
@@ -132,13 +135,14 @@ def test_graph_replay_matches_eager(cuda, variant):
         tol, max_it = 1e-10, 500
     op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
     f16 = variant == "mixed_f16_basis"
-    pc = BlockJacobi(op, keep_f32=not f16, keep_fp8=f16)
+    pc = BlockJacobi(op, keep_f32=not f16, keep_fp8=f16 and not fp4, keep_fp4=f16 and fp4)
+    pc_lo = (PREC_FP4 if fp4 else PREC_FP8) if f16 else PREC_F32
     rhs = op.mask(_dev(np.random.default_rng(4).uniform(-1, 1, op.ndof)))
     runs = []
     for graphs in (False, True, True):
         if variant.startswith("mixed"):
             hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
-            lo = InteriorSystem(op, PREC_MIXED, pc, PREC_FP8 if f16 else PREC_F32)
+            lo = InteriorSystem(op, PREC_MIXED, pc, pc_lo)
             basis = {"mixed_f64_basis": "f64", "mixed_f16_basis": "f16"}.get(variant, "f32")
             x, info = refine(hi, lo, rhs, None, rtol=tol, inner_rtol=1e-2, restart=7,
                              max_inner=max_it, chunk=3, graphs=graphs, basis=basis)
@@ -321,20 +325,21 @@ def test_fp8_apply_fp16_input(cuda):
 
 
 @pytest.mark.gpu
-def test_refine_fp16_basis_matches_spsolve(cuda):
-    """Mixed iterative refinement with an FP16 inner basis and FP8
-    block-Jacobi (the default of mixed precision) reaches the same discrete
-    solution as spsolve (FP64 outer residual 1e-10)."""
+@pytest.mark.parametrize("bj", ["fp8", "fp4"])
+def test_refine_fp16_basis_matches_spsolve(cuda, bj):
+    """Mixed iterative refinement with an FP16 inner basis and FP8 or FP4
+    block-Jacobi (FP4 is the default of mixed precision) reaches the same
+    discrete solution as spsolve (FP64 outer residual 1e-10)."""
     from rtx.operator import TransportOperator
     from rtx.krylov import BlockJacobi, InteriorSystem, refine, pick_basis
-    from rtx._lib import PREC_F64, PREC_MIXED, PREC_FP8
+    from rtx._lib import PREC_F64, PREC_MIXED, PREC_FP8, PREC_FP4
     mesh, wl = _c1_small()
     A, f, b, mask = _oracle_system(mesh, wl)
     u_ref, _, _ = osys.exact_solve(A, f, b, mask)
     op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
-    pc = BlockJacobi(op, keep_f64=False, keep_fp8=True)
+    pc = BlockJacobi(op, keep_f64=False, keep_fp8=bj == "fp8", keep_fp4=bj == "fp4")
     hi = InteriorSystem(op, PREC_F64, None, PREC_F64)
-    lo = InteriorSystem(op, PREC_MIXED, pc, PREC_FP8)
+    lo = InteriorSystem(op, PREC_MIXED, pc, PREC_FP8 if bj == "fp8" else PREC_FP4)
     assert pick_basis(lo, False) == "f16" and pick_basis(lo, True) == "f32"
     bb = np.where(mask, 0.0, b)
     rhs_full = np.where(mask, f - A @ bb, 0.0)
@@ -603,3 +608,120 @@ def test_block_jacobi_fp8_apply(cuda):
     x, info = learned.Session(op, bj=pc, precision="mixed", bj_precision=PREC_FP8).solve(
         rhs, rtol=1e-8)
     assert info["converged_reason"] > 0 and info["rel_residual"] <= 1e-8
+
+
+def _e2m1_values():
+    """Magnitudes of the OCP FP4 e2m1 codes 0 .. 7 (exponent bits 2-1 with
+    bias 1, mantissa bit 0, a subnormal at exponent 0)."""
+    return np.array([(c & 1) / 2.0 if c < 2 else (1 + (c & 1) / 2.0) * 2.0 ** ((c >> 1) - 1)
+                     for c in range(8)])
+
+
+def _e2m1_code(a):
+    """Nearest e2m1 code (ties to the even code), saturating at 6: sign in bit 3."""
+    vals = _e2m1_values()
+    x = np.minimum(np.abs(np.asarray(a, dtype=np.float64)), 6.0)
+    i = np.clip(np.searchsorted(vals, x), 1, len(vals) - 1)
+    lo, hi = vals[i - 1], vals[i]
+    pick_hi = (x - lo > hi - x) | ((x - lo == hi - x) & (i % 2 == 0))
+    mag = np.where(pick_hi, i, i - 1)
+    return (mag | np.where((np.asarray(a) < 0) & (mag > 0), 8, 0)).astype(np.uint8)
+
+
+def _e2m1_decode(codes):
+    codes = np.asarray(codes, dtype=np.int64)
+    return np.where(codes & 8, -1.0, 1.0) * _e2m1_values()[codes & 7]
+
+
+def test_e2m1_helpers():
+    """The numpy e2m1 reference: the OCP table, ties to the even code,
+    saturation, sign and round trips."""
+    assert list(_e2m1_values()) == [0.0, 0.5, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0]
+    codes = np.arange(16, dtype=np.uint8)
+    back = _e2m1_code(_e2m1_decode(codes))
+    assert ((back == codes) | (codes == 8)).all()                 # -0 encodes as +0
+    # midpoints: 0.25 -> 0, 0.75 -> 1.0, 1.25 -> 1.0, 1.75 -> 2, 2.5 -> 2, 3.5 -> 4, 5 -> 4
+    mids = [0.25, 0.75, 1.25, 1.75, 2.5, 3.5, 5.0]
+    assert list(_e2m1_decode(_e2m1_code(mids))) == [0.0, 1.0, 1.0, 2.0, 2.0, 4.0, 4.0]
+    assert _e2m1_decode(_e2m1_code(7.9)) == 6.0 and _e2m1_decode(_e2m1_code(-2.9)) == -3.0
+
+
+@pytest.mark.gpu
+def test_block_jacobi_fp4_apply(cuda):
+    """FP4 block-Jacobi (e2m1 codes, two per byte with the even row in the
+    low nibble, one FP32 scale max|column| / 6 per column): the device pool
+    against the oracle's inverses quantised the same way in numpy, the apply
+    against that pool decoded in numpy (FP32 accumulation only), the apply
+    from an FP16 r against the FP32 one bit for bit, on an hp mesh with
+    split cells and on whole 125-DOF cells; and a mixed refinement solve with
+    it reaches 1e-8.  The third case keeps cells of up to 150 DOFs whole
+    (the LDS build), so the apply runs the slab kernel."""
+    import torch
+    from rtx.operator import TransportOperator, OP_SYSTEM, OP_QUADW
+    from rtx.krylov import BlockJacobi
+    from rtx._lib import PREC_FP4, check
+    from rtx import learned
+    from test_gpu_apply import small_random_hp
+    big = problems.uniform_mesh([1.0, 1.0], [4, 4, 5], 1, 2)
+    keys = sorted(big.cols)
+    big.ref_col(keys[3], "spt", "p")
+    big.ref_cell(keys[-1], sorted(big.cols[keys[-1]].cells)[0], "p")
+    cases = ((small_random_hp(2), problems.two_scatterers_c3(), 128, -1),
+             (problems.uniform_mesh([1.0, 1.0], [5, 5, 5], 2, 2), problems.absorber_c2(), None,
+              -1),
+             (big, problems.two_scatterers(), None, 0))
+    for mesh, wl, split, method in cases:
+        A, f, b, mask = _oracle_system(mesh, wl)
+        op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+        check(op.lib.rtx_bjacobi_set_method(op.plan, method), "set method")
+        pc = BlockJacobi(op, keep_f64=False, keep_fp4=True)
+        blocks = osys.bj_block_dofs(mesh, split)
+        invs = osys.block_jacobi_inverses(A, mesh, mask, split=split)
+        r = np.random.default_rng(8).uniform(-1, 1, op.ndof).astype(np.float32)
+        z = pc.apply(_dev(r), torch.zeros(op.ndof, dtype=torch.float32, device="cuda"),
+                     PREC_FP4).cpu().numpy()
+        r16 = _dev(r).to(torch.float16)
+        z_h = pc.apply_h(r16, torch.zeros(op.ndof, dtype=torch.float32, device="cuda"), PREC_FP4)
+        z_f = pc.apply(r16.float(), torch.zeros(op.ndof, dtype=torch.float32, device="cuda"),
+                       PREC_FP4)
+        assert torch.equal(z_h, z_f)
+        # the pool: block b's codes at off8[b] / 2 + j ldp / 2 (column j,
+        # ldp = 16 ceil(n / 16) rows, padding code 0), then the FP32 scales
+        # after the codes rounded up to 16 bytes, at soff[b] + j
+        pool = pc.inv4.cpu().numpy()
+        ns = np.array([len(i) for i in blocks])
+        ldps = (ns + 15) // 16 * 16
+        off4 = np.concatenate([[0], np.cumsum(ns * ldps // 2)])
+        soff = np.concatenate([[0], np.cumsum(ns)])
+        codes4 = (off4[-1] + 15) // 16 * 16
+        assert pool.size == codes4 + 4 * soff[-1]
+        scales = pool[codes4:].view(np.float32).astype(np.float64)
+        z_pool = np.zeros(op.ndof)
+        n_diff = n_all = 0
+        for bk, (idx, inv) in enumerate(zip(blocks, invs)):
+            n, ldp = ns[bk], ldps[bk]
+            by = pool[off4[bk]:off4[bk + 1]].reshape(n, ldp // 2).astype(np.int64)
+            cd = np.stack([by & 15, by >> 4], axis=2).reshape(n, ldp)   # [column][row]
+            assert not cd[:, n:].any(), "padding rows must hold code 0"
+            sc = scales[soff[bk]:soff[bk + 1]]
+            B = inv.astype(np.float32).astype(np.float64)
+            sc_ref = np.abs(B).max(axis=0) / 6.0
+            sc_ref = np.where(sc_ref > 0, sc_ref, 1.0)
+            assert np.allclose(sc, sc_ref, rtol=1e-6, atol=0)
+            c_dev = cd[:, :n].T                                         # [row][column]
+            c_ref = _e2m1_code(B / sc[None, :]).astype(np.int64)
+            signed = lambda c: np.where(c & 8, -1, 1) * (c & 7)
+            step = np.abs(signed(c_dev) - signed(c_ref))
+            # one step at most, where the device's FP32 quotient sits on a
+            # rounding boundary
+            assert (step <= 1).all()
+            n_diff += int((step != 0).sum())
+            n_all += step.size
+            z_pool[idx] = (_e2m1_decode(c_dev) * sc[None, :]) @ r[idx].astype(np.float64)
+        assert n_diff <= max(2, n_all // 1000), (n_diff, n_all)
+        assert np.linalg.norm(z - z_pool) / np.linalg.norm(z_pool) < 2e-6
+        fd = op.apply(torch.from_numpy(op.nodal(wl.f)).cuda(), ops=OP_QUADW)
+        bd = op.mask(torch.from_numpy(op.nodal(wl.bcs)).cuda(), keep_interior=False)
+        rhs = op.mask(fd - op.apply(bd, ops=OP_SYSTEM), keep_interior=True)
+        x, info = learned.Session(op, bj=pc, precision="mixed").solve(rhs, rtol=1e-8)
+        assert info["converged_reason"] > 0 and info["rel_residual"] <= 1e-8
