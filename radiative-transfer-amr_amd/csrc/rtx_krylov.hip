@@ -727,8 +727,9 @@ extern "C" int rtx_mdot(const double *d_V, int64_t ld, int nv, int64_t n, const 
     RTX_CHECK(d_V && d_w && d_out && d_work && nv >= 1 && nv < KMAXV, "rtx_mdot: bad argument");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int nchunk = (nv + KCH - 1) / KCH;
-    launch_mdot<double>((ld % 4 == 0) && aligned16(d_V) && aligned16(d_w), dim3(KNB, nchunk), st,
-                        d_V, ld, nv, d_w, n, d_work, 0, nullptr);
+    // vector loads need every V_j on 16 bytes: any ld for a single vector
+    const bool vec = (nv == 1 || ld % 4 == 0) && aligned16(d_V) && aligned16(d_w);
+    launch_mdot<double>(vec, dim3(KNB, nchunk), st, d_V, ld, nv, d_w, n, d_work, 0, nullptr);
     k_reduce<<<nv, KT, 0, st>>>(d_work, KNB, d_out, 0, nullptr);
     RTX_HIP(hipGetLastError());
     return RTX_OK;

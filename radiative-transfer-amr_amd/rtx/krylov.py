@@ -320,14 +320,17 @@ class GMRES:
         # basis storage; w and z are FP32 for either reduced-precision basis
         vt = {"f64": torch.float64, "f32": torch.float32, "f16": torch.float16}[basis]
         bt = torch.float64 if basis == "f64" else torch.float32
-        self.V = torch.empty((m + 1, n), dtype=vt, device=dev)
-        self.Z = torch.empty((m, n), dtype=bt, device=dev) if flexible else None
+        # basis rows padded to a multiple of 8 entries, so every vector starts
+        # on 16 bytes and the Gram-Schmidt kernels take their vector loads
+        # for any n (V and Z are views of the first n entries of each row)
+        self.ld = ld = (int(n) + 7) // 8 * 8
+        self.V = torch.empty((m + 1, ld), dtype=vt, device=dev)[:, :n]
+        self.Z = torch.empty((m, ld), dtype=bt, device=dev)[:, :n] if flexible else None
         self.w = torch.empty(n, dtype=bt, device=dev)
         self.z = torch.empty(n, dtype=bt, device=dev)
         self.r = torch.empty(n, dtype=torch.float64, device=dev)
         ws = C.c_int64()
         check(self.lib.rtx_arnoldi_workspace(m + 1, C.byref(ws)), "workspace")
-        # zero-filled: the single-launch reductions keep their counters here
         self.work = torch.zeros(int(ws.value), dtype=torch.float64, device=dev)
         self.hcol = torch.zeros(m + 2, dtype=torch.float64, device=dev)
         check(self.lib.rtx_gmres_state_size(m, C.byref(ws)), "gmres state size")
@@ -357,7 +360,7 @@ class GMRES:
             if self._on_direction is not None:
                 self._on_direction(zj, w)
             # Arnoldi step with the Givens update of column j folded in
-            check(lib.rtx_gmres_arnoldi(bp, _vp(V), n, n, _vp(w), _vp(self.hcol),
+            check(lib.rtx_gmres_arnoldi(bp, _vp(V), self.ld, n, _vp(w), _vp(self.hcol),
                                         _vp(self.work), self.refine, j, m, _vp(self.state), st),
                   "rtx_gmres_arnoldi")
 
@@ -448,24 +451,24 @@ class GMRES:
                 check(lib.rtx_gmres_solve(k, m, _vp(self.state), _vp(self.ydev), st),
                       "rtx_gmres_solve")
                 if self.basis == "f16":
-                    check(lib.rtx_combine_f16(_vp(V), n, k, _vp(self.ydev), n, _vp(w), 1, 0, st),
-                          "combine")
+                    check(lib.rtx_combine_f16(_vp(V), self.ld, k, _vp(self.ydev), n, _vp(w), 1, 0,
+                                              st), "combine")
                     system.precondition_lo(w, z)
                     check(lib.rtx_axpby_mixed(0, n, 1.0, _vp(z), 1.0, _vp(x), st), "axpby")
                 elif self.basis == "f32":
                     if self.flexible:
-                        check(lib.rtx_combine_f32(_vp(Z), n, k, _vp(self.ydev), n, _vp(x), 0, 1,
-                                                  st), "combine")
+                        check(lib.rtx_combine_f32(_vp(Z), self.ld, k, _vp(self.ydev), n, _vp(x),
+                                                  0, 1, st), "combine")
                     else:
-                        check(lib.rtx_combine_f32(_vp(V), n, k, _vp(self.ydev), n, _vp(w), 1, 0,
-                                                  st), "combine")
+                        check(lib.rtx_combine_f32(_vp(V), self.ld, k, _vp(self.ydev), n, _vp(w),
+                                                  1, 0, st), "combine")
                         system.precondition_lo(w, z)
                         check(lib.rtx_axpby_mixed(0, n, 1.0, _vp(z), 1.0, _vp(x), st), "axpby")
                 elif self.flexible:
-                    check(lib.rtx_combine(_vp(Z), n, k, _vp(self.ydev), n, _vp(x), 1, st),
+                    check(lib.rtx_combine(_vp(Z), self.ld, k, _vp(self.ydev), n, _vp(x), 1, st),
                           "combine")
                 else:
-                    check(lib.rtx_combine(_vp(V), n, k, _vp(self.ydev), n, _vp(w), 0, st),
+                    check(lib.rtx_combine(_vp(V), self.ld, k, _vp(self.ydev), n, _vp(w), 0, st),
                           "combine")
                     system.precondition(w, z)
                     check(lib.rtx_axpby(PREC_F64, n, 1.0, _vp(z), 1.0, _vp(x), st), "axpby")
