@@ -326,11 +326,14 @@ __global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restric
                                                          int *__restrict__ status) {
     constexpr int NR = 16 * RR;
     static_assert(NR == TJ * CC, "square register tiles");
+    static_assert(NR <= 128, "pivot rows fit the key's 7 low bits");
     __shared__ double cth[NMq], sth[NMq];
     __shared__ double colk[2][NR];
     __shared__ double prow[NR];
-    __shared__ double red_v[16];
-    __shared__ int red_i[16];
+    // pivot key of each step (double-buffered): the bits of |value| with the
+    // low 7 mantissa bits replaced by 127 - row, so the largest key is the
+    // largest value (to 45 mantissa bits), the lowest row among ties
+    __shared__ unsigned long long red_key[2];
     __shared__ int perm[NR], qinv[NR];
     __shared__ int used[NR];
     __shared__ BJSelf sl;
@@ -377,6 +380,7 @@ __global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restric
         sth[tid] = s;
     }
     if (tid < NR) used[tid] = 0;
+    if (tid < 2) red_key[tid] = 0ull;
     // the block: angles a0 .. a0 + g - 1 of the cell at every spatial node;
     // block index t <-> cell DOF (t / g) nth + a0 + t % g
     const int n = (x.n / x.nth) * g;
@@ -412,20 +416,20 @@ __global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restric
                 best = better ? av : best;
                 bi = better ? row : bi;
             }
-            red_v[ti] = best;
-            red_i[ti] = bi;
+            // no candidate (or a zero column): key 0, which fails below
+            const unsigned long long key =
+                (bi < n && best > 0.0)
+                    ? ((static_cast<unsigned long long>(__double_as_longlong(best)) & ~0x7Full) |
+                       static_cast<unsigned long long>(127 - bi))
+                    : 0ull;
+            atomicMax(&red_key[buf], key);
         }
         __syncthreads();
-        double best = -1.0;
-        int pr = n;
-        for (int s = 0; s < 16; ++s) {
-            const double bv = red_v[s];
-            const int bi = red_i[s];
-            if (bv > best || (bv == best && bi < pr)) {
-                best = bv;
-                pr = bi;
-            }
-        }
+        const unsigned long long key = red_key[buf];
+        const double best = __longlong_as_double(static_cast<long long>(key & ~0x7Full));
+        const int pr = 127 - int(key & 0x7Full);
+        // the other buffer is next read after this step's second barrier
+        if (tid == 0) red_key[buf ^ 1] = 0ull;
         if (!(best > 0.0) || pr >= n) {
             if (tid == 0) atomicMax(status, k + 1);
             return;                    // uniform across the workgroup
