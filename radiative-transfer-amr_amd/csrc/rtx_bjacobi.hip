@@ -499,19 +499,17 @@ __global__ void k_to_f32_bj(const double *__restrict__ s, float *__restrict__ d,
 // BF16 storage of an inverse entry, widened on load.
 template <typename T, typename M>
 __global__ __launch_bounds__(BJ_THREADS) void k_bj_apply(DevMesh m, const int64_t *inv_off,
-                                                        const int32_t *__restrict__ blk,
+                                                        const int4 *__restrict__ apd,
                                                         const M *__restrict__ inv,
                                                         const T *__restrict__ r,
                                                         T *__restrict__ z) {
     __shared__ double rs[RTX_NMAX * RTX_NMAX * RTX_NMAX];
     __shared__ double part[BJ_THREADS];
     const int b = blockIdx.x, tid = threadIdx.x;
-    const int k = blk[3 * b], a0 = blk[3 * b + 1], g = blk[3 * b + 2];
-    const int c = m.cell_col[k];
-    const int nth = m.cell_nth[k];
-    const int n = m.col_nx[c] * m.col_ny[c] * g;
+    const int4 d = apd[b];                // n, first DOF, cell angles, angles g
+    const int n = d.x, nth = d.z, g = d.w;
     // block index t <-> cell DOF (t / g) nth + a0 + t % g (identity for a whole cell)
-    const int off = m.cell_dof_off[k] + a0;
+    const int off = d.y;
     for (int i = tid; i < n; i += BJ_THREADS) rs[i] = double(r[off + (i / g) * nth + i % g]);
     __syncthreads();
     const M *B = inv + inv_off[b];
@@ -571,19 +569,17 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_pack_bf16(DevMesh m, const in
 // column; the CG partial sums of each row are added in LDS.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_bj_apply_bf16(DevMesh m, const int64_t *off16,
-                                                              const int32_t *__restrict__ blk,
+                                                              const int4 *__restrict__ apd,
                                                               const uint16_t *__restrict__ inv,
                                                               const float *__restrict__ r,
                                                               float *__restrict__ z) {
     __shared__ float rs[RTX_NMAX * RTX_NMAX * RTX_NMAX];
     __shared__ float part[NT * 8];
     const int b = blockIdx.x, tid = threadIdx.x;
-    const int k = blk[3 * b], a0 = blk[3 * b + 1], g = blk[3 * b + 2];
-    const int c = m.cell_col[k];
-    const int nth = m.cell_nth[k];
-    const int n = m.col_nx[c] * m.col_ny[c] * g;
+    const int4 d = apd[b];                // n, first DOF, cell angles, angles g
+    const int n = d.x, nth = d.z, g = d.w;
     const int ldp = (n + 7) / 8 * 8, RG = ldp / 8, CG = NT / RG;
-    const int off = m.cell_dof_off[k] + a0;
+    const int off = d.y;
     const int rg = tid % RG, cg = tid / RG;
     const uint4 *B = reinterpret_cast<const uint4 *>(inv + off16[b]) + rg;
     const int ldq = ldp / 8;
@@ -791,7 +787,7 @@ struct BjCodes<4> {
 template <int NT, typename RT = float, int BITS = 8>
 __global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *off8,
                                                      const int64_t *soff,
-                                                     const int32_t *__restrict__ blk,
+                                                     const int4 *__restrict__ apd,
                                                      const uint8_t *__restrict__ inv,
                                                      const float *__restrict__ scales,
                                                      const RT *__restrict__ r,
@@ -799,12 +795,10 @@ __global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *o
     __shared__ float rs[RTX_NMAX * RTX_NMAX * RTX_NMAX];
     __shared__ float part[NT * 16];
     const int b = blockIdx.x, tid = threadIdx.x;
-    const int k = blk[3 * b], a0 = blk[3 * b + 1], g = blk[3 * b + 2];
-    const int c = m.cell_col[k];
-    const int nth = m.cell_nth[k];
-    const int n = m.col_nx[c] * m.col_ny[c] * g;
+    const int4 d = apd[b];                // n, first DOF, cell angles, angles g
+    const int n = d.x, nth = d.z, g = d.w;
     const int ldp = (n + 15) / 16 * 16, RG = ldp / 16, CG = NT / RG;
-    const int off = m.cell_dof_off[k] + a0;
+    const int off = d.y;
     const int rg = tid % RG, cg = tid / RG;
     typedef BjCodes<BITS> Codes;
     typedef typename Codes::V CV;
@@ -855,7 +849,7 @@ constexpr int BJ_W_KMAX = 16;      // columns per lane: ceil(n / CG) <= 16 for n
 constexpr int BJ_W_WAVES = 4;
 template <typename RT = float, int BITS = 8>
 __global__ __launch_bounds__(64 * BJ_W_WAVES) void k_bj_apply_fp8_w(
-    DevMesh m, const int64_t *off8, const int64_t *soff, const int32_t *__restrict__ blk, int nblk,
+    DevMesh m, const int64_t *off8, const int64_t *soff, const int4 *__restrict__ apd, int nblk,
     const uint8_t *__restrict__ inv, const float *__restrict__ scales,
     const RT *__restrict__ r, float *__restrict__ z) {
     __shared__ float part[BJ_W_WAVES][BJ_W_MAXN * 8];   // CG x ldp <= 1024 per wave
@@ -863,13 +857,11 @@ __global__ __launch_bounds__(64 * BJ_W_WAVES) void k_bj_apply_fp8_w(
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int b = blockIdx.x * BJ_W_WAVES + wave;
     if (b >= nblk) return;                               // whole waves; no block barrier below
-    const int k = blk[3 * b], a0 = blk[3 * b + 1], g = blk[3 * b + 2];
-    const int c = m.cell_col[k];
-    const int nth = m.cell_nth[k];
-    const int n = m.col_nx[c] * m.col_ny[c] * g;
+    const int4 d = apd[b];                // n, first DOF, cell angles, angles g
+    const int n = d.x, nth = d.z, g = d.w;
     const int ldp = (n + 15) / 16 * 16, RG = ldp / 16, CG = 64 / RG;
     const int rg = lane % RG, cg = lane / RG;
-    const int off = m.cell_dof_off[k] + a0;
+    const int off = d.y;
     typedef BjCodes<BITS> Codes;
     typedef typename Codes::V CV;
     const CV *B = reinterpret_cast<const CV *>(inv + (off8[b] >> Codes::SHIFT)) + rg;
@@ -929,8 +921,10 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
     RTX_HIP(hipMemcpy(ny.data(), p->m.col_ny, ny.size() * 4, hipMemcpyDeviceToHost));
     RTX_HIP(hipMemcpy(cc.data(), p->m.cell_col, cc.size() * 4, hipMemcpyDeviceToHost));
     RTX_HIP(hipMemcpy(nt.data(), p->m.cell_nth, nt.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<int32_t> dof0(p->m.ncell);
+    RTX_HIP(hipMemcpy(dof0.data(), p->m.cell_dof_off, dof0.size() * 4, hipMemcpyDeviceToHost));
     const int split = (p->bj_method != 0) ? 1 : 0;
-    std::vector<int32_t> blk;
+    std::vector<int32_t> blk, apd;
     std::vector<int64_t> off(1, 0), off16(1, 0), off8(1, 0), soff(1, 0);
     int32_t maxb = 0;
     for (int k = 0; k < p->m.ncell; ++k) {
@@ -942,6 +936,10 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
             blk.push_back(k);
             blk.push_back(a0);
             blk.push_back(gg);
+            apd.push_back(int32_t(n));
+            apd.push_back(dof0[k] + a0);
+            apd.push_back(nth);
+            apd.push_back(gg);
             off.push_back(off.back() + n * n);
             off16.push_back(off16.back() + n * ((n + 7) / 8 * 8));
             off8.push_back(off8.back() + n * ((n + 15) / 16 * 16));
@@ -955,8 +953,9 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
         if (p->bj_off8) RTX_HIP(hipFree(p->bj_off8));
         if (p->bj_soff) RTX_HIP(hipFree(p->bj_soff));
         if (p->bj_blk) RTX_HIP(hipFree(p->bj_blk));
+        if (p->bj_apd) RTX_HIP(hipFree(p->bj_apd));
         p->bj_off = p->bj_off16 = p->bj_off8 = p->bj_soff = nullptr;
-        p->bj_blk = nullptr;
+        p->bj_blk = p->bj_apd = nullptr;
         RTX_HIP(hipMalloc(&p->bj_off, off.size() * sizeof(int64_t)));
         RTX_HIP(hipMemcpy(p->bj_off, off.data(), off.size() * sizeof(int64_t),
                           hipMemcpyHostToDevice));
@@ -971,6 +970,9 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
                           hipMemcpyHostToDevice));
         RTX_HIP(hipMalloc(&p->bj_blk, blk.size() * sizeof(int32_t)));
         RTX_HIP(hipMemcpy(p->bj_blk, blk.data(), blk.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice));
+        RTX_HIP(hipMalloc(&p->bj_apd, apd.size() * sizeof(int32_t)));
+        RTX_HIP(hipMemcpy(p->bj_apd, apd.data(), apd.size() * sizeof(int32_t),
                           hipMemcpyHostToDevice));
         p->bj_split = split;
         p->bj_ready = false;
@@ -1140,13 +1142,14 @@ static void launch_bj_coded(const rtx_plan *p, const void *d_inv, const RT *r, f
     const uint8_t *codes = static_cast<const uint8_t *>(d_inv);
     const float *scales =
         reinterpret_cast<const float *>(codes + (BITS == 4 ? p->bj_codes4 : p->bj_codes8));
+    const int4 *apd = reinterpret_cast<const int4 *>(p->bj_apd);
     if (p->bj_max_block <= BJ_W_MAXN)
         k_bj_apply_fp8_w<RT, BITS><<<(p->bj_nblk + BJ_W_WAVES - 1) / BJ_W_WAVES,
                                      64 * BJ_W_WAVES, 0, st>>>(
-            p->m, p->bj_off8, p->bj_soff, p->bj_blk, p->bj_nblk, codes, scales, r, z);
+            p->m, p->bj_off8, p->bj_soff, apd, p->bj_nblk, codes, scales, r, z);
     else
         k_bj_apply_fp8<BJ_APPLY8_THREADS, RT, BITS><<<p->bj_nblk, BJ_APPLY8_THREADS, 0, st>>>(
-            p->m, p->bj_off8, p->bj_soff, p->bj_blk, codes, scales, r, z);
+            p->m, p->bj_off8, p->bj_soff, apd, codes, scales, r, z);
 }
 
 extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, const void *d_r,
@@ -1157,16 +1160,17 @@ extern "C" int rtx_bjacobi_apply(rtx_plan *p, int precision, const void *d_inv, 
         set_error("rtx_bjacobi_apply: call rtx_bjacobi_factor first");
         return RTX_ERR_STATE;
     }
+    const int4 *apd = reinterpret_cast<const int4 *>(p->bj_apd);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (precision == RTX_PREC_F64)
         k_bj_apply<double, double><<<p->bj_nblk, BJ_THREADS, 0, st>>>(
-            p->m, p->bj_off, p->bj_blk, (const double *)d_inv, (const double *)d_r, (double *)d_z);
+            p->m, p->bj_off, apd, (const double *)d_inv, (const double *)d_r, (double *)d_z);
     else if (precision == RTX_PREC_F32)
         k_bj_apply<float, float><<<p->bj_nblk, BJ_THREADS, 0, st>>>(
-            p->m, p->bj_off, p->bj_blk, (const float *)d_inv, (const float *)d_r, (float *)d_z);
+            p->m, p->bj_off, apd, (const float *)d_inv, (const float *)d_r, (float *)d_z);
     else if (precision == RTX_PREC_MIXED)
         k_bj_apply_bf16<BJ_APPLY16_THREADS><<<p->bj_nblk, BJ_APPLY16_THREADS, 0, st>>>(
-            p->m, p->bj_off16, p->bj_blk, (const uint16_t *)d_inv, (const float *)d_r,
+            p->m, p->bj_off16, apd, (const uint16_t *)d_inv, (const float *)d_r,
             (float *)d_z);
     else if (precision == RTX_PREC_FP8)
         launch_bj_coded<8, float>(p, d_inv, (const float *)d_r, (float *)d_z, st);

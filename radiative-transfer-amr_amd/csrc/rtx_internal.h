@@ -207,6 +207,9 @@ struct rtx_plan {
     // blocks: (cell, first angle, angle count) triples; a cell of more than
     // BJ_SPLIT DOFs is split into groups of angles unless bj_method == 0
     int32_t *bj_blk = nullptr;
+    // per block, for the applies: (n, first DOF, cell angle count, angle
+    // count g), one 16-byte load instead of a chain through the mesh tables
+    int32_t *bj_apd = nullptr;
     int32_t bj_nblk = 0, bj_max_block = 0;
     int bj_split = -1;           // split setting the block table was built with
     // coarse grid of the learned correction: per column a list of
