@@ -243,14 +243,21 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
                     const int r = le - m1 * nt;
                     const double *Et = m.Eth_pool + sh.l_et[li] + r;
                     const A *trl = tr + sh.l_tr[li] + m1 * nt1;
-                    // the E^theta column's (at most RTX_NMAX) loads in flight together
-                    A ev[HP_NM];
+                    // the E^theta column's (at most RTX_NMAX) loads in flight
+                    // together: unconditional (rows past nt1 reload row
+                    // nt1 - 1), so that no conversion or wait sits between
+                    // them
+                    double ev[HP_NM];
 #pragma unroll
-                    for (int a = 0; a < HP_NM; ++a) ev[a] = (a < nt1) ? A(Et[a * nt]) : A(0);
+                    for (int a = 0; a < HP_NM; ++a) ev[a] = Et[(a < nt1 ? a : nt1 - 1) * nt];
+                    // the trace reads likewise unconditional; terms past nt1
+                    // are zero times a finite trace value
                     A acc = A(0);
 #pragma unroll
-                    for (int a = 0; a < HP_NM; ++a)
-                        if (a < nt1) acc += ev[a] * trl[a];
+                    for (int a = 0; a < HP_NM; ++a) {
+                        const A tv = trl[a < nt1 ? a : nt1 - 1];
+                        acc += (a < nt1 ? A(ev[a]) : A(0)) * tv;
+                    }
                     inn[e] = acc;
                 }
             }
@@ -289,14 +296,19 @@ __device__ __forceinline__ void hp_column(const DevMesh &m, uint32_t ops, int ac
                     const int n1 = w1 >> 2;
                     const A *Es = es + sh.l_es[li] + m0;
                     const A *in_ = inn + sh.l_inn[li] + r;
-                    // the E^s row's (at most RTX_NMAX) reads in flight together
-                    A ev[HP_NM];
+                    // the E^s row's and the traces' (at most RTX_NMAX each)
+                    // reads in flight together: unconditional, rows past n1
+                    // reread row n1 - 1 and add zero times a finite value
+                    A ev[HP_NM], iv[HP_NM];
 #pragma unroll
-                    for (int m1 = 0; m1 < HP_NM; ++m1) ev[m1] = (m1 < n1) ? Es[m1 * N] : A(0);
+                    for (int m1 = 0; m1 < HP_NM; ++m1) {
+                        const int mm = m1 < n1 ? m1 : n1 - 1;
+                        ev[m1] = Es[mm * N];
+                        iv[m1] = in_[mm * nt];
+                    }
                     A f = A(0);
 #pragma unroll
-                    for (int m1 = 0; m1 < HP_NM; ++m1)
-                        if (m1 < n1) f += ev[m1] * in_[m1 * nt];
+                    for (int m1 = 0; m1 < HP_NM; ++m1) f += (m1 < n1 ? ev[m1] : A(0)) * iv[m1];
                     sum += fxy * f;
                 }
                 fv[u] = sum;
