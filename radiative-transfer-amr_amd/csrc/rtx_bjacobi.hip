@@ -318,10 +318,14 @@ __device__ inline double bj_entry(const DevMesh &m, const BJCell &x, const BJSel
     return v;
 }
 
+// One workgroup per block, the block's index from the size-class list bl
+// (blocks of at most 16 RR DOFs).
 template <int RR, int CC, int TJ>
-__global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restrict_,
+__global__ __launch_bounds__(16 * TJ)
+__attribute__((amdgpu_waves_per_eu(TJ == 24 ? 5 : 1))) void k_bj_build_reg(DevMesh m, int restrict_,
                                                          const int64_t *inv_off,
                                                          const int32_t *__restrict__ blk,
+                                                         const int32_t *__restrict__ bl,
                                                          double *__restrict__ out,
                                                          int *__restrict__ status) {
     constexpr int NR = 16 * RR;
@@ -337,7 +341,7 @@ __global__ __launch_bounds__(16 * TJ) void k_bj_build_reg(DevMesh m, int restric
     __shared__ int perm[NR], qinv[NR];
     __shared__ int used[NR];
     __shared__ BJSelf sl;
-    const int b = blockIdx.x, tid = threadIdx.x, ti = tid / TJ, tj = tid % TJ;
+    const int b = bl[blockIdx.x], tid = threadIdx.x, ti = tid / TJ, tj = tid % TJ;
     const int k = blk[3 * b], a0 = blk[3 * b + 1], g = blk[3 * b + 2];
     BJCell x;
     const int c = m.cell_col[k];
@@ -924,7 +928,7 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
     std::vector<int32_t> dof0(p->m.ncell);
     RTX_HIP(hipMemcpy(dof0.data(), p->m.cell_dof_off, dof0.size() * 4, hipMemcpyDeviceToHost));
     const int split = (p->bj_method != 0) ? 1 : 0;
-    std::vector<int32_t> blk, apd;
+    std::vector<int32_t> blk, apd, cls[4];
     std::vector<int64_t> off(1, 0), off16(1, 0), off8(1, 0), soff(1, 0);
     int32_t maxb = 0;
     for (int k = 0; k < p->m.ncell; ++k) {
@@ -940,6 +944,7 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
             apd.push_back(dof0[k] + a0);
             apd.push_back(nth);
             apd.push_back(gg);
+            cls[n <= 32 ? 0 : n <= 64 ? 1 : n <= 96 ? 2 : 3].push_back(int32_t(blk.size() / 3 - 1));
             off.push_back(off.back() + n * n);
             off16.push_back(off16.back() + n * ((n + 7) / 8 * 8));
             off8.push_back(off8.back() + n * ((n + 15) / 16 * 16));
@@ -954,8 +959,9 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
         if (p->bj_soff) RTX_HIP(hipFree(p->bj_soff));
         if (p->bj_blk) RTX_HIP(hipFree(p->bj_blk));
         if (p->bj_apd) RTX_HIP(hipFree(p->bj_apd));
+        if (p->bj_cls) RTX_HIP(hipFree(p->bj_cls));
         p->bj_off = p->bj_off16 = p->bj_off8 = p->bj_soff = nullptr;
-        p->bj_blk = p->bj_apd = nullptr;
+        p->bj_blk = p->bj_apd = p->bj_cls = nullptr;
         RTX_HIP(hipMalloc(&p->bj_off, off.size() * sizeof(int64_t)));
         RTX_HIP(hipMemcpy(p->bj_off, off.data(), off.size() * sizeof(int64_t),
                           hipMemcpyHostToDevice));
@@ -974,6 +980,16 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
         RTX_HIP(hipMalloc(&p->bj_apd, apd.size() * sizeof(int32_t)));
         RTX_HIP(hipMemcpy(p->bj_apd, apd.data(), apd.size() * sizeof(int32_t),
                           hipMemcpyHostToDevice));
+        std::vector<int32_t> all;
+        for (int q = 0; q < 4; ++q) {
+            p->bj_cls_start[q] = int32_t(all.size());
+            all.insert(all.end(), cls[q].begin(), cls[q].end());
+        }
+        p->bj_cls_start[4] = int32_t(all.size());
+        RTX_HIP(hipMalloc(&p->bj_cls, std::max<size_t>(all.size(), 1) * sizeof(int32_t)));
+        if (!all.empty())
+            RTX_HIP(hipMemcpy(p->bj_cls, all.data(), all.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice));
         p->bj_split = split;
         p->bj_ready = false;
     }
@@ -1080,18 +1096,26 @@ extern "C" int rtx_bjacobi_factor(rtx_plan *p, int restrict_interior, double *d_
     RTX_HIP(hipMemsetAsync(d_status, 0, sizeof(int), st));
     const bool reg = p->bj_method != 0 && nmax <= 128;
     if (reg) {
-        if (nmax <= 32)
-            k_bj_build_reg<2, 2, 16><<<p->bj_nblk, 256, 0, st>>>(p->m, restrict_interior,
-                                                                p->bj_off, p->bj_blk, d_inv,
-                                                                d_status);
-        else if (nmax <= 64)
-            k_bj_build_reg<4, 4, 16><<<p->bj_nblk, 256, 0, st>>>(p->m, restrict_interior,
-                                                                p->bj_off, p->bj_blk, d_inv,
-                                                                d_status);
-        else
-            k_bj_build_reg<8, 4, 32><<<p->bj_nblk, 512, 0, st>>>(p->m, restrict_interior,
-                                                                p->bj_off, p->bj_blk, d_inv,
-                                                                d_status);
+        // each size class on the smallest register grid that holds it:
+        // fewer padded elements per elimination step, more blocks per CU
+        const int32_t *cs = p->bj_cls, *c0 = p->bj_cls_start;
+        auto cnt = [&](int q) { return c0[q + 1] - c0[q]; };
+        if (cnt(0))
+            k_bj_build_reg<2, 2, 16><<<cnt(0), 256, 0, st>>>(p->m, restrict_interior, p->bj_off,
+                                                            p->bj_blk, cs + c0[0], d_inv,
+                                                            d_status);
+        if (cnt(1))
+            k_bj_build_reg<4, 4, 16><<<cnt(1), 256, 0, st>>>(p->m, restrict_interior, p->bj_off,
+                                                            p->bj_blk, cs + c0[1], d_inv,
+                                                            d_status);
+        if (cnt(2))
+            k_bj_build_reg<6, 4, 24><<<cnt(2), 384, 0, st>>>(p->m, restrict_interior, p->bj_off,
+                                                            p->bj_blk, cs + c0[2], d_inv,
+                                                            d_status);
+        if (cnt(3))
+            k_bj_build_reg<8, 4, 32><<<cnt(3), 512, 0, st>>>(p->m, restrict_interior, p->bj_off,
+                                                            p->bj_blk, cs + c0[3], d_inv,
+                                                            d_status);
     } else if (p->bj_method == 1) {
         (void)hipFreeAsync(d_status, st);
         set_error("rtx_bjacobi_factor: the register build handles blocks of up to 128 DOFs, "

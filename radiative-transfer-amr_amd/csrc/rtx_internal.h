@@ -210,6 +210,10 @@ struct rtx_plan {
     // per block, for the applies: (n, first DOF, cell angle count, angle
     // count g), one 16-byte load instead of a chain through the mesh tables
     int32_t *bj_apd = nullptr;
+    // the register build's size classes (blocks of <= 32, 64, 96, 128
+    // DOFs): block indices, concatenated, with the class starts
+    int32_t *bj_cls = nullptr;
+    int32_t bj_cls_start[5] = {0, 0, 0, 0, 0};
     int32_t bj_nblk = 0, bj_max_block = 0;
     int bj_split = -1;           // split setting the block table was built with
     // coarse grid of the learned correction: per column a list of

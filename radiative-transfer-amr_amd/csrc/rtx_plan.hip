@@ -533,6 +533,7 @@ extern "C" void rtx_plan_destroy(rtx_plan *p) {
     if (p->bj_soff) (void)hipFree(p->bj_soff);
     if (p->bj_blk) (void)hipFree(p->bj_blk);
     if (p->bj_apd) (void)hipFree(p->bj_apd);
+    if (p->bj_cls) (void)hipFree(p->bj_cls);
     if (p->coarse) (void)hipFree(p->coarse);
     if (p->bins) (void)hipFree(p->bins);
     if (p->gen_alloc) (void)hipFree(p->gen_alloc);
