@@ -944,7 +944,8 @@ extern "C" int rtx_bjacobi_size(rtx_plan *p, int64_t *n_entries) {
             apd.push_back(dof0[k] + a0);
             apd.push_back(nth);
             apd.push_back(gg);
-            cls[n <= 32 ? 0 : n <= 64 ? 1 : n <= 96 ? 2 : 3].push_back(int32_t(blk.size() / 3 - 1));
+            const int q = n <= 32 ? 0 : n <= 64 ? 1 : n <= 96 ? 2 : 3;
+            cls[q].push_back(int32_t(blk.size() / 3 - 1));
             off.push_back(off.back() + n * n);
             off16.push_back(off16.back() + n * ((n + 7) / 8 * 8));
             off8.push_back(off8.back() + n * ((n + 15) / 16 * 16));
