@@ -652,27 +652,24 @@ __global__ __launch_bounds__(ETHREADS, ETHREADS / 64) void k_uniform_e(DevMesh m
         const int t = t0 + u * XS;
         xv[u] = (t < NCD) ? A(x[cdof + t]) : A(0);
     }
-    double r_c = 0, r_s = 0, r_w = 0, r_kap = 0, r_sig = 0, r_wd = 0, r_g = 0;
-    int r_bd = 0, r_q = -1, r_src = 0, r_meta = 0, r_es = 0, r_et = 0;
-    if (tid < NA) {
-        r_c = m.ang_trig[3 * (ga0 + tid)];
-        r_s = m.ang_trig[3 * (ga0 + tid) + 1];
-        r_w = m.ang_trig[3 * (ga0 + tid) + 2];
-    }
-    if (tid < NSP) {
-        r_kap = m.kappa[c * NSP + tid];
-        r_sig = m.sigma[c * NSP + tid];
-    }
-    if (tid < N * N) {
-        const int p = tid / N, i = tid % N;
-        r_wd = m.lgl_w[N * UNM + i] * m.lgl_D[(size_t(N) * UNM + p) * UNM + i];
-    }
-    if (tid < NC) {
-        const int k = k0 + tid;
-        r_bd = m.cell_bdry[k];
-        r_q = m.cell_quad[k];
-        r_g = m.cell_geom[2 * k + 1] - m.cell_geom[2 * k];
-    }
+    // the tables' loads are unconditional (indices clamped to the last
+    // entry; only threads in range store them below), so they all go out
+    // before the first wait instead of one branch's loads at a time
+    const int tA = tid < NA ? tid : NA - 1, tS = tid < NSP ? tid : NSP - 1;
+    const int tD = tid < N * N ? tid : N * N - 1, tC = tid < NC ? tid : NC - 1;
+    const double r_c = m.ang_trig[3 * (ga0 + tA)];
+    const double r_s = m.ang_trig[3 * (ga0 + tA) + 1];
+    const double r_w = m.ang_trig[3 * (ga0 + tA) + 2];
+    const double r_kap = m.kappa[c * NSP + tS];
+    const double r_sig = m.sigma[c * NSP + tS];
+    const double r_wd = m.lgl_w[N * UNM + tD % N] *
+                        m.lgl_D[(size_t(N) * UNM + tD / N) * UNM + tD % N];
+    const int r_bd = m.cell_bdry[k0 + tC];
+    const int r_q = m.cell_quad[k0 + tC];
+    const double r_g = m.cell_geom[2 * (k0 + tC) + 1] - m.cell_geom[2 * (k0 + tC)];
+    const double r_wp = m.lgl_w[N * UNM + tS / N], r_wq = m.lgl_w[N * UNM + tS % N];
+    const double r_wx = m.lgl_w[N * UNM + (tid < N ? tid : N - 1)];
+    int r_src = 0, r_meta = 0, r_es = 0, r_et = 0;
     if (bconv && tid < nl) {
         const int l = L0 + tid;
         r_src = m.link_meta[4 * l + 0];
@@ -687,10 +684,10 @@ __global__ __launch_bounds__(ETHREADS, ETHREADS / 64) void k_uniform_e(DevMesh m
     }
     if (tid < NSP) {
         kap[tid] = A(r_kap);
-        sgw[tid] = A(dx * dy * m.lgl_w[N * UNM + tid / N] * m.lgl_w[N * UNM + tid % N] * r_sig);
+        sgw[tid] = A(dx * dy * r_wp * r_wq * r_sig);
     }
     if (tid < N * N) WD[tid] = A(r_wd);
-    if (tid < N) wx_[tid] = A(m.lgl_w[N * UNM + tid]);
+    if (tid < N) wx_[tid] = A(r_wx);
     if (tid < NSP) nfc[tid] = u_node_faces(tid / N, tid % N, N);
     if (tid < NC) {
         cbd[tid] = restrict_ ? r_bd : 0;
