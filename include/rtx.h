@@ -212,8 +212,11 @@ int rtx_assemble_entries(rtx_plan *plan, int part, int64_t n, const int32_t *d_k
 int rtx_bjacobi_size(rtx_plan *plan, int64_t *n_entries);
 /* Assemble and invert every cell block (FP64, Gauss-Jordan with partial
    pivoting) into d_inv (column-major per cell); optional FP32 copy into
-   d_inv_f32.  d_scratch (same size as d_inv) is needed only when a block does
-   not fit LDS.  Synchronises the stream; RTX_ERR_NUMERIC on a singular block. */
+   d_inv_f32.  With d_inv null the blocks are eliminated in FP32 straight into
+   d_inv_f32 (register build only: blocks of up to 128 DOFs, else
+   RTX_ERR_ARG), for callers that keep only reduced-precision copies.
+   d_scratch (same size as d_inv) is needed only when a block does not fit
+   LDS.  Synchronises the stream; RTX_ERR_NUMERIC on a singular block. */
 int rtx_bjacobi_factor(rtx_plan *plan, int restrict_interior, double *d_inv, float *d_inv_f32,
                        double *d_scratch, void *stream);
 /* Build method: -1 automatic (register-resident sweeps for blocks of up to

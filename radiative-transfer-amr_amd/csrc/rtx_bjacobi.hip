@@ -319,21 +319,24 @@ __device__ inline double bj_entry(const DevMesh &m, const BJCell &x, const BJSel
 }
 
 // One workgroup per block, the block's index from the size-class list bl
-// (blocks of at most 16 RR DOFs).
-template <int RR, int CC, int TJ>
+// (blocks of at most 16 RR DOFs).  T: the elimination's arithmetic and the
+// inverse's storage (FP64, or FP32 for the reduced-precision copies alone;
+// the entries are assembled in FP64 either way).
+template <int RR, int CC, int TJ, typename T = double>
 __global__ __launch_bounds__(16 * TJ)
-__attribute__((amdgpu_waves_per_eu(TJ == 24 ? 5 : 1))) void k_bj_build_reg(DevMesh m, int restrict_,
+__attribute__((amdgpu_waves_per_eu(TJ == 24 ? 5 : (TJ == 32 && sizeof(T) == 4 ? 6 : 1))))
+void k_bj_build_reg(DevMesh m, int restrict_,
                                                          const int64_t *inv_off,
                                                          const int32_t *__restrict__ blk,
                                                          const int32_t *__restrict__ bl,
-                                                         double *__restrict__ out,
+                                                         T *__restrict__ out,
                                                          int *__restrict__ status) {
     constexpr int NR = 16 * RR;
     static_assert(NR == TJ * CC, "square register tiles");
     static_assert(NR <= 128, "pivot rows fit the key's 7 low bits");
     __shared__ double cth[NMq], sth[NMq];
-    __shared__ double colk[2][NR];
-    __shared__ double prow[NR];
+    __shared__ T colk[2][NR];
+    __shared__ T prow[NR];
     // pivot key of each step (double-buffered): the bits of |value| with the
     // low 7 mantissa bits replaced by 127 - row, so the largest key is the
     // largest value (to 45 mantissa bits), the lowest row among ties
@@ -389,16 +392,16 @@ __attribute__((amdgpu_waves_per_eu(TJ == 24 ? 5 : 1))) void k_bj_build_reg(DevMe
     // block index t <-> cell DOF (t / g) nth + a0 + t % g
     const int n = (x.n / x.nth) * g;
     __syncthreads();
-    double A[RR][CC];
+    T A[RR][CC];
 #pragma unroll
     for (int u = 0; u < RR; ++u)
 #pragma unroll
         for (int v = 0; v < CC; ++v) {
             const int row = ti + 16 * u, col = tj + TJ * v;
             A[u][v] = (row < n && col < n)
-                          ? bj_entry(m, x, sl, cth, sth, (row / g) * x.nth + a0 + row % g,
-                                     (col / g) * x.nth + a0 + col % g)
-                          : 0.0;
+                          ? T(bj_entry(m, x, sl, cth, sth, (row / g) * x.nth + a0 + row % g,
+                                       (col / g) * x.nth + a0 + col % g))
+                          : T(0);
         }
     // Every step is written without per-element branches (selects around
     // one FMA per element), so the wave runs straight-line code.
@@ -409,13 +412,13 @@ __attribute__((amdgpu_waves_per_eu(TJ == 24 ? 5 : 1))) void k_bj_build_reg(DevMe
             int bi = n;
 #pragma unroll
             for (int u = 0; u < RR; ++u) {
-                double val = A[u][0];
+                T val = A[u][0];
 #pragma unroll
                 for (int v = 1; v < CC; ++v) val = (v == vk) ? A[u][v] : val;
                 const int row = ti + 16 * u;
                 const bool ok = row < n;
                 if (ok) colk[buf][row] = val;
-                const double av = fabs(val);
+                const double av = fabs(double(val));
                 const bool better = ok && !used[ok ? row : 0] && (av > best || (av == best && row < bi));
                 best = better ? av : best;
                 bi = better ? row : bi;
@@ -442,7 +445,7 @@ __attribute__((amdgpu_waves_per_eu(TJ == 24 ? 5 : 1))) void k_bj_build_reg(DevMe
             const int up = pr >> 4;
 #pragma unroll
             for (int v = 0; v < CC; ++v) {
-                double pval = A[0][v];
+                T pval = A[0][v];
 #pragma unroll
                 for (int u = 1; u < RR; ++u) pval = (u == up) ? A[u][v] : pval;
                 const int col = tj + TJ * v;
@@ -458,24 +461,24 @@ __attribute__((amdgpu_waves_per_eu(TJ == 24 ? 5 : 1))) void k_bj_build_reg(DevMe
         // kk; other rows -> row - c_r (pivot row / p) with -c_r / p in column
         // kk.  With the pivot row's kk entry taken as 1 and the other rows'
         // kk entries as 0 this is one FMA per element.
-        const double invp = 1.0 / colk[buf][pr];
-        double pv[CC];
+        const T invp = T(1) / colk[buf][pr];
+        T pv[CC];
 #pragma unroll
         for (int v = 0; v < CC; ++v) {
             const int col = tj + TJ * v;
-            const double pr_v = (col == kk) ? 1.0 : ((col < n) ? prow[col] : 0.0);
+            const T pr_v = (col == kk) ? T(1) : ((col < n) ? prow[col] : T(0));
             pv[v] = pr_v * invp;
         }
 #pragma unroll
         for (int u = 0; u < RR; ++u) {
             const int row = ti + 16 * u;
             const bool isp = row == pr;
-            const double ck = (row < n) ? colk[buf][row] : 0.0;
+            const T ck = (row < n) ? colk[buf][row] : T(0);
 #pragma unroll
             for (int v = 0; v < CC; ++v) {
                 const int col = tj + TJ * v;
-                const double base = (col == kk) ? 0.0 : A[u][v];
-                const double upd = fma(-ck, pv[v], base);
+                const T base = (col == kk) ? T(0) : A[u][v];
+                const T upd = fma(-ck, pv[v], base);
                 A[u][v] = isp ? pv[v] : upd;
             }
         }
@@ -483,7 +486,7 @@ __attribute__((amdgpu_waves_per_eu(TJ == 24 ? 5 : 1))) void k_bj_build_reg(DevMe
     __syncthreads();
     if (tid < n) qinv[perm[tid]] = tid;
     __syncthreads();
-    double *o = out + inv_off[b];
+    T *o = out + inv_off[b];
 #pragma unroll
     for (int u = 0; u < RR; ++u)
 #pragma unroll
@@ -1075,7 +1078,7 @@ extern "C" int rtx_bjacobi_pack_bf16(rtx_plan *p, const float *d_inv32, uint16_t
 
 extern "C" int rtx_bjacobi_factor(rtx_plan *p, int restrict_interior, double *d_inv,
                                   float *d_inv_f32, double *d_scratch, void *stream) {
-    RTX_CHECK(p && d_inv, "rtx_bjacobi_factor: null argument");
+    RTX_CHECK(p && (d_inv || d_inv_f32), "rtx_bjacobi_factor: null argument");
     if (!p->has_coeffs) {
         set_error("rtx_bjacobi_factor: coefficients not set");
         return RTX_ERR_STATE;
@@ -1096,27 +1099,40 @@ extern "C" int rtx_bjacobi_factor(rtx_plan *p, int restrict_interior, double *d_
     RTX_HIP(hipMallocAsync((void **)&d_status, sizeof(int), st));
     RTX_HIP(hipMemsetAsync(d_status, 0, sizeof(int), st));
     const bool reg = p->bj_method != 0 && nmax <= 128;
+    // no FP64 inverse wanted: the register build eliminates in FP32 straight
+    // into d_inv_f32 (the FP32, BF16, FP8 and FP4 copies all start there)
+    const bool f32_only = d_inv == nullptr;
+    if (f32_only && !reg) {
+        (void)hipFreeAsync(d_status, st);
+        set_error("rtx_bjacobi_factor: an FP32-only factorisation needs the register build "
+                  "(blocks of up to 128 DOFs, method -1 or 1); blocks of %lld DOFs",
+                  (long long)nmax);
+        return RTX_ERR_ARG;
+    }
     if (reg) {
         // each size class on the smallest register grid that holds it:
         // fewer padded elements per elimination step, more blocks per CU
         const int32_t *cs = p->bj_cls, *c0 = p->bj_cls_start;
         auto cnt = [&](int q) { return c0[q + 1] - c0[q]; };
-        if (cnt(0))
-            k_bj_build_reg<2, 2, 16><<<cnt(0), 256, 0, st>>>(p->m, restrict_interior, p->bj_off,
-                                                            p->bj_blk, cs + c0[0], d_inv,
-                                                            d_status);
-        if (cnt(1))
-            k_bj_build_reg<4, 4, 16><<<cnt(1), 256, 0, st>>>(p->m, restrict_interior, p->bj_off,
-                                                            p->bj_blk, cs + c0[1], d_inv,
-                                                            d_status);
-        if (cnt(2))
-            k_bj_build_reg<6, 4, 24><<<cnt(2), 384, 0, st>>>(p->m, restrict_interior, p->bj_off,
-                                                            p->bj_blk, cs + c0[2], d_inv,
-                                                            d_status);
-        if (cnt(3))
-            k_bj_build_reg<8, 4, 32><<<cnt(3), 512, 0, st>>>(p->m, restrict_interior, p->bj_off,
-                                                            p->bj_blk, cs + c0[3], d_inv,
-                                                            d_status);
+        auto run = [&](auto *out) {
+            using T = typename std::remove_pointer<decltype(out)>::type;
+            if (cnt(0))
+                k_bj_build_reg<2, 2, 16, T><<<cnt(0), 256, 0, st>>>(
+                    p->m, restrict_interior, p->bj_off, p->bj_blk, cs + c0[0], out, d_status);
+            if (cnt(1))
+                k_bj_build_reg<4, 4, 16, T><<<cnt(1), 256, 0, st>>>(
+                    p->m, restrict_interior, p->bj_off, p->bj_blk, cs + c0[1], out, d_status);
+            if (cnt(2))
+                k_bj_build_reg<6, 4, 24, T><<<cnt(2), 384, 0, st>>>(
+                    p->m, restrict_interior, p->bj_off, p->bj_blk, cs + c0[2], out, d_status);
+            if (cnt(3))
+                k_bj_build_reg<8, 4, 32, T><<<cnt(3), 512, 0, st>>>(
+                    p->m, restrict_interior, p->bj_off, p->bj_blk, cs + c0[3], out, d_status);
+        };
+        if (f32_only)
+            run(d_inv_f32);
+        else
+            run(d_inv);
     } else if (p->bj_method == 1) {
         (void)hipFreeAsync(d_status, st);
         set_error("rtx_bjacobi_factor: the register build handles blocks of up to 128 DOFs, "
@@ -1139,7 +1155,7 @@ extern "C" int rtx_bjacobi_factor(rtx_plan *p, int restrict_interior, double *d_
                                                            d_inv, d_scratch, d_status);
     }
     RTX_HIP(hipGetLastError());
-    if (d_inv_f32)
+    if (d_inv_f32 && !f32_only)
         k_to_f32_bj<<<2048, 256, 0, st>>>(d_inv, d_inv_f32, p->bj_entries);
     int h_status = 0;
     RTX_HIP(hipMemcpyAsync(&h_status, d_status, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -56,13 +56,23 @@ class BlockJacobi:
         check(self.lib.rtx_bjacobi_size(op.plan, C.byref(n)), "rtx_bjacobi_size")
         self.entries = int(n.value)
         dev = op.device
-        self.inv = torch.empty(self.entries, dtype=torch.float64, device=dev)
         # BF16 / FP8 / FP4 are rounded from FP32
         need32 = keep_f32 or keep_bf16 or keep_fp8 or keep_fp4
         self.inv32 = torch.empty(self.entries, dtype=torch.float32, device=dev) if need32 else None
         st = stream_handle()
-        rc = self.lib.rtx_bjacobi_factor(op.plan, 1 if restrict else 0, _vp(self.inv),
-                                         _vp(self.inv32) if need32 else None, None, st)
+        rc = -1
+        if need32 and not keep_f64:
+            # no FP64 copy wanted: eliminate in FP32 straight into inv32
+            # (register build, blocks of up to 128 DOFs; else the FP64 path)
+            self.inv = None
+            rc = self.lib.rtx_bjacobi_factor(op.plan, 1 if restrict else 0, None,
+                                             _vp(self.inv32), None, st)
+            if rc != 0 and not (rc == -1 and b"FP32-only" in self.lib.rtx_last_error()):
+                check(rc, "rtx_bjacobi_factor")
+        if rc != 0:
+            self.inv = torch.empty(self.entries, dtype=torch.float64, device=dev)
+            rc = self.lib.rtx_bjacobi_factor(op.plan, 1 if restrict else 0, _vp(self.inv),
+                                             _vp(self.inv32) if need32 else None, None, st)
         if rc == -1 and b"LDS" in self.lib.rtx_last_error():
             scratch = torch.empty_like(self.inv)
             rc = self.lib.rtx_bjacobi_factor(op.plan, 1 if restrict else 0, _vp(self.inv),

@@ -534,12 +534,34 @@ def test_e4m3_helpers():
     assert _e4m3_code(-3.0) == 0x80 | 0x44 and _e4m3_round(-3.0) == -3.0
 
 
+def _device_inverses(pc, blocks, invs):
+    """The device's FP32 inverses (column-major per block, blocks back to
+    back) as FP64 [row][column] arrays, each checked against the oracle's
+    FP64 inverse to the accuracy of an FP32 elimination: relative error
+    (Frobenius) at most 8 n u cond(B) with u the FP32 unit roundoff, and at
+    most 1e-3 whatever the condition number."""
+    inv32 = pc.inv32.cpu().numpy().astype(np.float64)
+    out, off = [], 0
+    for idx, inv in zip(blocks, invs):
+        n = len(idx)
+        B = inv32[off:off + n * n].reshape(n, n).T
+        off += n * n
+        err = np.linalg.norm(B - inv) / np.linalg.norm(inv)
+        bound = 8 * n * 2.0 ** -24 * np.linalg.cond(inv)
+        assert err <= min(bound, 1e-3), (n, err, bound)
+        out.append(B)
+    assert off == inv32.size
+    return out
+
+
 @pytest.mark.gpu
 def test_block_jacobi_fp8_apply(cuda):
     """FP8 block-Jacobi (e4m3 codes, one FP32 scale max|column| / 448 per
     column) against the oracle's inverses quantised the same way in numpy
     and applied in FP64 (to FP32 accumulation), on an hp mesh with split
-    cells; and a mixed refinement solve with it reaches 1e-8."""
+    cells; the device's FP32-eliminated inverses against the oracle's, and
+    the pool against those inverses quantised in numpy; and a mixed
+    refinement solve with it reaches 1e-8."""
     import torch
     from rtx.operator import TransportOperator, OP_SYSTEM, OP_QUADW
     from rtx.krylov import BlockJacobi
@@ -550,8 +572,9 @@ def test_block_jacobi_fp8_apply(cuda):
     wl = problems.two_scatterers_c3()
     A, f, b, mask = _oracle_system(mesh, wl)
     op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
-    pc = BlockJacobi(op, keep_f64=False, keep_fp8=True)
+    pc = BlockJacobi(op, keep_f64=False, keep_f32=True, keep_fp8=True)
     invs = osys.block_jacobi_inverses(A, mesh, mask, split=128)
+    dinvs = _device_inverses(pc, osys.bj_block_dofs(mesh, 128), invs)
     r = np.random.default_rng(8).uniform(-1, 1, op.ndof).astype(np.float32)
     z_ref = np.zeros(op.ndof)
     for idx, inv in zip(osys.bj_block_dofs(mesh, 128), invs):
@@ -579,12 +602,11 @@ def test_block_jacobi_fp8_apply(cuda):
     scales = pool[codes8:].view(np.float32).astype(np.float64)
     z_pool = np.zeros(op.ndof)
     n_diff = n_all = 0
-    for b, (idx, inv) in enumerate(zip(blocks, invs)):
+    for b, (idx, B) in enumerate(zip(blocks, dinvs)):
         n, ldp = ns[b], ldps[b]
         cd = pool[off8[b]:off8[b + 1]].reshape(n, ldp)           # [column][row]
         assert not cd[:, n:].any(), "padding rows must hold code 0"
         sc = scales[soff[b]:soff[b + 1]]
-        B = inv.astype(np.float32).astype(np.float64)
         sc_ref = np.abs(B).max(axis=0) / 448.0
         sc_ref = np.where(sc_ref > 0, sc_ref, 1.0)
         assert np.allclose(sc, sc_ref, rtol=1e-6, atol=0)
@@ -594,7 +616,7 @@ def test_block_jacobi_fp8_apply(cuda):
         signed = lambda c: np.where(c & 0x80, -1, 1) * (c & 0x7F)
         step = np.abs(signed(c_dev) - signed(c_ref))
         # a code may differ from the numpy one only by one step, where the
-        # device's FP32 inverse and quotient sit on a rounding boundary
+        # device's FP32 quotient sits on a rounding boundary
         assert (step <= 1).all()
         n_diff += int((step != 0).sum())
         n_all += step.size
@@ -649,13 +671,15 @@ def test_e2m1_helpers():
 @pytest.mark.gpu
 def test_block_jacobi_fp4_apply(cuda):
     """FP4 block-Jacobi (e2m1 codes, two per byte with the even row in the
-    low nibble, one FP32 scale max|column| / 6 per column): the device pool
-    against the oracle's inverses quantised the same way in numpy, the apply
+    low nibble, one FP32 scale max|column| / 6 per column): the device's
+    FP32 inverses against the oracle's, the pool against the device
+    inverses quantised the same way in numpy, the apply
     against that pool decoded in numpy (FP32 accumulation only), the apply
     from an FP16 r against the FP32 one bit for bit, on an hp mesh with
     split cells and on whole 125-DOF cells; and a mixed refinement solve with
-    it reaches 1e-8.  The third case keeps cells of up to 150 DOFs whole
-    (the LDS build), so the apply runs the slab kernel."""
+    it reaches 1e-8.  The first two cases eliminate in FP32 (register
+    build); the third keeps cells of up to 150 DOFs whole (the LDS build,
+    in FP64 then rounded), so the apply runs the slab kernel."""
     import torch
     from rtx.operator import TransportOperator, OP_SYSTEM, OP_QUADW
     from rtx.krylov import BlockJacobi
@@ -674,9 +698,10 @@ def test_block_jacobi_fp4_apply(cuda):
         A, f, b, mask = _oracle_system(mesh, wl)
         op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
         check(op.lib.rtx_bjacobi_set_method(op.plan, method), "set method")
-        pc = BlockJacobi(op, keep_f64=False, keep_fp4=True)
+        pc = BlockJacobi(op, keep_f64=False, keep_f32=True, keep_fp4=True)
         blocks = osys.bj_block_dofs(mesh, split)
         invs = osys.block_jacobi_inverses(A, mesh, mask, split=split)
+        dinvs = _device_inverses(pc, blocks, invs)
         r = np.random.default_rng(8).uniform(-1, 1, op.ndof).astype(np.float32)
         z = pc.apply(_dev(r), torch.zeros(op.ndof, dtype=torch.float32, device="cuda"),
                      PREC_FP4).cpu().numpy()
@@ -698,13 +723,12 @@ def test_block_jacobi_fp4_apply(cuda):
         scales = pool[codes4:].view(np.float32).astype(np.float64)
         z_pool = np.zeros(op.ndof)
         n_diff = n_all = 0
-        for bk, (idx, inv) in enumerate(zip(blocks, invs)):
+        for bk, (idx, B) in enumerate(zip(blocks, dinvs)):
             n, ldp = ns[bk], ldps[bk]
             by = pool[off4[bk]:off4[bk + 1]].reshape(n, ldp // 2).astype(np.int64)
             cd = np.stack([by & 15, by >> 4], axis=2).reshape(n, ldp)   # [column][row]
             assert not cd[:, n:].any(), "padding rows must hold code 0"
             sc = scales[soff[bk]:soff[bk + 1]]
-            B = inv.astype(np.float32).astype(np.float64)
             sc_ref = np.abs(B).max(axis=0) / 6.0
             sc_ref = np.where(sc_ref > 0, sc_ref, 1.0)
             assert np.allclose(sc, sc_ref, rtol=1e-6, atol=0)
