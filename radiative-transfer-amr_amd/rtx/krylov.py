@@ -375,33 +375,48 @@ class GMRES:
                   "rtx_gmres_arnoldi")
 
     def solve(self, b, x=None, rtol=1e-8, atol=0.0, max_it=5000, callback=None,
-              on_restart=None, on_direction=None):
+              on_restart=None, on_direction=None, bnorm=None, x_zero=False,
+              trust_estimate=False):
         """Converged when |b - A x| <= max(rtol |b|, atol) for the FP64 true
         residual.  ``on_restart(x, r)`` sees the iterate and its true residual
         at the start of every cycle; ``on_direction(z, w)`` sees every
         preconditioned direction z and w = A z before orthogonalisation (an
         exact residual / correction pair); either hook (or ``callback``)
         runs the iterations eagerly.  Returns (x, info) with PETSc-like
-        convergence information."""
+        convergence information.
+
+        For the inner solves of iterative refinement: ``bnorm`` passes |b|
+        when the caller has it; ``x_zero`` says x is zero, so the first
+        cycle starts from r = b without an operator apply; with
+        ``trust_estimate`` the solve ends after the cycle whose residual
+        estimate reaches the target, without forming the true residual
+        (``res_f`` is then that estimate; the outer FP64 residual checks the
+        step).  Each saves an FP64 operator apply and a host read per inner
+        solve."""
         system, lib = self.system, self.lib
         n, m = self.n, self.m
         V, Z, w, z, r = self.V, self.Z, self.w, self.z, self.r
         x = torch.zeros_like(b) if x is None else x
         use_graphs = self.graphs and callback is None and on_direction is None
         self._on_direction = on_direction
-        bnorm = _norm(lib, b, self.work, self.scal)
+        bnorm = _norm(lib, b, self.work, self.scal) if bnorm is None else float(bnorm)
         target = max(rtol * bnorm, atol)
         history, true_hist = [], []
         its, reason, k_last, stop = 0, 0, 0, None
         est_met = False          # the last cycle's estimate reached the target
         floor = False
+        first = True
         t0 = time.perf_counter()
         while True:
-            system.residual(b, x, r)
-            beta = _norm(lib, r, self.work, self.scal)
+            if first and x_zero:
+                rv, beta = b, bnorm               # r = b - A 0
+            else:
+                system.residual(b, x, r)
+                rv, beta = r, _norm(lib, r, self.work, self.scal)
+            first = False
             true_hist.append(beta)
             if on_restart is not None:
-                on_restart(x, r)
+                on_restart(x, rv)
             if its == 0:
                 history.append(beta)
             if beta <= target:
@@ -421,11 +436,11 @@ class GMRES:
                 break
             st = stream_handle()
             if self.basis == "f32":
-                check(lib.rtx_axpby_mixed(1, n, 1.0 / beta, _vp(r), 0.0, _vp(V[0]), st), "axpby")
+                check(lib.rtx_axpby_mixed(1, n, 1.0 / beta, _vp(rv), 0.0, _vp(V[0]), st), "axpby")
             elif self.basis == "f16":
-                check(lib.rtx_axpby_mixed(2, n, 1.0 / beta, _vp(r), 0.0, _vp(V[0]), st), "axpby")
+                check(lib.rtx_axpby_mixed(2, n, 1.0 / beta, _vp(rv), 0.0, _vp(V[0]), st), "axpby")
             else:
-                check(lib.rtx_axpby(PREC_F64, n, 1.0 / beta, _vp(r), 0.0, _vp(V[0]), st),
+                check(lib.rtx_axpby(PREC_F64, n, 1.0 / beta, _vp(rv), 0.0, _vp(V[0]), st),
                       "axpby")
             check(lib.rtx_gmres_begin(m, beta, _vp(self.state), st), "rtx_gmres_begin")
             gkey = system.graph_key() if hasattr(system, "graph_key") else None
@@ -485,6 +500,9 @@ class GMRES:
             elif stop is not None:
                 reason = -9
                 break
+            if trust_estimate and est_met:
+                reason = 2 if history[-1] <= rtol * bnorm else 3
+                break
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         if k_last > 0:
@@ -492,12 +510,13 @@ class GMRES:
             sv = hessenberg_singular_values(Hd, k_last)
         else:
             sv = [float("nan")] * 2
+        res_f = history[-1] if (trust_estimate and est_met) else true_hist[-1]
         info = GMRESResult(converged_reason=reason, convergence_history=history,
-                           iteration_number=its, res_f=true_hist[-1], res_best=min(true_hist),
+                           iteration_number=its, res_f=res_f, res_best=min(true_hist + [res_f]),
                            true_residual_history=true_hist, rtol=rtol, atol=atol,
                            divtol=float(np.sqrt(np.finfo(np.float64).max)), max_it=max_it,
                            bnorm=bnorm,
-                           rel_residual=true_hist[-1] / bnorm if bnorm > 0 else 0.0,
+                           rel_residual=res_f / bnorm if bnorm > 0 else 0.0,
                            seconds=elapsed, restart=m, extreme_singular_values=sv,
                            graphs=len(self.chunks.graphs), attainable_floor=floor)
         return x, info
@@ -589,7 +608,8 @@ def refine(system_hi, system_lo, b, x=None, rtol=1e-8, inner_rtol=1e-2, restart=
                 reason = -3
                 break
         d.zero_()
-        d, info = inner.solve(r, d, rtol=inner_rtol, max_it=budget)
+        d, info = inner.solve(r, d, rtol=inner_rtol, max_it=budget, bnorm=rn, x_zero=True,
+                              trust_estimate=True)
         inner_its.append(info["iteration_number"])
         check(lib.rtx_axpby(PREC_F64, n, 1.0, _vp(d), 1.0, _vp(x), stream_handle()), "axpby")
     torch.cuda.synchronize()
