@@ -117,11 +117,11 @@ class Problem:
         restart=50, precision="fp64" | "fp32" | "mixed" (FP64 iterative
         refinement around reduced-precision applies: "mixed" runs FP32
         convection, BF16 scattering and FP4 block-Jacobi inverses with
-        inner GMRES(20) unless ``restart`` is given; "fp32" runs FP32
-        throughout), inner_rtol=1e-2 (refinement only)."""
+        inner GMRES(15) on an FP16 basis unless ``restart`` is given; "fp32"
+        runs FP32 throughout), inner_rtol=1e-2 (refinement only)."""
         kw = {"verbose": False, "ksp_type": "gmres", "pc_type": "bjacobi",
               "rtol": consts.EPS, "atol": consts.EPS, "max_it": 5000,
-              "restart": 20 if kwargs.get("precision") == "mixed" else 50,
+              "restart": krylov.INNER_RESTART if kwargs.get("precision") == "mixed" else 50,
               "precision": "fp64", "inner_rtol": 1e-2, **kwargs}
         t_start = time.perf_counter()
         op = self.operator(mesh)
@@ -161,7 +161,8 @@ class Problem:
             lo = krylov.InteriorSystem(op, prec, pc, pc_prec)
             x, info = krylov.refine(hi, lo, rhs, x0, rtol=max(kw["rtol"], 1e-14),
                                     inner_rtol=kw["inner_rtol"], restart=kw["restart"],
-                                    max_inner=kw["max_it"], max_total=kw["max_it"])
+                                    max_inner=kw["max_it"], max_total=kw["max_it"],
+                                    chunk=krylov.INNER_CHUNK if prec == PREC_MIXED else 10)
         u = (x + b_bdry).cpu().numpy()
         t_end = time.perf_counter()
         uh = Projection(mesh)

@@ -41,6 +41,17 @@ from .operator import stream_handle
 # rounding floor of the FP64 apply (about eps |A| |x|).
 ATTAINABLE = 1e-10
 
+# The inner solves of mixed refinement on an FP16 basis: GMRES(15), the host
+# reading the residual estimates every 10 iterations.  Measured to 1e-8
+# (warm solves, FP4 block-Jacobi) against GMRES(20): the uniform absorber of
+# 2,048,000 DOF 0.0156 against 0.0189 s (176 against 206 inner iterations);
+# the two-scatterer hp mesh of 8,263,762 DOF 0.0829 against 0.0998 s (173
+# against 206); the random-field scatterer of 8,192,000 DOF 0.1228 against
+# 0.1233 s at kappa scale 1, but 0.1189 against 0.1090 s (423 against 360)
+# at kappa scale 30.  Longer cycles needed more iterations on the FP16 basis
+# on the first two meshes; GMRES(10) stagnated on the random field.
+INNER_RESTART, INNER_CHUNK = 15, 10
+
 
 def _vp(t):
     return C.c_void_p(t.data_ptr())

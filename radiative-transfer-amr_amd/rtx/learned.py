@@ -40,7 +40,8 @@ from . import _lib, problems
 from ._lib import (check, PREC_F64, PREC_F32, PREC_MIXED, PREC_FP8, PREC_FP4, OP_SYSTEM,
                    OP_RESTRICT)
 from .operator import stream_handle
-from .krylov import BlockJacobi, InteriorSystem, GMRES, gmres, refine, pick_basis, _norm
+from .krylov import (BlockJacobi, InteriorSystem, GMRES, gmres, refine, pick_basis, _norm,
+                     INNER_RESTART, INNER_CHUNK)
 
 
 def _vp(t):
@@ -548,11 +549,6 @@ class Session:
                 self.gated = True
                 bj = pc.bj if bj is None else bj
                 pc = None
-        # GMRES(50) as the reference's KSP (src/rt/solve.py:87) for "fp64";
-        # the inner solves of mixed refinement restart after 20 steps
-        # (shorter Gram-Schmidt, fewer total iterations on these problems)
-        if restart is None:
-            restart = 50 if precision == "fp64" else 20
         if precision not in ("mixed", "fp64"):
             raise ValueError("precision must be 'mixed' or 'fp64'")
         # mixed: FP4 inverses by default (on the uniform absorber, the
@@ -576,10 +572,11 @@ class Session:
                              keep_fp4=precision != "fp64" and bj_precision == PREC_FP4,
                              keep_f64=precision == "fp64")
         self.op, self.pc, self.bj, self.precision = op, pc, bj, precision
-        self.restart = restart
         the_pc = pc if pc is not None else bj
         flexible = pc is not None
         if precision == "fp64":
+            # GMRES(50) as the reference's KSP (src/rt/solve.py:87)
+            restart = 50 if restart is None else restart
             self.system = InteriorSystem(op, PREC_F64, the_pc, PREC_F64)
             self.gm = GMRES(self.system, op.ndof, op.device, restart=restart, flexible=flexible)
         else:
@@ -587,8 +584,14 @@ class Session:
             self.lo = InteriorSystem(op, PREC_MIXED, the_pc, bj_precision)
             if basis == "auto":
                 basis = pick_basis(self.lo, flexible)
+            # inner solves: GMRES(INNER_RESTART) on the FP16 basis, else
+            # GMRES(20) (the flexible method of the learned correction)
+            if restart is None:
+                restart = INNER_RESTART if basis == "f16" else 20
+            chunk = INNER_CHUNK if basis == "f16" else 10
             self.gm = GMRES(self.lo, op.ndof, op.device, restart=restart, flexible=flexible,
-                            basis=basis)
+                            basis=basis, chunk=chunk)
+        self.restart = restart
 
     def solve(self, rhs, x0=None, rtol=1e-8, max_it=5000, inner_rtol=1e-2, max_outer=60,
               judge=True):
