@@ -653,6 +653,27 @@ __device__ inline uint8_t bj_to_e4m3(float v) {
     return sgn | uint8_t((eb << 3) | mq);
 }
 
+// Column scales of one block for the coded copies: scale_j = max_i |B_ij| /
+// div (1 for an all-zero column) into sc[j] and its reciprocal into inv_s[j].
+// One wavefront per column: coalesced loads of the column and a shuffle
+// reduction (the maximum does not depend on the order).
+__device__ inline void bj_col_scales(const float *__restrict__ S, int n, float div,
+                                     float *__restrict__ sc, float *inv_s) {
+    static_assert(BJ_THREADS % 64 == 0, "whole wavefronts");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = wave; j < n; j += BJ_THREADS / 64) {
+        float mx = 0.f;
+        for (int i = lane; i < n; i += 64) mx = fmaxf(mx, fabsf(S[size_t(j) * n + i]));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        if (lane == 0) {
+            const float v = mx > 0.f ? mx / div : 1.f;
+            sc[j] = v;
+            inv_s[j] = 1.f / v;
+        }
+    }
+}
+
 // FP8 copy of block b: column j of codes at off8[b] + j * ldp (ldp = n
 // rounded up to 16, padding zero), scale_j = max_i |B_ij| / 448 at
 // scales[soff[b] + j]; code = e4m3(B_ij / scale_j).
@@ -668,13 +689,7 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_pack_fp8(DevMesh m, const int
     const int n = bj_block_n(m, blk, b), ldp = (n + 15) / 16 * 16;
     const float *S = src + off[b];
     uint8_t *D = dst + off8[b];
-    for (int j = threadIdx.x; j < n; j += BJ_THREADS) {
-        float mx = 0.f;
-        for (int i = 0; i < n; ++i) mx = fmaxf(mx, fabsf(S[size_t(j) * n + i]));
-        const float sc = mx > 0.f ? mx / 448.f : 1.f;
-        scales[soff[b] + j] = sc;
-        inv_s[j] = 1.f / sc;
-    }
+    bj_col_scales(S, n, 448.f, scales + soff[b], inv_s);
     __syncthreads();
     for (int e = threadIdx.x; e < n * ldp; e += BJ_THREADS) {
         const int j = e / ldp, i = e % ldp;
@@ -718,13 +733,7 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_pack_fp4(DevMesh m, const int
     const int n = bj_block_n(m, blk, b), ldp = (n + 15) / 16 * 16, hp = ldp / 2;
     const float *S = src + off[b];
     uint8_t *D = dst + off8[b] / 2;
-    for (int j = threadIdx.x; j < n; j += BJ_THREADS) {
-        float mx = 0.f;
-        for (int i = 0; i < n; ++i) mx = fmaxf(mx, fabsf(S[size_t(j) * n + i]));
-        const float sc = mx > 0.f ? mx / 6.f : 1.f;
-        scales[soff[b] + j] = sc;
-        inv_s[j] = 1.f / sc;
-    }
+    bj_col_scales(S, n, 6.f, scales + soff[b], inv_s);
     __syncthreads();
     for (int e = threadIdx.x; e < n * hp; e += BJ_THREADS) {
         const int j = e / hp, i = 2 * (e % hp);
