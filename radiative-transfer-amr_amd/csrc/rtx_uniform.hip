@@ -924,19 +924,33 @@ __global__ __launch_bounds__(ETHREADS, ETHREADS / 64) void k_uniform_e(DevMesh m
         }
     }
     __syncthreads();
-    // the outputs into the dead column tile in DOF order (cell, node, angle)
+    // the outputs into the dead column tile in DOF order (cell, node, angle),
+    // masked outputs as zero
     if (rok) {
         const int kk = R / NTH, r = R % NTH;
+        const int cb = cbd[kk];
 #pragma unroll
         for (int t = 0; t < MT; ++t)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int s = t * 16 + ((PREC == RTX_PREC_F64) ? (lg + 4 * v) : (4 * lg + v));
-                if (s < NSP) xs[kk * CD + s * NTH + r] = outv[t][v];
+                if (s < NSP) xs[kk * CD + s * NTH + r] = (cb & nfc[s]) ? A(0) : outv[t][v];
             }
     }
     __syncthreads();
-    // ---- coalesced write-back, masked
+    // ---- coalesced write-back: 16-byte copies when y is overwritten in the
+    // tile's type and aligned, else element by element (accumulate adds the
+    // unmasked outputs only)
+    constexpr int VW = 16 / sizeof(T);
+    if constexpr (std::is_same<T, A>::value && NCD % (16 / sizeof(T)) == 0) {
+        if (!accumulate && (reinterpret_cast<uintptr_t>(y + cdof) & 15) == 0) {
+            typedef T V16 __attribute__((ext_vector_type(VW)));
+            const V16 *src = reinterpret_cast<const V16 *>(xs);
+            V16 *dst = reinterpret_cast<V16 *>(y + cdof);
+            for (int e = tid; e < NCD / VW; e += ETHREADS) dst[e] = src[e];
+            return;
+        }
+    }
     for (int e = tid; e < NCD; e += ETHREADS) {
         const int kk = e / CD, lt = e % CD, sn = lt / NTH;
         const bool masked = (cbd[kk] & nfc[sn]) != 0;
