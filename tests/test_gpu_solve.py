@@ -371,6 +371,43 @@ def test_refine_fp32_basis_matches_spsolve(cuda):
     assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < TOL_SOL
 
 
+@pytest.mark.gpu
+def test_inner_solve_shortcuts(cuda):
+    """The inner-solve shortcuts of iterative refinement.  With ``x_zero``
+    and ``bnorm`` (r = b, no operator apply) GMRES gives the iterate, the
+    estimate history and the iteration count of the plain call from x = 0
+    bit for bit.  With ``trust_estimate`` as well it stops after the first
+    cycle whose estimate meets the target: reason 2, ``res_f`` that estimate,
+    no more iterations than the plain call, and the plain call's iterate
+    when both stop after the same iteration."""
+    import torch
+    from rtx.operator import TransportOperator
+    from rtx.krylov import BlockJacobi, InteriorSystem, GMRES, _norm
+    from rtx._lib import PREC_MIXED, PREC_FP4
+    mesh, wl = _c1_small()
+    A, f, b, mask = _oracle_system(mesh, wl)
+    op = TransportOperator(mesh, wl.kappa, wl.sigma, wl.Phi)
+    pc = BlockJacobi(op, keep_f64=False, keep_fp4=True)
+    lo = InteriorSystem(op, PREC_MIXED, pc, PREC_FP4)
+    bb = np.where(mask, 0.0, b)
+    r = _dev(np.where(mask, f - A @ bb, 0.0))
+    gm = GMRES(lo, op.ndof, op.device, restart=6, basis="f16")
+    rn = _norm(lo.lib, r, gm.work, gm.scal)
+    x1, i1 = gm.solve(r, torch.zeros_like(r), rtol=1e-4, max_it=300)
+    assert i1["converged_reason"] > 0 and i1["iteration_number"] > 6    # restarts happen
+    x2, i2 = gm.solve(r, torch.zeros_like(r), rtol=1e-4, max_it=300, bnorm=rn, x_zero=True)
+    assert torch.equal(x1, x2)
+    assert i2["convergence_history"] == i1["convergence_history"]
+    assert i2["iteration_number"] == i1["iteration_number"]
+    x3, i3 = gm.solve(r, torch.zeros_like(r), rtol=1e-4, max_it=300, bnorm=rn, x_zero=True,
+                      trust_estimate=True)
+    assert i3["converged_reason"] == 2
+    assert i3["res_f"] == i3["convergence_history"][-1] <= 1e-4 * rn
+    assert i3["iteration_number"] <= i1["iteration_number"]
+    if i3["iteration_number"] == i1["iteration_number"]:
+        assert torch.equal(x3, x1)
+
+
 def _bf16_round(a):
     """Round float32 values to BF16 (nearest even), returned as float64."""
     u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
