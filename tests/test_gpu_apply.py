@@ -255,9 +255,10 @@ def test_uniform_output_layout_kernel(cuda, n, nref_ang, nref_spt):
     the scattering MFMA leaves it) against the line-pass kernel (one part per
     column) on compiled shapes: the same operations in the same order, so the
     results agree to the last bits (1e-15 relative; equality is reported);
-    both match the oracle at 1e-12; repeated applies are bit-identical;
-    accumulate mode adds exactly.  FP32 and mixed default to the line-pass
-    kernel with one part."""
+    both match the oracle at 1e-12; repeated applies are bit-identical, also
+    into a y that is only 8-byte aligned (element-wise write-back instead of
+    16-byte copies); accumulate mode adds exactly.  FP32 and mixed default to
+    the line-pass kernel with one part."""
     import torch
     from rtx.operator import (TransportOperator, OP_SYSTEM, OP_RESTRICT, OP_ACCUM, PREC_F32,
                               PREC_MIXED)
@@ -273,6 +274,10 @@ def test_uniform_output_layout_kernel(cuda, n, nref_ang, nref_spt):
     ye = [op.apply(xd, ops=ops).clone() for _ in range(3)]
     for yy in ye[1:]:
         assert torch.equal(yy, ye[0])
+    # y only 8-byte aligned: the element-wise write-back, the same values
+    buf = torch.full((op.ndof + 1,), float("nan"), dtype=torch.float64, device="cuda")
+    op.apply(xd, buf[1:], ops=ops)
+    assert torch.equal(buf[1:], ye[0]) and torch.isnan(buf[0])
     op.set_uniform_parts(1)
     assert not op.plan_info()["uniform_output_layout"]
     yl = op.apply(xd, ops=ops)
