@@ -859,32 +859,40 @@ __global__ __launch_bounds__(NT) void k_bj_apply_fp8(DevMesh m, const int64_t *o
 // columns per lane), all its 16-byte code loads and r entries in flight
 // together; the CG partial sums of a row are added in a per-wave LDS tile.
 // A quarter of the threads per block of k_bj_apply_fp8 and no block-wide
-// barrier, so more blocks' loads are in flight per CU.  The grid holds at
-// most seven workgroups per CU; each wavefront applies blocks b, b + W, ...
-// (W wavefronts in the grid), the next block's descriptor loaded while the
-// current block is applied (FP4 on 16,384 blocks of 125: 26.9 against 28.5
-// us with one block per wavefront).
+// barrier, so more blocks' loads are in flight per CU.  LOOP: the grid holds
+// at most seven workgroups per CU and each wavefront applies blocks b,
+// b + W, ... (W wavefronts in the grid), the next block's descriptor loaded
+// while the current block is applied; taken for pools that fit the 256 MB
+// last-level cache (FP4 on 16,384 blocks of 125: 26.9 against 28.5 us).
+// Streaming from HBM, one block per wavefront was faster (FP4 on 65,536
+// blocks of 125: 110 against 120 us).
 constexpr int BJ_W_MAXN = 128;
 constexpr int BJ_W_KMAX = 16;      // columns per lane: ceil(n / CG) <= 16 for n <= 128
 constexpr int BJ_W_WAVES = 4;
-template <typename RT = float, int BITS = 8>
+template <typename RT = float, int BITS = 8, bool LOOP = false>
 __global__ __launch_bounds__(64 * BJ_W_WAVES) void k_bj_apply_fp8_w(
     DevMesh m, const int64_t *off8, const int64_t *soff, const int4 *__restrict__ apd, int nblk,
     const uint8_t *__restrict__ inv, const float *__restrict__ scales,
     const RT *__restrict__ r, float *__restrict__ z) {
     __shared__ float part[BJ_W_WAVES][BJ_W_MAXN * 8];   // CG x ldp <= 1024 per wave
     __shared__ float rsh[BJ_W_WAVES][BJ_W_MAXN];        // r_j s_j of the block
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int nw = gridDim.x * BJ_W_WAVES;
+    const int wave = LOOP ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int nw = LOOP ? gridDim.x * BJ_W_WAVES : nblk;
     int b = blockIdx.x * BJ_W_WAVES + wave;
     if (b >= nblk) return;                               // whole waves; no block barrier below
     int4 dcur = apd[b];
     int64_t o8cur = off8[b], socur = soff[b];
     for (; b < nblk; b += nw) {
         // the next block's descriptor in flight while this one is applied
-        const int bn = b + nw < nblk ? b + nw : b;
-        const int4 dnx = apd[bn];
-        const int64_t o8nx = off8[bn], sonx = soff[bn];
+        int4 dnx = dcur;
+        int64_t o8nx = o8cur, sonx = socur;
+        if constexpr (LOOP) {
+            const int bn = b + nw < nblk ? b + nw : b;
+            dnx = apd[bn];
+            o8nx = off8[bn];
+            sonx = soff[bn];
+        }
         const int4 d = dcur;                  // n, first DOF, cell angles, angles g
         const int n = d.x, nth = d.z, g = d.w;
         const int ldp = (n + 15) / 16 * 16, RG = ldp / 16, CG = 64 / RG;
@@ -936,6 +944,7 @@ __global__ __launch_bounds__(64 * BJ_W_WAVES) void k_bj_apply_fp8_w(
             for (int qq = 0; qq < CG; ++qq) s += pw[qq * ldp + i];
             z[off + (i / g) * nth + i % g] = s;
         }
+        if constexpr (!LOOP) break;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1212,10 +1221,13 @@ static void launch_bj_coded(const rtx_plan *p, const void *d_inv, const RT *r, f
     const float *scales =
         reinterpret_cast<const float *>(codes + (BITS == 4 ? p->bj_codes4 : p->bj_codes8));
     const int4 *apd = reinterpret_cast<const int4 *>(p->bj_apd);
-    if (p->bj_max_block <= BJ_W_MAXN)
-        k_bj_apply_fp8_w<RT, BITS><<<std::min((p->bj_nblk + BJ_W_WAVES - 1) / BJ_W_WAVES,
-                                              std::max(7 * p->n_cu, 1)),
-                                     64 * BJ_W_WAVES, 0, st>>>(
+    const int nwg = (p->bj_nblk + BJ_W_WAVES - 1) / BJ_W_WAVES;
+    if (p->bj_max_block <= BJ_W_MAXN && (BITS == 4 ? p->bj_bytes4 : p->bj_bytes8) <= (int64_t(256) << 20))
+        k_bj_apply_fp8_w<RT, BITS, true><<<std::min(nwg, std::max(7 * p->n_cu, 1)),
+                                           64 * BJ_W_WAVES, 0, st>>>(
+            p->m, p->bj_off8, p->bj_soff, apd, p->bj_nblk, codes, scales, r, z);
+    else if (p->bj_max_block <= BJ_W_MAXN)
+        k_bj_apply_fp8_w<RT, BITS, false><<<nwg, 64 * BJ_W_WAVES, 0, st>>>(
             p->m, p->bj_off8, p->bj_soff, apd, p->bj_nblk, codes, scales, r, z);
     else
         k_bj_apply_fp8<BJ_APPLY8_THREADS, RT, BITS><<<p->bj_nblk, BJ_APPLY8_THREADS, 0, st>>>(
